@@ -1,0 +1,65 @@
+// Shared definitions of the gfx950 kernels and the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include "blaze_gpu.h"
+
+namespace bzg {
+
+// Per-thread text of the last failure, returned by bzg_last_error().
+void set_error(const std::string& msg);
+
+#define BZG_HIP_CHECK(expr)                                                          \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      ::bzg::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));           \
+      return BZG_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+#define BZG_REQUIRE(cond, code, msg)                                                 \
+  do {                                                                               \
+    if (!(cond)) {                                                                   \
+      ::bzg::set_error(msg);                                                         \
+      return (code);                                                                 \
+    }                                                                                \
+  } while (0)
+
+// Launch check: the launch itself (bad configuration) is reported immediately.
+#define BZG_LAUNCH_CHECK() BZG_HIP_CHECK(hipGetLastError())
+
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace bzg
+
+struct bzg_context {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  // Cached scratch for kernels that need it (split GEMM slices, solver staging).
+  bzg::Workspace ws[4];
+};
+
+namespace bzg {
+// Returns a cached device buffer of at least `bytes` (slot-indexed); grows on demand.
+bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out);
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// Wavefront-wide helpers (wave64).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+}  // namespace bzg

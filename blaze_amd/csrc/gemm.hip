@@ -1,0 +1,366 @@
+// Dense GEMM for gfx950:  C = alpha * A * B + beta * C, row-major operands.
+//
+// Replaces the assignment kernels of DMatDMatMultExpr (blaze/math/expressions/
+// DMatDMatMultExpr.h:516-569 dispatch, :864-1887 small kernel, MMM.h:105-600 blocked kernel).
+//
+// FP64 / FP32: MFMA 16x16x4 tiles (v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32).  A workgroup
+// of 4 wavefronts computes a BM x BN tile of C; each wave owns a WM x WN sub-tile held in
+// MFMA accumulators.  A and B K-slabs of depth BK are staged through LDS with two buffers:
+// global loads of slab t+1 are issued into registers before the MFMAs of slab t and written
+// to the idle LDS buffer afterwards, so one barrier per slab suffices.
+// Every C element is produced by one wave in a fixed k order (no split-K, no atomics), so the
+// result is identical run to run and independent of the launch shape chosen for other calls.
+//
+// INT32: no integer MFMA takes 32-bit operands; a VALU kernel with LDS tiles and 4x4 register
+// micro-tiles in wrap-around uint32 arithmetic (bit-exact against any summation order).
+#include "common.hpp"
+
+namespace bzg {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kBK = 16;
+constexpr int kPadA = 1;   // A slab stored [BM][BK+1]: conflict-free fragment reads
+constexpr int kPadB = 16;  // B slab stored [BK][BN+16]
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  using acc_t = d4;
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // Accumulator register i of lane l holds row 4*i + l/16, column l%16 (measured on gfx950).
+  __device__ static int row(int i, int lane) { return 4 * i + lane / 16; }
+};
+
+template <>
+struct Mfma<float> {
+  using acc_t = f4;
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // Accumulator register i of lane l holds row 4*(l/16) + i, column l%16.
+  __device__ static int row(int i, int lane) { return 4 * (lane / 16) + i; }
+};
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+// Maps the linear workgroup id to a (tile_m, tile_n) pair.  Consecutive workgroup ids are
+// dispatched round-robin over the 8 XCDs; the remap gives each XCD a contiguous range of
+// logical tiles, and logical tiles are walked in column-major groups of 8 tile rows so the
+// workgroups resident on one XCD share A row panels and B column panels in that XCD's L2.
+__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
+  const int nt = tiles_m * tiles_n;
+  int bid = blockIdx.x;
+  int logical = bid;
+  if (nt >= 64 && (nt % 8) == 0) logical = (bid % 8) * (nt / 8) + bid / 8;
+  constexpr int G = 8;
+  const int group = logical / (G * tiles_n);
+  const int first_m = group * G;
+  const int gsize = min(tiles_m - first_m, G);
+  const int in_group = logical % (G * tiles_n);
+  tm = first_m + in_group % gsize;
+  tn = in_group / gsize;
+}
+
+template <typename T, int BM, int BN, int WM, int WN, bool VEC>
+__global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(int M, int N, int K, T alpha,
+                                                             const T* __restrict__ A, long lda,
+                                                             const T* __restrict__ B, long ldb,
+                                                             T beta, T* __restrict__ C,
+                                                             long ldc) {
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  using V2 = typename Vec2<T>::type;
+  constexpr int WAVES_N = BN / WN;
+  constexpr int FM = WM / 16;  // MFMA blocks per wave along m
+  constexpr int FN = WN / 16;  // MFMA blocks per wave along n
+  constexpr int LDA_S = kBK + kPadA;
+  constexpr int LDB_S = BN + kPadB;
+  constexpr int NA = BM * kBK / (2 * kThreads);  // 2-element vectors per thread, A slab
+  constexpr int NB = kBK * BN / (2 * kThreads);  // 2-element vectors per thread, B slab
+  constexpr int A_VPR = kBK / 2;                 // vectors per A slab row
+  constexpr int A_RPP = kThreads / A_VPR;        // A rows per pass
+  constexpr int B_VPR = BN / 2;
+  constexpr int B_RPP = kThreads / B_VPR;
+  static_assert(NA >= 1 && NB >= 1, "tile too small for 256 threads");
+  static_assert((BM / WM) * (BN / WN) == 4, "four wavefronts per workgroup");
+
+  __shared__ T As[2][BM * LDA_S];
+  __shared__ T Bs[2][kBK * LDB_S];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave / WAVES_N) * WM;
+  const int wn0 = (wave % WAVES_N) * WN;
+
+  const int tiles_m = (M + BM - 1) / BM;
+  const int tiles_n = (N + BN - 1) / BN;
+  int tm, tn;
+  tile_coords(tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+
+  V2 ra[NA], rb[NB];
+
+  auto load_slab = [&](int k0) {
+    const bool full = (m0 + BM <= M) && (n0 + BN <= N) && (k0 + kBK <= K);
+    if (VEC && full) {
+#pragma unroll
+      for (int i = 0; i < NA; ++i) {
+        const int r = tid / A_VPR + i * A_RPP, c = (tid % A_VPR) * 2;
+        ra[i] = *reinterpret_cast<const V2*>(A + (long)(m0 + r) * lda + k0 + c);
+      }
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int r = tid / B_VPR + i * B_RPP, c = (tid % B_VPR) * 2;
+        rb[i] = *reinterpret_cast<const V2*>(B + (long)(k0 + r) * ldb + n0 + c);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NA; ++i) {
+        const int r = tid / A_VPR + i * A_RPP, c = (tid % A_VPR) * 2;
+        const int gr = m0 + r, gc = k0 + c;
+        const bool rin = gr < M;
+        ra[i].x = (rin && gc < K) ? A[(long)gr * lda + gc] : T(0);
+        ra[i].y = (rin && gc + 1 < K) ? A[(long)gr * lda + gc + 1] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int r = tid / B_VPR + i * B_RPP, c = (tid % B_VPR) * 2;
+        const int gr = k0 + r, gc = n0 + c;
+        const bool rin = gr < K;
+        rb[i].x = (rin && gc < N) ? B[(long)gr * ldb + gc] : T(0);
+        rb[i].y = (rin && gc + 1 < N) ? B[(long)gr * ldb + gc + 1] : T(0);
+      }
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int r = tid / A_VPR + i * A_RPP, c = (tid % A_VPR) * 2;
+      As[buf][r * LDA_S + c] = ra[i].x;
+      As[buf][r * LDA_S + c + 1] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int r = tid / B_VPR + i * B_RPP, c = (tid % B_VPR) * 2;
+      *reinterpret_cast<V2*>(&Bs[buf][r * LDB_S + c]) = rb[i];
+    }
+  };
+
+  const int nk = (K + kBK - 1) / kBK;
+  load_slab(0);
+  store_slab(0);
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) load_slab((kt + 1) * kBK);
+    const T* as = As[buf];
+    const T* bs = Bs[buf];
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 4) {
+      T a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[i] = as[(wm0 + i * 16 + lane % 16) * LDA_S + kk + lane / 16];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[j] = bs[(kk + lane / 16) * LDB_S + wn0 + j * 16 + lane % 16];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::mma(a[i], b[j], acc[i][j]);
+    }
+    if (kt + 1 < nk) store_slab(buf ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue: C = alpha*acc + beta*C; beta == 0 never reads C (it may hold NaN/garbage).
+  const bool read_c = beta != T(0);
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = m0 + wm0 + i * 16 + MF::row(r, lane);
+        const int gc = n0 + wn0 + j * 16 + lane % 16;
+        if (gr < M && gc < N) {
+          T* p = C + (long)gr * ldc + gc;
+          T v = alpha * acc[i][j][r];
+          if (read_c) v += beta * *p;
+          *p = v;
+        }
+      }
+}
+
+// C = beta*C for k == 0 (and the empty product).
+template <typename T>
+__global__ void scale_matrix_kernel(int M, int N, T beta, T* C, long ldc) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)M * N) return;
+  const long r = idx / N, c = idx % N;
+  T* p = C + r * ldc + c;
+  *p = (beta == T(0)) ? T(0) : beta * *p;
+}
+
+// INT32 GEMM on VALU: 64x64 tile, 256 threads, 4x4 micro-tile per thread.
+__global__ __launch_bounds__(256) void igemm_kernel(int M, int N, int K, uint32_t alpha,
+                                                    const int32_t* __restrict__ A, long lda,
+                                                    const int32_t* __restrict__ B, long ldb,
+                                                    uint32_t beta, int32_t* __restrict__ C,
+                                                    long ldc) {
+  constexpr int T = 64, BK = 16;
+  __shared__ uint32_t As[BK][T + 4];  // transposed: As[k][m]
+  __shared__ uint32_t Bs[BK][T + 4];
+  const int tid = threadIdx.x;
+  const int tiles_n = (N + T - 1) / T;
+  const int m0 = (blockIdx.x / tiles_n) * T, n0 = (blockIdx.x % tiles_n) * T;
+  const int tr = (tid / 16) * 4, tc = (tid % 16) * 4;
+  uint32_t acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    for (int e = tid; e < T * BK; e += 256) {
+      const int r = e / BK, c = e % BK;  // A element (m0+r, k0+c)
+      As[c][r] = (m0 + r < M && k0 + c < K) ? (uint32_t)A[(long)(m0 + r) * lda + k0 + c] : 0u;
+      const int kr = e / T, nc = e % T;  // B element (k0+kr, n0+nc)
+      Bs[kr][nc] = (k0 + kr < K && n0 + nc < N) ? (uint32_t)B[(long)(k0 + kr) * ldb + n0 + nc] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BK; ++k) {
+      uint32_t a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[k][tr + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[k][tc + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gr = m0 + tr + i, gc = n0 + tc + j;
+      if (gr < M && gc < N) {
+        int32_t* p = C + (long)gr * ldc + gc;
+        uint32_t v = alpha * acc[i][j];
+        if (beta != 0u) v += beta * (uint32_t)*p;
+        *p = (int32_t)v;
+      }
+    }
+}
+
+template <typename T>
+bzg_status check_gemm_args(bzg_context* ctx, int64_t m, int64_t n, int64_t k, const T* A,
+                           int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemm: null context");
+  BZG_REQUIRE(m >= 0 && n >= 0 && k >= 0, BZG_ERR_INVALID_ARGUMENT, "gemm: negative size");
+  BZG_REQUIRE(m < (1LL << 31) && n < (1LL << 31) && k < (1LL << 31), BZG_ERR_UNSUPPORTED,
+              "gemm: dimension >= 2^31");
+  BZG_REQUIRE(lda >= (k > 0 ? k : 1) && ldb >= (n > 0 ? n : 1) && ldc >= (n > 0 ? n : 1),
+              BZG_ERR_INVALID_ARGUMENT, "gemm: leading dimension smaller than row length");
+  if (m > 0 && n > 0) BZG_REQUIRE(C != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemm: null C");
+  if (m > 0 && n > 0 && k > 0)
+    BZG_REQUIRE(A != nullptr && B != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemm: null A or B");
+  return BZG_OK;
+}
+
+template <typename T, int BM, int BN, int WM, int WN>
+bzg_status launch_tile(bzg_context* ctx, int m, int n, int k, T alpha, const T* A, long lda,
+                       const T* B, long ldb, T beta, T* C, long ldc) {
+  const bool vec = ((uintptr_t)A % (2 * sizeof(T)) == 0) && ((uintptr_t)B % (2 * sizeof(T)) == 0) &&
+                   (lda % 2 == 0) && (ldb % 2 == 0);
+  const long tiles = (long)((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+  BZG_REQUIRE(tiles < (1L << 31), BZG_ERR_UNSUPPORTED, "gemm: too many tiles");
+  dim3 grid((unsigned)tiles), block(kThreads);
+  if (vec)
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, true>), grid, block, 0, ctx->stream,
+                       m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  else
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, false>), grid, block, 0, ctx->stream,
+                       m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+// Tile choice by output size only (never by load), so a product's result never depends on
+// what else is running.  Large tiles cut L2->LDS traffic; small tiles keep >= ~256
+// workgroups in flight for small products.
+template <typename T>
+bzg_status gemm_dispatch(bzg_context* ctx, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
+                         int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
+  bzg_status st = check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+  if (st != BZG_OK) return st;
+  if (m == 0 || n == 0) return BZG_OK;
+  if (k == 0) {
+    const long total = m * n;
+    hipLaunchKernelGGL(scale_matrix_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (int)m, (int)n, beta, C, (long)ldc);
+    BZG_LAUNCH_CHECK();
+    return BZG_OK;
+  }
+  const long t128 = ((m + 127) / 128) * ((n + 127) / 128);
+  const long t64 = ((m + 63) / 64) * ((n + 63) / 64);
+  if (t128 >= 240)
+    return launch_tile<T, 128, 128, 64, 64>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (t64 >= 200)
+    return launch_tile<T, 64, 64, 32, 32>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  return launch_tile<T, 32, 32, 16, 16>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+}  // namespace
+}  // namespace bzg
+
+extern "C" {
+
+bzg_status bzg_dgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, double alpha,
+                     const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                     double* C, int64_t ldc) {
+  return bzg::gemm_dispatch<double>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+bzg_status bzg_sgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, float alpha,
+                     const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                     float* C, int64_t ldc) {
+  return bzg::gemm_dispatch<float>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,
+                     const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
+                     int32_t beta, int32_t* C, int64_t ldc) {
+  bzg_status st = bzg::check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+  if (st != BZG_OK) return st;
+  if (m == 0 || n == 0) return BZG_OK;
+  const long tiles = ((m + 63) / 64) * ((n + 63) / 64);
+  BZG_REQUIRE(tiles < (1L << 31), BZG_ERR_UNSUPPORTED, "igemm: too many tiles");
+  hipLaunchKernelGGL(bzg::igemm_kernel, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, (int)m,
+                     (int)n, (int)k, (uint32_t)alpha, A, (long)lda, B, (long)ldb, (uint32_t)beta, C,
+                     (long)ldc);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+}  // extern "C"

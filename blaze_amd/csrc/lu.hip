@@ -1,0 +1,686 @@
+// Batched dense solves for gfx950: LU with partial pivoting (LAPACK dgetrf/dgetrs semantics)
+// and the mixed-precision variant (FP32 LU + FP64 iterative refinement, LAPACK dsgesv).
+//
+// Replaces solve(A,x,b) / solve(A,X,B) (blaze/math/dense/LSE.h:6343-6373, 6452-6482) whose
+// N >= 7 branch copies A and calls gesv (LSE.h:5229-5257 -> lapack/gesv.h:136-175 ->
+// dgesv_ at lapack/clapack/gesv.h:207).  Here one call solves many independent systems.
+//
+// Execution model: a persistent grid (a few workgroups per CU) pulls system indices from an
+// atomic work counter in longest-first order, so uneven sizes (n = 1..256) balance without a
+// host round trip.  One workgroup (256 threads) owns one system at a time:
+//   factor: blocked right-looking LU, panel width 16.  The 16-column panel lives in registers
+//           (thread r holds panel row r); each column step is an argmax over the workgroup,
+//           a row swap and a rank-1 update (two barriers).  The swaps of the panel are then
+//           applied to the other columns in one gather (each thread follows the 16 swaps
+//           backwards to find the source row of every touched position), the U12 block row
+//           is solved against the unit-lower L11 in registers, and the trailing matrix gets
+//           A22 -= L21 * U12 with MFMA 16x16x4 tiles (operands from LDS, A22 in place in HBM).
+//   solve:  P*B is loaded into LDS; forward/backward substitution by 16-row blocks: the
+//           off-diagonal part is a 16-lane-per-row dot product over the factored rows, the
+//           16x16 diagonal block is solved by one thread per right-hand side.
+// Pivot rule = dgetf2/idamax: the first row of maximal |a| in the column; an exactly-zero pivot
+// sets info (1-based) and the column is left unscaled, as LAPACK does.
+#include "common.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace bzg {
+namespace {
+
+constexpr int NT = 256;       // threads per workgroup
+constexpr int NB = 16;        // panel width
+constexpr int NMAX = 256;     // largest system order
+constexpr int RMAX = 8;       // largest number of right-hand sides
+constexpr int UPAD = 16;      // row padding of the U12 buffer (conflict-free MFMA B reads)
+
+template <typename T>
+struct FactorShm {
+  T panel[NMAX][NB + 1];      // L11 / L21 of the current panel (rows beyond n are zero)
+  T ublk[NB][NMAX + UPAD];    // U12 of the current panel
+};
+
+template <typename T>
+struct SolveShm {
+  T x[NMAX * RMAX];           // right-hand sides / solution, row-major, ld = RMAX
+};
+
+struct Misc {
+  double pv[4];
+  int pr[4];
+  int piv[NB];                // absolute pivot rows of the current panel (0-based)
+  int perm[NMAX];             // row i of the factored system = original row perm[i]
+  int sys;
+  unsigned long long cmax[RMAX * 2];  // column maxima (bit patterns of non-negative doubles)
+  int flag;
+};
+
+template <typename T>
+struct MfmaLU;
+template <>
+struct MfmaLU<double> {
+  using acc_t = d4;
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int i, int lane) { return 4 * i + lane / 16; }
+};
+template <>
+struct MfmaLU<float> {
+  using acc_t = f4;
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int i, int lane) { return 4 * (lane / 16) + i; }
+};
+
+template <typename T>
+__device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
+
+// LU factorisation of the n x n row-major matrix A (leading dimension lda) in place.
+// Writes 1-based pivots to ipiv (may be null) and the row permutation to misc.perm.
+// Returns info (0 or 1-based index of the first exactly-zero pivot).
+template <typename T>
+__device__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict__ ipiv,
+                         FactorShm<T>& sh, Misc& misc) {
+  using MF = MfmaLU<T>;
+  using acc_t = typename MF::acc_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int info = 0;
+  for (int i = tid; i < n; i += NT) misc.perm[i] = i;
+  __syncthreads();
+
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int m = n - j0;
+    const int nbe = min(NB, m);
+    const bool rin = tid < m;
+    T prow[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      prow[c] = (rin && c < nbe) ? A[(long)(j0 + tid) * lda + j0 + c] : T(0);
+
+    // ---- panel factorisation (columns j0 .. j0+nbe-1) ----
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+      if (jj >= nbe) break;
+      // Candidates: rows jj..m-1; others -1.  A NaN entry ranks below every number (as in
+      // idamax, where NaN never compares greater) but above the non-candidates, so the
+      // chosen pivot row is always a row of the matrix.
+      double v = (rin && tid >= jj) ? (double)tabs(prow[jj]) : -1.0;
+      if (v != v) v = -0.5;
+      int r = tid;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int orr = __shfl_xor(r, off, 64);
+        if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; }
+      }
+      if (lane == 0) { misc.pv[wave] = v; misc.pr[wave] = r; }
+      __syncthreads();
+      double bv = misc.pv[0];
+      int p = misc.pr[0];
+#pragma unroll
+      for (int w = 1; w < 4; ++w)
+        if (misc.pv[w] > bv) { bv = misc.pv[w]; p = misc.pr[w]; }
+      T* rowP = &sh.ublk[0][0];          // scratch rows (ublk is free during the panel)
+      T* rowJ = &sh.ublk[1][0];
+      if (tid == p) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) rowP[c] = prow[c];
+      }
+      if (tid == jj && p != jj) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) rowJ[c] = prow[c];
+      }
+      __syncthreads();
+      const T piv = rowP[jj];
+      if (piv != T(0)) {
+        if (tid == 0) {
+          misc.piv[jj] = j0 + p;
+          const int t = misc.perm[j0 + jj];
+          misc.perm[j0 + jj] = misc.perm[j0 + p];
+          misc.perm[j0 + p] = t;
+        }
+        if (p != jj) {
+          if (tid == jj) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) prow[c] = rowP[c];
+          } else if (tid == p) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) prow[c] = rowJ[c];
+          }
+        }
+        if (rin && tid > jj) {
+          const T l = prow[jj] * (T(1) / piv);
+          prow[jj] = l;
+#pragma unroll
+          for (int c = jj + 1; c < NB; ++c) prow[c] -= l * rowP[c];
+        }
+      } else {
+        // Exactly-zero pivot: no interchange, no scaling (dgetf2); the column below is zero,
+        // so the rank-1 update is a no-op.
+        if (tid == 0) misc.piv[jj] = j0 + jj;
+        if (info == 0) info = j0 + jj + 1;
+      }
+      // rowP/rowJ are rewritten only after the next step's first barrier: safe.
+    }
+    __syncthreads();
+    if (ipiv && tid < nbe) ipiv[j0 + tid] = misc.piv[tid] + 1;
+
+    // ---- panel to HBM and LDS ----
+    if (rin) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c < nbe) A[(long)(j0 + tid) * lda + j0 + c] = prow[c];
+        sh.panel[tid][c] = prow[c];
+      }
+    } else if (tid < NMAX) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) sh.panel[tid][c] = T(0);
+    }
+    __syncthreads();
+
+    // ---- interchanges on the other columns, then U12 = L11^-1 A12 ----
+    const int other = n - nbe;
+    if (tid < other) {
+      const int c = tid < j0 ? tid : tid + nbe;
+      // Source of each touched position: follow the panel's swaps backwards.
+      int pv[NB];
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) pv[jj] = (jj < nbe) ? misc.piv[jj] : (j0 + jj);
+      auto src_of = [&](int pos) {
+#pragma unroll
+        for (int jj = NB - 1; jj >= 0; --jj) {
+          if (jj >= nbe) continue;
+          if (pos == j0 + jj) pos = pv[jj];
+          else if (pos == pv[jj]) pos = j0 + jj;
+        }
+        return pos;
+      };
+      T top[NB], low[NB];
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        top[jj] = T(0);
+        low[jj] = T(0);
+        if (jj < nbe) {
+          top[jj] = A[(long)src_of(j0 + jj) * lda + c];
+          if (pv[jj] >= j0 + nbe) low[jj] = A[(long)src_of(pv[jj]) * lda + c];
+        }
+      }
+      if (c > j0) {  // right of the panel: U12 column
+#pragma unroll
+        for (int jj = 1; jj < NB; ++jj)
+#pragma unroll
+          for (int kk = 0; kk < jj; ++kk) top[jj] -= sh.panel[jj][kk] * top[kk];
+        const int uc = c - j0 - NB;
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj) sh.ublk[jj][uc] = top[jj];
+      }
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) {
+        if (jj < nbe) {
+          A[(long)(j0 + jj) * lda + c] = top[jj];
+          if (pv[jj] >= j0 + nbe) A[(long)pv[jj] * lda + c] = low[jj];
+        }
+      }
+    }
+    // zero the unused tail of the U12 buffer (columns beyond n) for the MFMA tiles
+    const int rem = n - j0 - NB;
+    if (rem > 0) {
+      const int padded = (rem + 15) & ~15;
+      for (int e = tid; e < NB * (padded - rem); e += NT) {
+        const int rr = e / (padded - rem), cc = rem + e % (padded - rem);
+        sh.ublk[rr][cc] = T(0);
+      }
+    }
+    __syncthreads();
+
+    // ---- trailing update A22 -= L21 * U12 (MFMA 16x16x4) ----
+    if (rem > 0) {
+      const int tiles = (rem + 15) / 16;
+      const int r0 = j0 + NB;
+      for (int t = wave; t < tiles * tiles; t += 4) {
+        const int ti = t / tiles, tj = t % tiles;
+        acc_t acc;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int gr = r0 + ti * 16 + MF::row(i, lane), gc = r0 + tj * 16 + lane % 16;
+          acc[i] = (gr < n && gc < n) ? A[(long)gr * lda + gc] : T(0);
+        }
+#pragma unroll
+        for (int kk = 0; kk < NB; kk += 4) {
+          const T a = -sh.panel[NB + ti * 16 + lane % 16][kk + lane / 16];
+          const T b = sh.ublk[kk + lane / 16][tj * 16 + lane % 16];
+          acc = MF::mma(a, b, acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int gr = r0 + ti * 16 + MF::row(i, lane), gc = r0 + tj * 16 + lane % 16;
+          if (gr < n && gc < n) A[(long)gr * lda + gc] = acc[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  return info;
+}
+
+// Dot products of 16 consecutive rows of M (row0 .. row0+nrows-1) with the columns of the
+// LDS matrix X (ld RMAX) over k in [kb, ke):  out(r, c) = sum_k M(row0+r, k) * X(k, c).
+// Thread t handles row t/16 and k = kb + t%16 (mod 16); the 16 partials are combined by a
+// fixed butterfly.  Returns the sum in every lane of the row's 16-lane group.
+template <typename TM, typename TX, typename TA>
+__device__ __forceinline__ void rows_dot(const TM* __restrict__ M, int lda, int row0, int nrows,
+                                         int kb, int ke, const TX* __restrict__ X, int nrhs,
+                                         TA (&acc)[RMAX]) {
+  const int tid = threadIdx.x;
+  const int r = tid / 16, ks = tid % 16;
+#pragma unroll
+  for (int c = 0; c < RMAX; ++c) acc[c] = TA(0);
+  if (r < nrows) {
+    const TM* row = M + (long)(row0 + r) * lda;
+    for (int k = kb + ks; k < ke; k += 16) {
+      const TA a = (TA)row[k];
+#pragma unroll
+      for (int c = 0; c < RMAX; ++c)
+        if (c < nrhs) acc[c] += a * (TA)X[k * RMAX + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < RMAX; ++c) {
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+  }
+}
+
+// Solves (L U) X = X in place for the factored n x n matrix LU (row-major, unit-lower L),
+// X in LDS (n x nrhs, ld RMAX), already permuted (X = P * B).
+template <typename T>
+__device__ void lu_solve(const T* __restrict__ LU, int n, int lda, T* X, int nrhs) {
+  const int tid = threadIdx.x;
+  const int r = tid / 16, ks = tid % 16;
+  // forward: L y = b
+  for (int b0 = 0; b0 < n; b0 += 16) {
+    const int nr = min(16, n - b0);
+    if (b0 > 0) {
+      T acc[RMAX];
+      rows_dot<T, T, T>(LU, lda, b0, nr, 0, b0, X, nrhs, acc);
+      if (r < nr && ks == 0) {
+#pragma unroll
+        for (int c = 0; c < RMAX; ++c)
+          if (c < nrhs) X[(b0 + r) * RMAX + c] -= acc[c];
+      }
+      __syncthreads();
+    }
+    if (tid < nrhs) {
+      const int c = tid;
+      for (int i = 1; i < nr; ++i) {
+        const T* row = LU + (long)(b0 + i) * lda + b0;
+        T s = X[(b0 + i) * RMAX + c];
+        for (int k = 0; k < i; ++k) s -= row[k] * X[(b0 + k) * RMAX + c];
+        X[(b0 + i) * RMAX + c] = s;
+      }
+    }
+    __syncthreads();
+  }
+  // backward: U x = y
+  const int last = ((n - 1) / 16) * 16;
+  for (int b0 = last; b0 >= 0; b0 -= 16) {
+    const int nr = min(16, n - b0);
+    if (b0 + 16 < n) {
+      T acc[RMAX];
+      rows_dot<T, T, T>(LU, lda, b0, nr, b0 + 16, n, X, nrhs, acc);
+      if (r < nr && ks == 0) {
+#pragma unroll
+        for (int c = 0; c < RMAX; ++c)
+          if (c < nrhs) X[(b0 + r) * RMAX + c] -= acc[c];
+      }
+      __syncthreads();
+    }
+    if (tid < nrhs) {
+      const int c = tid;
+      for (int i = nr - 1; i >= 0; --i) {
+        const T* row = LU + (long)(b0 + i) * lda + b0;
+        T s = X[(b0 + i) * RMAX + c];
+        for (int k = i + 1; k < nr; ++k) s -= row[k] * X[(b0 + k) * RMAX + c];
+        X[(b0 + i) * RMAX + c] = s / row[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int next_system(int* counter, Misc& misc) {
+  if (threadIdx.x == 0) misc.sys = atomicAdd(counter, 1);
+  __syncthreads();
+  const int s = misc.sys;
+  __syncthreads();
+  return s;
+}
+
+// ---- plain FP64 (or FP32) dgesv: A and B in place --------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32_t* __restrict__ nv,
+                                                          const int32_t* __restrict__ rv,
+                                                          const int64_t* __restrict__ a_off,
+                                                          const int64_t* __restrict__ b_off,
+                                                          const int64_t* __restrict__ p_off,
+                                                          T* __restrict__ A, T* __restrict__ B,
+                                                          int32_t* __restrict__ ipiv,
+                                                          int32_t* __restrict__ info,
+                                                          const int32_t* __restrict__ order,
+                                                          int* counter) {
+  __shared__ union {
+    FactorShm<T> f;
+    SolveShm<T> s;
+  } sh;
+  __shared__ Misc misc;
+  const int tid = threadIdx.x;
+  for (;;) {
+    const int idx = next_system(counter, misc);
+    if (idx >= count) break;
+    const int s = order ? order[idx] : idx;
+    const int n = nv[s], nrhs = rv[s];
+    if (n < 1 || n > NMAX || nrhs < 0 || nrhs > RMAX) {  // outside the supported range
+      if (tid == 0) info[s] = -1;
+      continue;
+    }
+    T* a = A + a_off[s];
+    T* b = B + b_off[s];
+    const int inf = lu_factor<T>(a, n, n, ipiv ? ipiv + p_off[s] : nullptr, sh.f, misc);
+    if (tid == 0) info[s] = inf;
+    if (inf == 0) {
+      for (int e = tid; e < n * RMAX; e += NT) {
+        const int i = e / RMAX, c = e % RMAX;
+        sh.s.x[e] = (c < nrhs) ? b[(long)misc.perm[i] * nrhs + c] : T(0);
+      }
+      __syncthreads();
+      lu_solve<T>(a, n, n, sh.s.x, nrhs);
+      for (int e = tid; e < n * nrhs; e += NT) b[e] = sh.s.x[(e / nrhs) * RMAX + e % nrhs];
+    }
+    __syncthreads();
+  }
+}
+
+// ---- mixed precision: FP32 LU + FP64 refinement (dsgesv) --------------------------------
+struct IRShm {
+  double x[NMAX * RMAX];
+  double r[NMAX * RMAX];
+  float d[NMAX * RMAX];
+};
+
+__device__ __forceinline__ void col_max_reset(Misc& misc) {
+  if (threadIdx.x < 2 * RMAX) misc.cmax[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
+    int count, const int32_t* __restrict__ nv, const int32_t* __restrict__ rv,
+    const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
+    const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ X,
+    int32_t* __restrict__ iter, int32_t* __restrict__ info, const int32_t* __restrict__ order,
+    int* counter, float* __restrict__ ws32, int32_t* __restrict__ fallback_list,
+    int* fallback_count, double tol, int max_iter) {
+  __shared__ union {
+    FactorShm<float> f;
+    IRShm ir;
+  } sh;
+  __shared__ Misc misc;
+  const int tid = threadIdx.x;
+  float* W = ws32 + (long)blockIdx.x * NMAX * NMAX;
+  for (;;) {
+    const int idx = next_system(counter, misc);
+    if (idx >= count) break;
+    const int s = order ? order[idx] : idx;
+    const int n = nv[s], nrhs = rv[s];
+    if (n < 1 || n > NMAX || nrhs < 0 || nrhs > RMAX) {
+      if (tid == 0) { info[s] = -1; iter[s] = 0; }
+      continue;
+    }
+    const double* a = A + a_off[s];
+    const double* b = B + b_off[s];
+    double* x = X + b_off[s];
+    // FP32 copy of A (ld n) and ||A||_inf
+    if (tid < 2 * RMAX) misc.cmax[tid] = 0ull;
+    __syncthreads();
+    {
+      const int r = tid / 16, ks = tid % 16;
+      for (int rb = 0; rb < n; rb += 16) {
+        double rs = 0.0;
+        if (rb + r < n)
+          for (int k = ks; k < n; k += 16) {
+            const double v = a[(long)(rb + r) * n + k];
+            W[(long)(rb + r) * n + k] = (float)v;
+            rs += fabs(v);
+          }
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) rs += __shfl_xor(rs, off, 64);
+        if (ks == 0 && rb + r < n) atomicMax(&misc.cmax[2 * RMAX - 1], (unsigned long long)__double_as_longlong(rs));
+      }
+    }
+    __syncthreads();
+    const double anrm = __longlong_as_double((long long)misc.cmax[2 * RMAX - 1]);
+    const double eps = tol > 0 ? tol : 0x1.0p-53;
+    const double cte = anrm * eps * sqrt((double)n);
+    __syncthreads();
+    const int inf32 = lu_factor<float>(W, n, n, nullptr, sh.f, misc);
+    int it_out = -2;
+    if (inf32 == 0) {
+      // x = FP32 solve of P b
+      for (int e = tid; e < n * RMAX; e += NT) {
+        const int i = e / RMAX, c = e % RMAX;
+        sh.ir.d[e] = (c < nrhs) ? (float)b[(long)misc.perm[i] * nrhs + c] : 0.0f;
+      }
+      __syncthreads();
+      lu_solve<float>(W, n, n, sh.ir.d, nrhs);
+      for (int e = tid; e < n * RMAX; e += NT) sh.ir.x[e] = (double)sh.ir.d[e];
+      __syncthreads();
+      double prev = 1e308;
+      int stalls = 0;
+      it_out = -3;
+      for (int it = 0; it <= max_iter; ++it) {
+        // r = b - A x (FP64), column maxima of |r| and |x|
+        if (tid < 2 * RMAX - 1) misc.cmax[tid] = 0ull;
+        __syncthreads();
+        for (int rb = 0; rb < n; rb += 16) {
+          const int nr = min(16, n - rb);
+          double acc[RMAX];
+          rows_dot<double, double, double>(a, n, rb, nr, 0, n, sh.ir.x, nrhs, acc);
+          const int r = tid / 16, ks = tid % 16;
+          if (r < nr && ks < nrhs) {
+            // lane ks of the row group writes column ks (all lanes hold every sum)
+            double v = acc[0];
+#pragma unroll
+            for (int c = 1; c < RMAX; ++c)
+              if (c == ks) v = acc[c];
+            const double rv2 = b[(long)(rb + r) * nrhs + ks] - v;
+            sh.ir.r[(rb + r) * RMAX + ks] = rv2;
+            atomicMax(&misc.cmax[ks], (unsigned long long)__double_as_longlong(fabs(rv2)));
+            atomicMax(&misc.cmax[RMAX + ks],
+                      (unsigned long long)__double_as_longlong(fabs(sh.ir.x[(rb + r) * RMAX + ks])));
+          }
+        }
+        __syncthreads();
+        bool ok = true;
+        double worst = 0.0;
+        for (int c = 0; c < nrhs; ++c) {
+          const double rn = __longlong_as_double((long long)misc.cmax[c]);
+          const double xn = __longlong_as_double((long long)misc.cmax[RMAX + c]);
+          if (!(rn <= xn * cte)) ok = false;
+          worst = fmax(worst, rn);
+        }
+        if (ok) { it_out = it; break; }
+        if (it == max_iter) break;
+        // stagnation: the residual must at least halve every other step
+        if (!(worst < 0.5 * prev)) {
+          if (++stalls >= 2) break;
+        } else {
+          stalls = 0;
+        }
+        prev = worst;
+        // d = FP32 solve of P r; x += d
+        for (int e = tid; e < n * RMAX; e += NT) {
+          const int i = e / RMAX, c = e % RMAX;
+          sh.ir.d[e] = (c < nrhs) ? (float)sh.ir.r[misc.perm[i] * RMAX + c] : 0.0f;
+        }
+        __syncthreads();
+        lu_solve<float>(W, n, n, sh.ir.d, nrhs);
+        for (int e = tid; e < n * RMAX; e += NT) sh.ir.x[e] += (double)sh.ir.d[e];
+        __syncthreads();
+      }
+      if (it_out >= 0)
+        for (int e = tid; e < n * nrhs; e += NT) x[e] = sh.ir.x[(e / nrhs) * RMAX + e % nrhs];
+    }
+    if (tid == 0) {
+      iter[s] = it_out;
+      info[s] = 0;
+      if (it_out < 0) fallback_list[atomicAdd(fallback_count, 1)] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// FP64 fallback for the systems listed by the mixed-precision kernel: copy A to the
+// workgroup's FP64 scratch, factor, solve B into X.
+__global__ __launch_bounds__(NT) void gesv_fallback_kernel(
+    const int32_t* __restrict__ nv, const int32_t* __restrict__ rv,
+    const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
+    const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ X,
+    int32_t* __restrict__ info, const int32_t* __restrict__ list, const int* list_count,
+    int* counter, double* __restrict__ ws64) {
+  __shared__ union {
+    FactorShm<double> f;
+    SolveShm<double> s;
+  } sh;
+  __shared__ Misc misc;
+  const int tid = threadIdx.x;
+  double* W = ws64 + (long)blockIdx.x * NMAX * NMAX;
+  const int count = *list_count;
+  for (;;) {
+    const int idx = next_system(counter, misc);
+    if (idx >= count) break;
+    const int s = list[idx];
+    const int n = nv[s], nrhs = rv[s];
+    const double* a = A + a_off[s];
+    for (int e = tid; e < n * n; e += NT) W[e] = a[e];
+    __syncthreads();
+    const int inf = lu_factor<double>(W, n, n, nullptr, sh.f, misc);
+    if (tid == 0) info[s] = inf;
+    if (inf == 0) {
+      const double* b = B + b_off[s];
+      for (int e = tid; e < n * RMAX; e += NT) {
+        const int i = e / RMAX, c = e % RMAX;
+        sh.s.x[e] = (c < nrhs) ? b[(long)misc.perm[i] * nrhs + c] : 0.0;
+      }
+      __syncthreads();
+      lu_solve<double>(W, n, n, sh.s.x, nrhs);
+      double* x = X + b_off[s];
+      for (int e = tid; e < n * nrhs; e += NT) x[e] = sh.s.x[(e / nrhs) * RMAX + e % nrhs];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void reset_counters_kernel(int* c, int n) {
+  if (threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+// Occupancy of the persistent grids (workgroups per CU); chosen by the autotuner.
+struct SolverTuning {
+  int wg_per_cu_fp64 = 2;
+  int wg_per_cu_mixed = 2;
+};
+static SolverTuning g_tuning;
+
+}  // namespace bzg
+
+extern "C" {
+
+bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_host,
+                             int32_t* order_dev) {
+  BZG_REQUIRE(ctx && (count == 0 || (n_host && order_dev)), BZG_ERR_INVALID_ARGUMENT,
+              "batched_order: null argument");
+  std::vector<int32_t> ord(count);
+  // counting sort by descending n (stable): longest processing time first
+  std::vector<int64_t> cnt(bzg::NMAX + 2, 0);
+  for (int64_t i = 0; i < count; ++i) {
+    const int v = std::min(std::max(n_host[i], 0), bzg::NMAX + 1);
+    cnt[v]++;
+  }
+  std::vector<int64_t> start(bzg::NMAX + 2, 0);
+  int64_t acc = 0;
+  for (int v = bzg::NMAX + 1; v >= 0; --v) { start[v] = acc; acc += cnt[v]; }
+  for (int64_t i = 0; i < count; ++i) {
+    const int v = std::min(std::max(n_host[i], 0), bzg::NMAX + 1);
+    ord[start[v]++] = (int32_t)i;
+  }
+  if (count) {
+    BZG_HIP_CHECK(hipMemcpyAsync(order_dev, ord.data(), count * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  return BZG_OK;
+}
+
+bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,
+                             const int64_t* a_off, const int64_t* b_off, const int64_t* p_off,
+                             double* A, double* B, int32_t* ipiv, int32_t* info,
+                             const int32_t* order) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dgesv_batched: null context");
+  BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "dgesv_batched: bad count");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(n && nrhs && a_off && b_off && A && B && info, BZG_ERR_INVALID_ARGUMENT,
+              "dgesv_batched: null array");
+  BZG_REQUIRE(ipiv == nullptr || p_off != nullptr, BZG_ERR_INVALID_ARGUMENT,
+              "dgesv_batched: ipiv without p_off");
+  // Scratch: [0] work counter; [1..] (unused)
+  void* scratch = nullptr;
+  bzg_status st = bzg::workspace(ctx, 0, 256, &scratch);
+  if (st) return st;
+  int* counter = static_cast<int*>(scratch);
+  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counter, 4);
+  BZG_LAUNCH_CHECK();
+  const int grid = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
+  hipLaunchKernelGGL(bzg::gesv_batched_kernel<double>, dim3(grid), dim3(bzg::NT), 0, ctx->stream,
+                     (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order, counter);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
+                              const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
+                              const double* A, const double* B, double* X, int32_t* iter,
+                              int32_t* info, double tol, int32_t max_iter, const int32_t* order) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dsgesv_batched: null context");
+  BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "dsgesv_batched: bad count");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(n && nrhs && a_off && b_off && A && B && X && iter && info, BZG_ERR_INVALID_ARGUMENT,
+              "dsgesv_batched: null array");
+  BZG_REQUIRE(max_iter >= 0 && max_iter <= 1000, BZG_ERR_INVALID_ARGUMENT, "dsgesv_batched: bad max_iter");
+  const int grid32 = ctx->num_cus * bzg::g_tuning.wg_per_cu_mixed;
+  const int grid64 = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
+  // scratch layout (slot 0): counters; slot 1: FP32 per-workgroup matrices; slot 2: FP64
+  // per-workgroup matrices; slot 3: fallback list
+  void *cnt = nullptr, *w32 = nullptr, *w64 = nullptr, *lst = nullptr;
+  bzg_status st = bzg::workspace(ctx, 0, 256, &cnt);
+  if (!st) st = bzg::workspace(ctx, 1, (size_t)grid32 * bzg::NMAX * bzg::NMAX * sizeof(float), &w32);
+  if (!st) st = bzg::workspace(ctx, 2, (size_t)grid64 * bzg::NMAX * bzg::NMAX * sizeof(double), &w64);
+  if (!st) st = bzg::workspace(ctx, 3, (size_t)count * sizeof(int32_t), &lst);
+  if (st) return st;
+  int* counters = static_cast<int*>(cnt);
+  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 4);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bzg::dsgesv_batched_kernel, dim3(grid32), dim3(bzg::NT), 0, ctx->stream,
+                     (int)count, n, nrhs, a_off, b_off, A, B, X, iter, info,
+                     order, counters + 0, static_cast<float*>(w32),
+                     static_cast<int32_t*>(lst), counters + 1, tol, max_iter);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bzg::gesv_fallback_kernel, dim3(grid64), dim3(bzg::NT), 0, ctx->stream, n, nrhs,
+                     a_off, b_off, A, B, X, info, static_cast<const int32_t*>(lst), counters + 1,
+                     counters + 2, static_cast<double*>(w64));
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+"""Batched solve helpers over the C ABI (packing of uneven-size systems into one pool).
+
+Layout (include/blaze_gpu.h): A_i row-major n_i x n_i at A + a_off[i]; B_i/X_i row-major
+n_i x nrhs_i at B + b_off[i]; ipiv_i at ipiv + p_off[i].  Device memory comes from torch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+import blaze_amd as bz
+
+
+def pack(As, Bs):
+    n = np.array([a.shape[0] for a in As], np.int32)
+    r = np.array([b.shape[1] for b in Bs], np.int32)
+    a_off = np.zeros(len(As), np.int64)
+    b_off = np.zeros(len(As), np.int64)
+    if len(As):
+        a_off[1:] = np.cumsum((n.astype(np.int64) ** 2))[:-1]
+        b_off[1:] = np.cumsum(n.astype(np.int64) * r)[:-1]
+    Ap = np.concatenate([np.ascontiguousarray(a, np.float64).ravel() for a in As]) if As else np.zeros(1)
+    Bp = np.concatenate([np.ascontiguousarray(b, np.float64).ravel() for b in Bs]) if Bs else np.zeros(1)
+    return n, r, a_off, b_off, Ap, Bp
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def order_longest_first(ctx: bz.Context, n: np.ndarray):
+    import torch
+    order = torch.empty(max(len(n), 1), dtype=torch.int32, device="cuda")
+    ctx.call("bzg_batched_order", len(n), n.ctypes.data_as(C.c_void_p), _p(order))
+    return order
+
+
+def gesv_batched(ctx: bz.Context, As, Bs, with_ipiv: bool = False, lpt: bool = True):
+    """Plain FP64 batched LU solve. Returns (Xs, infos[, LUs, ipivs])."""
+    import torch
+    n, r, a_off, b_off, Ap, Bp = pack(As, Bs)
+    p_off = np.zeros(len(As), np.int64)
+    if len(As):
+        p_off[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    dn, dr, da, db, dp = (_dev(torch, x) for x in (n, r, a_off, b_off, p_off))
+    dA, dB = _dev(torch, Ap), _dev(torch, Bp)
+    dpiv = torch.zeros(max(int(n.sum()), 1), dtype=torch.int32, device="cuda")
+    dinfo = torch.full((max(len(As), 1),), -99, dtype=torch.int32, device="cuda")
+    order = order_longest_first(ctx, n) if lpt else None
+    ctx.call("bzg_dgesv_batched", len(As), _p(dn), _p(dr), _p(da), _p(db), _p(dp), _p(dA), _p(dB),
+             _p(dpiv), _p(dinfo), _p(order) if order is not None else C.c_void_p(0))
+    ctx.sync()
+    Xh, Ah, ph, infoh = dB.cpu().numpy(), dA.cpu().numpy(), dpiv.cpu().numpy(), dinfo.cpu().numpy()
+    Xs = [Xh[b_off[i]:b_off[i] + n[i] * r[i]].reshape(n[i], r[i]) for i in range(len(As))]
+    if not with_ipiv:
+        return Xs, infoh[:len(As)]
+    LUs = [Ah[a_off[i]:a_off[i] + n[i] * n[i]].reshape(n[i], n[i]) for i in range(len(As))]
+    ipivs = [ph[p_off[i]:p_off[i] + n[i]] for i in range(len(As))]
+    return Xs, infoh[:len(As)], LUs, ipivs
+
+
+def dsgesv_batched(ctx: bz.Context, As, Bs, tol: float = 0.0, max_iter: int = 30, lpt: bool = True):
+    """Mixed-precision batched solve. Returns (Xs, iters, infos)."""
+    import torch
+    n, r, a_off, b_off, Ap, Bp = pack(As, Bs)
+    dn, dr, da, db = (_dev(torch, x) for x in (n, r, a_off, b_off))
+    dA, dB = _dev(torch, Ap), _dev(torch, Bp)
+    dX = torch.full_like(dB, float("nan"))
+    dit = torch.full((max(len(As), 1),), -99, dtype=torch.int32, device="cuda")
+    dinfo = torch.full((max(len(As), 1),), -99, dtype=torch.int32, device="cuda")
+    order = order_longest_first(ctx, n) if lpt else None
+    ctx.call("bzg_dsgesv_batched", len(As), _p(dn), _p(dr), _p(da), _p(db), _p(dA), _p(dB), _p(dX),
+             _p(dit), _p(dinfo), tol, max_iter, _p(order) if order is not None else C.c_void_p(0))
+    ctx.sync()
+    Xh = dX.cpu().numpy()
+    Xs = [Xh[b_off[i]:b_off[i] + n[i] * r[i]].reshape(n[i], r[i]) for i in range(len(As))]
+    return Xs, dit.cpu().numpy()[:len(As)], dinfo.cpu().numpy()[:len(As)]

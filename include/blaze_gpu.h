@@ -1,0 +1,170 @@
+/*
+ * blaze_gpu.h -- C ABI of the MI355X (gfx950) multiplication & solve library.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, throws nothing and returns
+ * a bzg_status.  Matrix/vector pointers passed to compute calls are DEVICE pointers (HBM);
+ * dense matrices are row-major with an explicit leading dimension (Blaze's `spacing()`).
+ * Work is enqueued on the context's HIP stream and is asynchronous unless stated otherwise.
+ *
+ * Status convention (mirrors LAPACK `info`):
+ *   0      success
+ *   > 0    numerical failure: 1-based index of the first exactly-zero pivot (singular matrix)
+ *   < 0    invalid argument / runtime error (see bzg_status_string)
+ *
+ * Reference interfaces replaced (paths relative to the Blaze source tree):
+ *   bzg_dgemm / bzg_sgemm / bzg_igemm  <- DMatDMatMultExpr assign (blaze/math/expressions/
+ *        DMatDMatMultExpr.h:516-569) and its BLAS seam cblas_dgemm (blaze/math/blas/cblas/gemm.h:161)
+ *   bzg_dgemv / bzg_sgemv / bzg_igemv  <- DMatDVecMultExpr assign (DMatDVecMultExpr.h:408-458)
+ *        and cblas_dgemv (blaze/math/blas/cblas/gemv.h:153)
+ *   bzg_dcsrmv / bzg_scsrmv / bzg_icsrmv <- SMatDVecMultExpr element evaluation
+ *        (SMatDVecMultExpr.h:204-207 -> SVecDVecInnerExpr.h:127-135)
+ *   bzg_dgesv_batched                  <- solve() (blaze/math/dense/LSE.h:6343-6373) ->
+ *        gesv (blaze/math/lapack/gesv.h:136-175) -> dgesv_ (blaze/math/lapack/clapack/gesv.h:207)
+ *   bzg_dsgesv_batched                 <- same call site; LAPACK dsgesv semantics (mixed precision)
+ *   bzg_dgemm_fp64acc                  <- no reference function: FP64-accurate GEMM from BF16
+ *        residue slices; drop-in for bzg_dgemm at the same call site (DMatDMatMultExpr.h:516)
+ */
+#ifndef BLAZE_GPU_H
+#define BLAZE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t bzg_status;
+
+#define BZG_OK 0
+#define BZG_ERR_INVALID_ARGUMENT (-1) /* bad size/pointer/leading dimension */
+#define BZG_ERR_SIZE_MISMATCH (-2)    /* operand shapes do not conform */
+#define BZG_ERR_HIP (-3)              /* a HIP runtime call failed */
+#define BZG_ERR_OUT_OF_MEMORY (-4)    /* device allocation failed or budget exceeded */
+#define BZG_ERR_NOT_SQUARE (-5)       /* solve(): non-square system matrix */
+#define BZG_ERR_UNSUPPORTED (-6)      /* shape/option outside the implemented range */
+#define BZG_ERR_NO_DEVICE (-7)        /* no gfx950 device visible */
+#define BZG_ERR_BUSY (-8)             /* server: queue full / not admitted yet */
+#define BZG_ERR_CANCELLED (-9)        /* server: request cancelled */
+
+typedef struct bzg_context bzg_context;
+
+/* ---- context, stream, memory ------------------------------------------------------------ */
+
+/* Creates a context on the current HIP device with its own (default-priority) stream. */
+bzg_status bzg_create(bzg_context** ctx);
+bzg_status bzg_destroy(bzg_context* ctx);
+/* Enqueue all later work of this context on the given hipStream_t (passed as void*);
+ * NULL selects the device's default (null) stream.  The context's own stream is
+ * bzg_own_stream(ctx). */
+bzg_status bzg_set_stream(bzg_context* ctx, void* hip_stream);
+void* bzg_get_stream(bzg_context* ctx);
+void* bzg_own_stream(bzg_context* ctx);
+/* Blocks until all work enqueued on the context's stream has finished. */
+bzg_status bzg_synchronize(bzg_context* ctx);
+const char* bzg_status_string(bzg_status s);
+/* Text of the last error detected on this thread (HIP error name, failing argument ...). */
+const char* bzg_last_error(void);
+
+/* Device memory (hipMalloc/hipFree) for hosts that do not include HIP headers. */
+bzg_status bzg_malloc(void** dptr, size_t bytes);
+bzg_status bzg_free(void* dptr);
+/* Synchronous copies on the context stream (kind: 0 = H2D, 1 = D2H, 2 = D2D). */
+bzg_status bzg_memcpy(bzg_context* ctx, void* dst, const void* src, size_t bytes, int kind);
+bzg_status bzg_memset_zero(bzg_context* ctx, void* dptr, size_t bytes);
+
+/* ---- dense GEMM:  C = alpha * A * B + beta * C  (row-major, device pointers) --------------
+ * A is m x k (lda >= k), B is k x n (ldb >= n), C is m x n (ldc >= n).  beta == 0 never
+ * reads C.  k == 0 sets C = beta*C (Blaze resets C: DMatDMatMultExpr.h:527-530). */
+bzg_status bzg_dgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, double alpha,
+                     const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                     double* C, int64_t ldc);
+bzg_status bzg_sgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, float alpha,
+                     const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                     float* C, int64_t ldc);
+/* int32 GEMM: wrap-around (mod 2^32) arithmetic, bit-exact against any summation order. */
+bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,
+                     const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
+                     int32_t beta, int32_t* C, int64_t ldc);
+
+/* ---- dense GEMV:  y = alpha * A * x + beta * y  (row-major A, device pointers) ---------- */
+bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,
+                     int64_t lda, const double* x, double beta, double* y);
+bzg_status bzg_sgemv(bzg_context* ctx, int64_t m, int64_t n, float alpha, const float* A,
+                     int64_t lda, const float* x, float beta, float* y);
+bzg_status bzg_igemv(bzg_context* ctx, int64_t m, int64_t n, int32_t alpha, const int32_t* A,
+                     int64_t lda, const int32_t* x, int32_t beta, int32_t* y);
+
+/* ---- CSR SpMV:  y = A * x  (row-major CompressedMatrix packed as CSR) --------------------
+ * rowptr has n_rows+1 int32 entries (rowptr[0] == 0, nnz = rowptr[n_rows] < 2^31); colidx
+ * int32 strictly increasing per row.  Every row's sum runs sequentially in storage order
+ * (the reference order, SVecDVecInnerExpr.h:127-135) or as a fixed-shape tree; results are
+ * deterministic run to run.
+ *
+ * A plan analyses the row-length histogram once (host side, O(n_rows)) and picks the kernel:
+ * short rows -> several rows per wavefront (vector-CSR with 2..64 lanes per row), long rows
+ * -> one wavefront per row; rows are grouped in nnz-balanced blocks.  The plan owns device
+ * metadata; the matrix arrays stay owned by the caller. */
+typedef struct bzg_csr_plan bzg_csr_plan;
+bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
+                               const int32_t* rowptr_host, bzg_csr_plan** plan);
+bzg_status bzg_csr_plan_destroy(bzg_csr_plan* plan);
+/* Describes the plan: writes a short text (kernel choice, lanes per row, blocks) to buf. */
+bzg_status bzg_csr_plan_describe(const bzg_csr_plan* plan, char* buf, size_t len);
+bzg_status bzg_dcsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,
+                      const int32_t* colidx, const double* val, const double* x, double* y);
+bzg_status bzg_scsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,
+                      const int32_t* colidx, const float* val, const float* x, float* y);
+bzg_status bzg_icsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,
+                      const int32_t* colidx, const int32_t* val, const int32_t* x, int32_t* y);
+
+/* ---- batched dense solves (LAPACK dgesv / dsgesv semantics per system) ------------------
+ * `count` independent systems A_i X_i = B_i of order n[i] with nrhs[i] right-hand sides.
+ * All per-system arrays are DEVICE arrays.  A_i is stored row-major n x n at
+ * A + a_off[i] (leading dimension n[i]); B_i / X_i are stored row-major n x nrhs at
+ * B + b_off[i] (leading dimension nrhs[i]).  ipiv_i (1-based, LAPACK convention) at
+ * ipiv + p_off[i] (n[i] entries).  info[i] = 0 or the 1-based index of an exactly-zero
+ * pivot.  Row-major storage means the factorisation is of A itself (not of A^T), with
+ * partial pivoting by rows exactly as dgetf2: the first row of maximal |a|.
+ * 1 <= n[i] <= 256, 0 <= nrhs[i] <= 8; a system outside that range gets info[i] = -1.
+ * `order` (device, may be NULL) lists the systems in the order the persistent workgroups
+ * should take them -- bzg_batched_order() gives longest-first; NULL = index order.
+ * ipiv may be NULL (then p_off is ignored). */
+bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
+                             const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
+                             const int64_t* p_off, double* A, double* B, int32_t* ipiv,
+                             int32_t* info, const int32_t* order);
+/* Mixed precision: FP32 LU with partial pivoting + FP64 iterative refinement.  A and B are
+ * read only; X (same layout as B) receives the solution.  Per system (LAPACK dsgesv):
+ * iter[i] >= 0  converged after iter[i] refinement steps;
+ * iter[i] <  0  fell back to the FP64 LU:  -2 FP32 factorisation hit a zero pivot,
+ *               -3 refinement did not converge in max_iter steps (-31 in LAPACK).
+ * Stopping test per RHS column: ||r||_inf <= ||x||_inf * ||A||_inf * eps * sqrt(n) with
+ * eps = tol (pass 0 to use LAPACK's 2^-53).  Refinement also stops (-> FP64) when the
+ * largest residual fails to halve on two consecutive steps.  info[i] as in
+ * bzg_dgesv_batched.  Scratch (per-workgroup FP32/FP64 matrices) is cached in the context. */
+bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
+                              const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
+                              const double* A, const double* B, double* X, int32_t* iter,
+                              int32_t* info, double tol, int32_t max_iter, const int32_t* order);
+/* Longest-first work order for the batched solvers: writes to order_dev (device, count
+ * entries) the system indices sorted by descending n (stable), computed from n_host. */
+bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_host,
+                             int32_t* order_dev);
+
+/* ---- FP64-accurate GEMM from BF16 residue slices (Ozaki scheme II on MFMA) --------------
+ * Same contract as bzg_dgemm with alpha = 1, beta = 0.  `moduli` selects the number of
+ * coprime moduli (BF16 slice GEMMs); 0 picks it from k (>= 1e-14 relative Frobenius
+ * accuracy for inputs without extreme exponent spread).  Workspace is allocated from the
+ * context on first use and cached. */
+bzg_status bzg_dgemm_fp64acc(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
+                             const double* A, int64_t lda, const double* B, int64_t ldb,
+                             double* C, int64_t ldc, int32_t moduli);
+/* Number of BF16 slice GEMMs that bzg_dgemm_fp64acc will run for this k / moduli choice. */
+int32_t bzg_fp64acc_slices(int64_t k, int32_t moduli);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLAZE_GPU_H */

@@ -1,0 +1,198 @@
+"""CPU ORACLE -- test infrastructure only.
+
+numpy-facing wrappers around oracle/liboracle.so (the C restatement of the reference
+algorithms, oracle.c) and oracle/_ref/libblazeref.so (the reference's own kernels compiled
+from its headers, when that build exists).  Only tests/, the smoke check and the benchmark's
+CPU-baseline leg import this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SO = HERE / "_ref" / "libblazeref.so"
+
+i64, i32, f64, vp = C.c_int64, C.c_int32, C.c_double, C.c_void_p
+_orc = None
+_ref = None
+
+
+def build() -> None:
+    """Compile liboracle.so and, when the reference tree is present, _ref/libblazeref.so."""
+    r = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
+def orc() -> C.CDLL:
+    global _orc
+    if _orc is None:
+        if not ORACLE_SO.exists():
+            build()
+        L = C.CDLL(str(ORACLE_SO))
+        L.orc_getrf.restype = i32
+        L.orc_sgetrf.restype = i32
+        L.orc_gesv.restype = i32
+        L.orc_solve_closed.restype = i32
+        L.orc_dsgesv.restype = i32
+        L.orc_backward_error.restype = f64
+        _orc = L
+    return _orc
+
+
+def have_ref() -> bool:
+    return REF_SO.exists()
+
+
+def ref() -> C.CDLL:
+    global _ref
+    if _ref is None:
+        if not REF_SO.exists():
+            raise FileNotFoundError(f"{REF_SO} not built (needs the reference tree)")
+        L = C.CDLL(str(REF_SO))
+        L.ref_dsolve_small.restype = i32
+        L.ref_get_threads.restype = i32
+        for n in ("ref_time_dgemm", "ref_time_dcsrmv", "ref_time_dgemv"):
+            getattr(L, n).restype = f64
+        _ref = L
+    return _ref
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(vp)
+
+
+# ---- oracle (restatement) ---------------------------------------------------------------
+def gemm(A: np.ndarray, B: np.ndarray, fused: bool = True) -> np.ndarray:
+    m, k = A.shape
+    k2, n = B.shape
+    assert k == k2
+    if A.dtype == np.float64:
+        Cm = np.empty((m, n), np.float64)
+        orc().orc_gemm_d(i64(m), i64(n), i64(k), _p(A), i64(k), _p(B), i64(n), _p(Cm), i64(n), C.c_int(int(fused)))
+    elif A.dtype == np.float32:
+        Cm = np.empty((m, n), np.float32)
+        orc().orc_gemm_s(i64(m), i64(n), i64(k), _p(A), i64(k), _p(B), i64(n), _p(Cm), i64(n))
+    elif A.dtype == np.int32:
+        Cm = np.empty((m, n), np.int32)
+        orc().orc_gemm_i32(i64(m), i64(n), i64(k), _p(A), i64(k), _p(B), i64(n), _p(Cm), i64(n))
+    else:
+        raise TypeError(A.dtype)
+    return Cm
+
+
+def gemm_dd_entries(A: np.ndarray, B: np.ndarray, rows: np.ndarray, cols: np.ndarray):
+    """Double-double C[rows, cols] of A @ B: returns (hi, lo)."""
+    k = A.shape[1]
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    hi = np.empty(len(rows)); lo = np.empty(len(rows))
+    orc().orc_gemm_dd_entries(i64(k), _p(A), i64(A.shape[1]), _p(B), i64(B.shape[1]), i64(len(rows)),
+                              _p(rows), _p(cols), _p(hi), _p(lo))
+    return hi, lo
+
+
+def gemv(A: np.ndarray, x: np.ndarray) -> np.ndarray:
+    m, n = A.shape
+    fn = {np.dtype(np.float64): "orc_gemv_d", np.dtype(np.float32): "orc_gemv_s",
+          np.dtype(np.int32): "orc_gemv_i32"}[A.dtype]
+    y = np.empty(m, A.dtype)
+    getattr(orc(), fn)(i64(m), i64(n), _p(A), i64(n), _p(x), _p(y))
+    return y
+
+
+def csrmv(rowptr, col, val, x) -> np.ndarray:
+    n = len(rowptr) - 1
+    fn = {np.dtype(np.float64): "orc_csrmv_d", np.dtype(np.float32): "orc_csrmv_s",
+          np.dtype(np.int32): "orc_csrmv_i32"}[val.dtype]
+    y = np.empty(n, val.dtype)
+    getattr(orc(), fn)(i64(n), _p(rowptr), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+def getrf(A: np.ndarray):
+    """LU with partial pivoting (dgetf2 rule) of a copy of A: returns (LU, ipiv 1-based, info)."""
+    LU = np.array(A, np.float64, order="C", copy=True)
+    n = LU.shape[0]
+    ip = np.zeros(n, np.int32)
+    info = orc().orc_getrf(i32(n), _p(LU), i32(n), _p(ip))
+    return LU, ip, info
+
+
+def gesv(A: np.ndarray, B: np.ndarray):
+    """(X, LU, ipiv, info) of A X = B (B is n x nrhs)."""
+    LU = np.array(A, np.float64, order="C", copy=True)
+    X = np.array(B, np.float64, order="C", copy=True)
+    n = LU.shape[0]
+    nrhs = X.shape[1]
+    ip = np.zeros(n, np.int32)
+    info = orc().orc_gesv(i32(n), i32(nrhs), _p(LU), i32(n), _p(ip), _p(X), i32(nrhs))
+    return X, LU, ip, info
+
+
+def solve_closed(A: np.ndarray, b: np.ndarray):
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, np.float64); b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n)
+    rc = orc().orc_solve_closed(i32(n), _p(A), _p(b), _p(x))
+    return x, rc
+
+
+def dsgesv(A: np.ndarray, B: np.ndarray, eps: float = 0.0, max_iter: int = 30):
+    """Mixed-precision solve: returns (X, iter, info)."""
+    A = np.ascontiguousarray(A, np.float64); B = np.ascontiguousarray(B, np.float64)
+    n, nrhs = B.shape
+    X = np.zeros_like(B)
+    info = C.c_int(0)
+    it = orc().orc_dsgesv(i32(n), i32(nrhs), _p(A), _p(B), _p(X), f64(eps), i32(max_iter), C.byref(info))
+    return X, it, info.value
+
+
+def backward_error(A, B, X) -> float:
+    A = np.ascontiguousarray(A, np.float64); B = np.ascontiguousarray(B, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    n, nrhs = B.shape
+    return orc().orc_backward_error(i32(n), i32(nrhs), _p(A), _p(B), _p(X))
+
+
+# ---- compiled reference ----------------------------------------------------------------
+def ref_gemm(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    m, k = A.shape
+    n = B.shape[1]
+    fn = {np.dtype(np.float64): "ref_dgemm", np.dtype(np.float32): "ref_sgemm",
+          np.dtype(np.int32): "ref_igemm"}[A.dtype]
+    Cm = np.empty((m, n), A.dtype)
+    getattr(ref(), fn)(i64(m), i64(n), i64(k), _p(A), _p(B), _p(Cm))
+    return Cm
+
+
+def ref_gemv(A: np.ndarray, x: np.ndarray) -> np.ndarray:
+    m, n = A.shape
+    fn = {np.dtype(np.float64): "ref_dgemv", np.dtype(np.float32): "ref_sgemv",
+          np.dtype(np.int32): "ref_igemv"}[A.dtype]
+    y = np.empty(m, A.dtype)
+    getattr(ref(), fn)(i64(m), i64(n), _p(A), _p(x), _p(y))
+    return y
+
+
+def ref_csrmv(rowptr, col, val, x, n_cols: int) -> np.ndarray:
+    n = len(rowptr) - 1
+    fn = {np.dtype(np.float64): "ref_dcsrmv", np.dtype(np.float32): "ref_scsrmv",
+          np.dtype(np.int32): "ref_icsrmv"}[val.dtype]
+    y = np.empty(n, val.dtype)
+    getattr(ref(), fn)(i64(n), i64(n_cols), _p(rowptr), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
+def ref_solve_small(A: np.ndarray, b: np.ndarray):
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, np.float64); b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n)
+    rc = ref().ref_dsolve_small(i32(n), _p(A), _p(b), _p(x))
+    return x, rc

@@ -1,0 +1,161 @@
+"""GPU parity: dense GEMM (C = A * B) against the CPU oracle and the reference's known answers.
+
+Mirrors blazetest's dmatdmatmult operation tests (blazetest/src/mathtest/operations/
+dmatdmatmult/MDaMDa.cpp:77-92 size grid; OperationTest.h checks with relaxed equality) and the
+known answer of matrices/dynamicmatrix/ClassTest2.cpp:1244-1273.
+Tolerances: int32 and integer-valued doubles bit-exact; random doubles within Blaze's relaxed
+equality (|a-b| <= 1e-8 * max(1,|a|), Equal.h:214-224) and within the Higham bound
+gamma_k * (|A||B|)_ij of the exact product; float within 1e-6 relative.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.gpu_util import ctx, dev, empty, full, have_gpu, host, p
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not have_gpu(), reason="no GPU")]
+
+FN = {np.float64: "bzg_dgemm", np.float32: "bzg_sgemm", np.int32: "bzg_igemm"}
+
+
+def gpu_gemm(A, B, alpha=1, beta=0, Cinit=None, lda=None, ldb=None, ldc=None):
+    dt = A.dtype.type
+    m, k = A.shape
+    n = B.shape[1]
+    lda = lda or max(k, 1); ldb = ldb or max(n, 1); ldc = ldc or max(n, 1)
+    Ap = np.zeros((m, lda), A.dtype); Ap[:, :k] = A
+    Bp = np.zeros((k, ldb), A.dtype); Bp[:, :n] = B
+    dA = dev(Ap) if m * lda else empty((1,), A.dtype)
+    dB = dev(Bp) if k * ldb else empty((1,), A.dtype)
+    if Cinit is not None:
+        Cp = np.zeros((m, ldc), A.dtype); Cp[:, :n] = Cinit
+        dC = dev(Cp)
+    else:
+        dC = full((max(m, 1), ldc), A.dtype, float("nan") if dt != np.int32 else -7)
+    ctx().call(FN[dt], m, n, k, alpha, p(dA), lda, p(dB), ldb, beta, p(dC), ldc)
+    return host(dC)[:m, :n]
+
+
+def relaxed_equal(a, b, eps):
+    return np.all(np.abs(a - b) <= eps * np.maximum(1.0, np.abs(b)))
+
+
+def test_known_answer_classtest2():
+    # ClassTest2.cpp:1244-1273: (1 0 2; 0 3 0; 4 0 5) * (0 2 0 0; 1 3 0 4; 0 0 0 5)
+    A = np.array([[1, 0, 2], [0, 3, 0], [4, 0, 5]])
+    B = np.array([[0, 2, 0, 0], [1, 3, 0, 4], [0, 0, 0, 5]])
+    expect = np.array([[0, 2, 0, 10], [3, 9, 0, 12], [0, 8, 0, 25]])
+    for dt in (np.int32, np.float64, np.float32):
+        got = gpu_gemm(A.astype(dt), B.astype(dt))
+        assert np.array_equal(got, expect.astype(dt)), dt
+
+
+def test_aliasing_fixture_products():
+    # AliasingTest.cpp:1803-1850: dA3x4 * dB4x3
+    A = np.array([[-1, 0, -2, 0], [0, 2, -3, 1], [0, 1, 2, 2]])
+    B = np.array([[1, 0, -3], [0, -1, 0], [0, 2, 1], [2, 1, -2]])
+    for dt in (np.int32, np.float64):
+        assert np.array_equal(gpu_gemm(A.astype(dt), B.astype(dt)), (A @ B).astype(dt))
+
+
+SIZES = list(range(0, 7)) + [15, 16, 32, 37, 63, 64]
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.float64])
+def test_blazetest_size_grid(dt):
+    rng = np.random.default_rng(7)
+    for m in SIZES:
+        for n in SIZES:
+            for k in SIZES:
+                if dt == np.int32:
+                    A = rng.integers(-10, 11, (m, k)).astype(np.int32)
+                    B = rng.integers(-10, 11, (k, n)).astype(np.int32)
+                else:
+                    A = rng.uniform(-1, 1, (m, k)); B = rng.uniform(-1, 1, (k, n))
+                got = gpu_gemm(A, B)
+                ref = oracle.gemm(A, B)
+                if dt == np.int32:
+                    assert np.array_equal(got, ref), (m, n, k)
+                else:
+                    assert relaxed_equal(got, ref, 1e-8), (m, n, k)
+
+
+def test_float_grid():
+    rng = np.random.default_rng(8)
+    for (m, n, k) in [(1, 1, 1), (5, 3, 7), (37, 63, 15), (64, 64, 64), (130, 70, 257)]:
+        A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+        B = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+        got = gpu_gemm(A, B)
+        ref = A.astype(np.float64) @ B.astype(np.float64)
+        bound = 2 * k * 2.0 ** -24 * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)) + 1e-30
+        assert np.all(np.abs(got - ref) <= bound), (m, n, k)
+
+
+def test_integer_valued_doubles_bit_exact_all_tiles():
+    # Integer entries in [-10,10]: every partial sum is an integer < 2^53, so every summation
+    # order is exact -> bit-exact against the oracle regardless of the tile shape chosen.
+    rng = np.random.default_rng(9)
+    for (m, n, k) in [(512, 512, 512), (1000, 1100, 300), (2048, 2048, 64), (129, 257, 1031)]:
+        A = rng.integers(-10, 11, (m, k)).astype(np.float64)
+        B = rng.integers(-10, 11, (k, n)).astype(np.float64)
+        assert np.array_equal(gpu_gemm(A, B), A @ B), (m, n, k)
+
+
+def test_config1_512_mt19937_64():
+    # 512^3, A then B uniform[-1,1) from one MT19937-64 stream (seed 128753984).
+    import blaze_amd as bz
+    m = n = k = 512
+    A = np.empty((m, k)); B = np.empty((k, n))
+    bz.gen().bzg_gen_uniform_mt(C.c_uint64(128753984), C.c_int64(A.size), A.ctypes.data_as(C.c_void_p),
+                                C.c_int64(B.size), B.ctypes.data_as(C.c_void_p))
+    got = gpu_gemm(A, B)
+    ref = oracle.gemm(A, B)
+    assert relaxed_equal(got, ref, 1e-8)
+    # Higham: |C_hat - C| <= gamma_k |A||B|; check against the double-double exact entries.
+    rows = np.arange(0, 512, 7); cols = (rows * 13) % 512
+    hi, lo = oracle.gemm_dd_entries(A, B, rows, cols)
+    absAB = np.abs(A) @ np.abs(B)
+    gam = k * 2.0 ** -53 / (1 - k * 2.0 ** -53)
+    assert np.all(np.abs(got[rows, cols] - hi) <= gam * absAB[rows, cols])
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert rel < 1e-14
+
+
+def test_alpha_beta_and_padding():
+    rng = np.random.default_rng(10)
+    m, n, k = 70, 90, 33
+    A = rng.uniform(-1, 1, (m, k)); B = rng.uniform(-1, 1, (k, n)); C0 = rng.uniform(-1, 1, (m, n))
+    got = gpu_gemm(A, B, alpha=2.5, beta=-0.5, Cinit=C0, lda=40, ldb=96, ldc=97)
+    assert relaxed_equal(got, 2.5 * (A @ B) - 0.5 * C0, 1e-12)
+
+
+def test_k_zero_resets():
+    # DMatDMatMultExpr.h:527-530: with K == 0 the target is reset.
+    A = np.zeros((5, 0)); B = np.zeros((0, 4))
+    got = gpu_gemm(A, B, Cinit=np.full((5, 4), 3.0))
+    assert np.array_equal(got, np.zeros((5, 4)))
+
+
+def test_beta_zero_never_reads_nan_target():
+    rng = np.random.default_rng(11)
+    A = rng.uniform(-1, 1, (300, 200)); B = rng.uniform(-1, 1, (200, 260))
+    got = gpu_gemm(A, B)  # target pre-filled with NaN
+    assert np.isfinite(got).all()
+
+
+def test_invalid_arguments_rejected():
+    import blaze_amd as bz
+    with pytest.raises(bz.BlazeGpuError):
+        ctx().call("bzg_dgemm", 4, 4, 4, 1.0, C.c_void_p(0), 4, C.c_void_p(0), 4, 0.0, C.c_void_p(0), 4)
+    with pytest.raises(bz.BlazeGpuError):
+        d = empty((16,), np.float64)
+        ctx().call("bzg_dgemm", 4, 4, 4, 1.0, p(d), 3, p(d), 4, 0.0, p(d), 4)
+
+
+def test_deterministic_repeat():
+    rng = np.random.default_rng(12)
+    A = rng.uniform(-1, 1, (700, 900)); B = rng.uniform(-1, 1, (900, 650))
+    a = gpu_gemm(A, B); b = gpu_gemm(A, B)
+    assert np.array_equal(a, b)
