@@ -49,7 +49,9 @@ SIGNATURES = {
     "bzg_dsgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, i32, vp]),
     "bzg_batched_order": (i32, [vp, i64, vp, vp]),
     "bzg_dgemm_fp64acc": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]),
+    "bzg_dgemm_fp64acc_i8": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]),
     "bzg_fp64acc_slices": (i32, [i64, i32]),
+    "bzg_fp64acc_operand_bits": (i32, [i64, i32, i32]),
 }
 
 

@@ -44,7 +44,7 @@ struct bzg_context {
   hipStream_t stream = nullptr;
   int num_cus = 256;
   // Cached scratch for kernels that need it (split GEMM slices, solver staging).
-  bzg::Workspace ws[4];
+  bzg::Workspace ws[8];
 };
 
 namespace bzg {

@@ -1,0 +1,624 @@
+// FP64-accurate GEMM from low-precision residue slices on MFMA (Ozaki scheme II).
+//
+// No reference function: this is a drop-in for the FP64 C = A * B assignment
+// (blaze/math/expressions/DMatDMatMultExpr.h:516-569) that computes the product through
+// exact integer arithmetic on the BF16 (or INT8) matrix cores.
+//
+// Method (Ozaki, Uchino & Imamura, "Ozaki Scheme II: a GEMM-oriented emulation of floating-
+// point matrix multiplication using an integer modular technique", 2025; here with BF16
+// operands and FP32 accumulation):
+//   1. scale every row i of A by 2^s_i and every column j of B by 2^t_j so that the rounded
+//      integers A' = rint(A_i * 2^s_i), B' = rint(B_j * 2^t_j) satisfy |A'|, |B'| <= 2^P;
+//   2. for each of N pairwise-coprime odd moduli m_l <= 511, form the symmetric residues
+//      A'_l = A' mod m_l, B'_l = B' mod m_l (|r| <= 255: exact in BF16);
+//   3. C'_l = A'_l B'_l mod m_l on MFMA v_mfma_f32_32x32x16_bf16.  Every product is exact in
+//      FP32 (|r r'| <= 65025) and every 256-long K chunk sums to at most 256*65025 + 767 < 2^24,
+//      so the FP32 accumulator is an exact integer; it is reduced mod m_l after each chunk;
+//   4. CRT: C' = sum_l C'_l W_l mod M, W_l = (M/m_l) * ((M/m_l)^-1 mod m_l), M = prod m_l.  W_l
+//      and M are split into 36-bit chunks so that the three partial sums are exact in FP64;
+//      |C'| <= k 2^(2P) <= M/4 makes the symmetric reduction unique;
+//   5. C = C' * 2^-(s_i + t_j), rounded once to FP64.
+// The INT8 variant uses moduli <= 255 (|r| <= 127) on v_mfma_i32_32x32x32_i8 with exact INT32
+// accumulation (no chunking for k <= 2^17).
+// Error: |C - AB| <= 2^-P * (row max |A_i|)(col max |B_j|) * k  +  one FP64 rounding.
+#include "common.hpp"
+
+#include <cmath>
+#include <type_traits>
+#include <cstring>
+#include <vector>
+
+namespace bzg {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kMaxModuli = 24;
+
+struct CrtConsts {
+  double m[kMaxModuli];
+  double minv[kMaxModuli];
+  double c30[kMaxModuli];  // 2^30 mod m
+  double w1[kMaxModuli], w2[kMaxModuli], w3[kMaxModuli];
+  double M1, M2, M3, Minv;
+  int n;
+};
+
+// ------------------------------------------------------------------------------------------
+// host: moduli and CRT constants (small big-integer arithmetic, 32-bit limbs)
+// ------------------------------------------------------------------------------------------
+struct Big {
+  std::vector<uint32_t> d;  // little-endian limbs
+  explicit Big(uint64_t v = 0) {
+    d.push_back((uint32_t)v);
+    d.push_back((uint32_t)(v >> 32));
+    trim();
+  }
+  void trim() {
+    while (d.size() > 1 && d.back() == 0) d.pop_back();
+  }
+  void mul_small(uint32_t s) {
+    uint64_t c = 0;
+    for (auto& x : d) {
+      uint64_t v = (uint64_t)x * s + c;
+      x = (uint32_t)v;
+      c = v >> 32;
+    }
+    if (c) d.push_back((uint32_t)c);
+  }
+  uint32_t mod_small(uint32_t m) const {
+    uint64_t r = 0;
+    for (size_t i = d.size(); i-- > 0;) r = ((r << 32) | d[i]) % m;
+    return (uint32_t)r;
+  }
+  int bits() const {
+    int b = (int)(d.size() - 1) * 32;
+    uint32_t t = d.back();
+    while (t) {
+      ++b;
+      t >>= 1;
+    }
+    return b;
+  }
+  // floor(this / 2^sh) mod 2^36 as a double
+  double chunk36(int sh) const {
+    uint64_t v = 0;
+    for (int b = 35; b >= 0; --b) {
+      const int pos = sh + b;
+      uint64_t bit = 0;
+      if (pos >= 0) {
+        const size_t limb = (size_t)pos / 32;
+        if (limb < d.size()) bit = (d[limb] >> (pos % 32)) & 1u;
+      }
+      v = (v << 1) | bit;
+    }
+    return (double)v;
+  }
+};
+
+static uint32_t modinv(uint32_t a, uint32_t m) {
+  int64_t t = 0, nt = 1, r = m, nr = a % m;
+  while (nr) {
+    const int64_t q = r / nr;
+    int64_t tmp = t - q * nt; t = nt; nt = tmp;
+    tmp = r - q * nr; r = nr; nr = tmp;
+  }
+  if (t < 0) t += m;
+  return (uint32_t)t;
+}
+
+static uint32_t gcd32(uint32_t a, uint32_t b) {
+  while (b) { const uint32_t t = a % b; a = b; b = t; }
+  return a;
+}
+
+// Pairwise-coprime odd moduli, largest first, not above `top`.
+static std::vector<uint32_t> pick_moduli(int count, uint32_t top) {
+  std::vector<uint32_t> ms;
+  for (uint32_t m = top; m >= 3 && (int)ms.size() < count; m -= 2) {
+    bool ok = true;
+    for (uint32_t x : ms)
+      if (gcd32(m, x) != 1) { ok = false; break; }
+    if (ok) ms.push_back(m);
+  }
+  return ms;
+}
+
+static double log2_prod(const std::vector<uint32_t>& ms) {
+  double s = 0;
+  for (auto m : ms) s += std::log2((double)m);
+  return s;
+}
+
+// Bits per operand after scaling: k * 2^(2P) <= M / 4.
+static int operand_bits(const std::vector<uint32_t>& ms, int64_t k) {
+  const double lm = log2_prod(ms) - 1e-9;
+  const double lk = std::log2((double)std::max<int64_t>(k, 1));
+  return (int)std::floor((lm - 2.0 - lk) / 2.0);
+}
+
+static void build_crt(const std::vector<uint32_t>& ms, CrtConsts& c) {
+  const int n = (int)ms.size();
+  Big M(1);
+  for (auto m : ms) M.mul_small(m);
+  const int E = M.bits();
+  c.n = n;
+  for (int l = 0; l < n; ++l) {
+    Big Ml(1);
+    for (int q = 0; q < n; ++q)
+      if (q != l) Ml.mul_small(ms[q]);
+    const uint32_t inv = modinv(Ml.mod_small(ms[l]), ms[l]);
+    Big W = Ml;
+    W.mul_small(inv);  // W < M * m_l; reduce mod M below by using W mod M == W - M*floor
+    // W = Ml * inv with inv < m_l, so W < M: no reduction needed.
+    c.m[l] = (double)ms[l];
+    c.minv[l] = 1.0 / (double)ms[l];
+    c.c30[l] = (double)((1ull << 30) % ms[l]);
+    c.w1[l] = std::ldexp(W.chunk36(E - 36), E - 36);
+    c.w2[l] = std::ldexp(W.chunk36(E - 72), E - 72);
+    c.w3[l] = std::ldexp(W.chunk36(E - 108), E - 108);
+  }
+  c.M1 = std::ldexp(M.chunk36(E - 36), E - 36);
+  c.M2 = std::ldexp(M.chunk36(E - 72), E - 72);
+  c.M3 = std::ldexp(M.chunk36(E - 108), E - 108);
+  c.Minv = 1.0 / (c.M1 + c.M2 + c.M3);
+}
+
+// ------------------------------------------------------------------------------------------
+// device: scaling exponents
+// ------------------------------------------------------------------------------------------
+// Row exponents of A (m x k, lda): s_i = P - 1 - ilogb(max_j |a_ij|)  (0 for zero rows).
+__global__ __launch_bounds__(256) void row_scale_kernel(int m, int k, const double* __restrict__ A,
+                                                        long lda, int P, int* __restrict__ s) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x % 64;
+  if (wave >= m) return;
+  const double* row = A + (long)wave * lda;
+  double mx = 0.0;
+  for (int j = lane; j < k; j += 64) mx = fmax(mx, fabs(row[j]));
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) s[wave] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
+}
+
+// Column exponents of B (k x n, ldb): t_j = P - 1 - ilogb(max_i |b_ij|).
+__global__ __launch_bounds__(256) void col_scale_kernel(int k, int n, const double* __restrict__ B,
+                                                        long ldb, int P, int* __restrict__ t) {
+  // 64 columns per block, 4 row phases; partial maxima combined in LDS
+  __shared__ double part[4][64];
+  const int c = blockIdx.x * 64 + threadIdx.x % 64, ph = threadIdx.x / 64;
+  double mx = 0.0;
+  if (c < n)
+    for (int i = ph; i < k; i += 4) mx = fmax(mx, fabs(B[(long)i * ldb + c]));
+  part[ph][threadIdx.x % 64] = mx;
+  __syncthreads();
+  if (ph == 0 && c < n) {
+    mx = fmax(fmax(part[0][threadIdx.x], part[1][threadIdx.x]), fmax(part[2][threadIdx.x], part[3][threadIdx.x]));
+    t[c] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
+  }
+}
+
+// Symmetric residue of the integer-valued double v modulo m (|result| <= (m-1)/2).
+__device__ __forceinline__ double sym_mod(double v, double m, double minv) {
+  double r = fma(-m, rint(v * minv), v);  // exact: the true value is a small integer
+  const double h = 0.5 * (m - 1.0);
+  r = (r > h) ? r - m : r;
+  r = (r < -h) ? r + m : r;
+  return r;
+}
+
+// Same for |v| up to 2^106: split v = vh 2^30 + vl exactly, reduce vh, then fold in vl.
+// (sym_mod alone needs |v / m| < 2^50 for the quotient to be within one of the truth.)
+__device__ __forceinline__ double sym_mod_wide(double v, double m, double minv, double c30) {
+  const double vh = floor(v * 0x1.0p-30);
+  const double vl = fma(-vh, 0x1.0p30, v);  // exact, 0 <= vl < 2^30
+  const double rh = sym_mod(vh, m, minv);
+  return sym_mod(fma(rh, c30, vl), m, minv);
+}
+
+template <int MODE>
+struct SliceOf;
+template <>
+struct SliceOf<0> {  // BF16
+  using T = __bf16;
+  __device__ static T cvt(double r) { return (__bf16)(float)r; }
+};
+template <>
+struct SliceOf<1> {  // INT8
+  using T = int8_t;
+  __device__ static T cvt(double r) { return (int8_t)(int)r; }
+};
+
+// A residues: Ares[l][i][kk] (rows padded to mp, k padded to kp, zeros in the padding).
+template <int MODE>
+__global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const double* __restrict__ A,
+                                                         long lda, const int* __restrict__ s, int mp,
+                                                         int kp, int nmod, int wide,
+                                                         const CrtConsts* __restrict__ cc,
+                                                         typename SliceOf<MODE>::T* __restrict__ out) {
+  using S = SliceOf<MODE>;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)mp * kp;
+  if (idx >= total) return;
+  const int i = (int)(idx / kp), kk = (int)(idx % kp);
+  double v = 0.0;
+  if (i < m && kk < k) v = rint(ldexp(A[(long)i * lda + kk], s[i]));
+  if (wide) {
+    for (int l = 0; l < nmod; ++l)
+      out[(long)l * total + idx] = S::cvt(sym_mod_wide(v, cc->m[l], cc->minv[l], cc->c30[l]));
+  } else {
+    for (int l = 0; l < nmod; ++l) out[(long)l * total + idx] = S::cvt(sym_mod(v, cc->m[l], cc->minv[l]));
+  }
+}
+
+// B residues, transposed: Bres[l][j][kk] (n rows padded to np, k padded to kp).
+template <int MODE>
+__global__ __launch_bounds__(256) void residues_b_kernel(int k, int n, const double* __restrict__ B,
+                                                         long ldb, const int* __restrict__ t, int np,
+                                                         int kp, int nmod, int wide,
+                                                         const CrtConsts* __restrict__ cc,
+                                                         typename SliceOf<MODE>::T* __restrict__ out) {
+  using S = SliceOf<MODE>;
+  __shared__ double tile[64][65];
+  const int kb = blockIdx.x * 64, jb = blockIdx.y * 64;
+  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  for (int r = ty; r < 64; r += 4) {  // read B[kb+r][jb+tx]
+    const int kk = kb + r, j = jb + tx;
+    double v = 0.0;
+    if (kk < k && j < n) v = rint(ldexp(B[(long)kk * ldb + j], t[j]));
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  const long total = (long)np * kp;
+  for (int r = ty; r < 64; r += 4) {  // write Bres[l][jb+r][kb+tx]
+    const int j = jb + r, kk = kb + tx;
+    if (j < np && kk < kp) {
+      const double v = tile[tx][r];
+      for (int l = 0; l < nmod; ++l)
+        out[(long)l * total + (long)j * kp + kk] =
+            S::cvt(wide ? sym_mod_wide(v, cc->m[l], cc->minv[l], cc->c30[l]) : sym_mod(v, cc->m[l], cc->minv[l]));
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// device: residue GEMM  C_l = A_l * B_l^T mod m_l   (256 x 256 tiles, 4 wavefronts)
+// ------------------------------------------------------------------------------------------
+constexpr int RT = 256;        // tile rows / cols
+constexpr int RBYTES = 128;    // bytes of K per LDS stage and row
+constexpr int RSTAGE = RT * RBYTES;  // bytes per operand per stage
+
+// 16-byte chunk c of row r lives at chunk c ^ (r & 7): conflict-free ds_read_b128 of 32 rows.
+__device__ __forceinline__ int swz(int r, int c) { return r * RBYTES + ((c ^ (r & 7)) << 4); }
+
+// FN = 32-column MFMA blocks per wavefront: 4 -> 256 x 256 workgroup tile (INT8), 2 -> 256 x 128
+// (BF16: its FP32 accumulators are reduced on the VALU every 256 K, which needs them in VGPRs).
+template <int MODE, int FN>
+__global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, int kp,
+                                                              const char* __restrict__ Ares,
+                                                              const char* __restrict__ Bres,
+                                                              int16_t* __restrict__ Cres,
+                                                              const CrtConsts* __restrict__ cc) {
+  constexpr int ESZ = (MODE == 0) ? 2 : 1;          // bytes per element
+  constexpr int KSTEP_B = 32;                        // bytes of K per MFMA (16 bf16 / 32 i8)
+  constexpr int BN = 64 * FN;                        // workgroup tile columns
+  constexpr int NBCH = BN * 8 / 256;                 // B chunks staged per thread
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  char* As = smem;                   // [2][RT rows][RBYTES]
+  char* Bs = smem + 2 * RSTAGE;      // [2][BN rows][RBYTES]
+
+  const int l = blockIdx.z;
+  const long a_plane = (long)mp * kp * ESZ, b_plane = (long)np * kp * ESZ;
+  const char* Ag = Ares + l * a_plane;
+  const char* Bg = Bres + l * b_plane;
+  const float mf = (float)cc->m[l];
+  const float minvf = 1.0f / mf;
+
+  const int tiles_n = np / BN;
+  const int tiles_m = mp / RT;
+  const int nt = tiles_m * tiles_n;
+  int bid = blockIdx.x;
+  if (nt >= 64 && nt % 8 == 0) bid = (bid % 8) * (nt / 8) + bid / 8;  // XCD-contiguous
+  // walk tiles in column-major groups of 4 tile rows for L2 reuse
+  const int G = 4;
+  const int group = bid / (G * tiles_n);
+  const int first_m = group * G;
+  const int gsize = min(tiles_m - first_m, G);
+  const int in_group = bid % (G * tiles_n);
+  const int tm = first_m + in_group % gsize, tn = in_group / gsize;
+  const long row0 = (long)tm * RT, col0 = (long)tn * BN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * 128, wn0 = (wave & 1) * (32 * FN);
+  const long kbytes = (long)kp * ESZ;
+
+  using acc_t = typename std::conditional<MODE == 0, f16v, v16i>::type;
+  acc_t acc[4][FN];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+
+  // each thread stages 8 chunks of A and NBCH of B per stage: chunk q = tid + 256*e
+  v4i ra[8], rb[NBCH];
+  auto gload = [&](int st) {
+    const long kb = (long)st * RBYTES;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int q = tid + 256 * e, r = q >> 3, c = q & 7;
+      ra[e] = *reinterpret_cast<const v4i*>(Ag + (row0 + r) * kbytes + kb + c * 16);
+    }
+#pragma unroll
+    for (int e = 0; e < NBCH; ++e) {
+      const int q = tid + 256 * e, r = q >> 3, c = q & 7;
+      rb[e] = *reinterpret_cast<const v4i*>(Bg + (col0 + r) * kbytes + kb + c * 16);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int q = tid + 256 * e, r = q >> 3, c = q & 7;
+      *reinterpret_cast<v4i*>(As + buf * RSTAGE + swz(r, c)) = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NBCH; ++e) {
+      const int q = tid + 256 * e, r = q >> 3, c = q & 7;
+      *reinterpret_cast<v4i*>(Bs + buf * RSTAGE + swz(r, c)) = rb[e];
+    }
+  };
+
+  const int nst = (int)(kbytes / RBYTES);
+  // K elements per stage: 64 (bf16) / 128 (i8); bf16 reduces every 4 stages (256 K)
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    if (st + 1 < nst) gload(st + 1);
+    const char* as = As + buf * RSTAGE;
+    const char* bs = Bs + buf * RSTAGE;
+#pragma unroll
+    for (int ks = 0; ks < RBYTES / KSTEP_B; ++ks) {
+      v4i a[4], b[FN];
+      const int c = ks * 2 + (lane >> 5);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const v4i*>(as + swz(wm0 + i * 32 + (lane & 31), c));
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + swz(wn0 + j * 32 + (lane & 31), c));
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          if constexpr (MODE == 0) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a[i]),
+                                                                 __builtin_bit_cast(bf8, b[j]),
+                                                                 acc[i][j], 0, 0, 0);
+          } else {
+            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+          }
+        }
+    }
+    if constexpr (MODE == 0) {
+      // exact-integer FP32 accumulators: reduce mod m after every 256 K (4 stages)
+      if ((st & 3) == 3 && st + 1 < nst) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const float v = acc[i][j][r];
+              acc[i][j][r] = __builtin_fmaf(-mf, __builtin_rintf(v * minvf), v);
+            }
+      }
+    }
+    if (st + 1 < nst) sstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  // epilogue: symmetric residue mod m as int16
+  const long cplane = (long)mp * np;
+  int16_t* Cg = Cres + l * cplane;
+  const int mi = (int)cc->m[l];
+  const int half = (mi - 1) / 2;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        int v;
+        if constexpr (MODE == 0) {
+          const float f = acc[i][j][r];
+          v = (int)__builtin_fmaf(-mf, __builtin_rintf(f * minvf), f);
+        } else {
+          v = acc[i][j][r] % mi;
+        }
+        v = (v > half) ? v - mi : v;
+        v = (v < -half) ? v + mi : v;
+        const long gr = row0 + wm0 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long gc = col0 + wn0 + j * 32 + (lane & 31);
+        Cg[gr * np + gc] = (int16_t)v;
+      }
+}
+
+// ------------------------------------------------------------------------------------------
+// device: CRT recombination and scaling
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void crt_kernel(int m, int n, int mp, int np, int nmod,
+                                                  const int16_t* __restrict__ Cres,
+                                                  const int* __restrict__ s, const int* __restrict__ t,
+                                                  const CrtConsts* __restrict__ cc, double* __restrict__ C,
+                                                  long ldc) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)m * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  const long plane = (long)mp * np;
+  const long off = (long)i * np + j;
+  double S1 = 0.0, S2 = 0.0, S3 = 0.0;
+  for (int l = 0; l < nmod; ++l) {
+    const double r = (double)Cres[l * plane + off];
+    S1 = fma(r, cc->w1[l], S1);  // exact: multiples of a common ulp, below 2^53 ulps
+    S2 = fma(r, cc->w2[l], S2);
+    S3 = fma(r, cc->w3[l], S3);
+  }
+  const double q = rint(S1 * cc->Minv);
+  const double T1 = fma(-q, cc->M1, S1);  // exact
+  const double T2 = fma(-q, cc->M2, S2);  // exact
+  const double T3 = fma(-q, cc->M3, S3);
+  // T1 + T2 needs up to ~70 bits: add it error-free (TwoSum) so that C' is rounded only once.
+  const double sum = T1 + T2;
+  const double bb = sum - T1;
+  const double err = (T1 - (sum - bb)) + (T2 - bb);
+  const double Cp = sum + (err + T3);
+  C[(long)i * ldc + j] = ldexp(Cp, -(s[i] + t[j]));
+}
+
+struct Fp64AccPlan {
+  int mode;  // 0 bf16, 1 int8
+  int nmod;
+  int P;
+  CrtConsts host;
+};
+
+bzg_status plan_fp64acc(int64_t k, int32_t moduli, int mode, Fp64AccPlan& pl) {
+  const uint32_t top = mode == 0 ? 511u : 255u;
+  int nm = moduli;
+  if (nm <= 0) {
+    // smallest count giving P >= 50 bits per operand (rel. error ~2^-50 for row/col-scaled data)
+    for (nm = 4; nm <= kMaxModuli; ++nm)
+      if (operand_bits(pick_moduli(nm, top), k) >= 50) break;
+  }
+  BZG_REQUIRE(nm >= 2 && nm <= kMaxModuli, BZG_ERR_INVALID_ARGUMENT, "fp64acc: moduli must be 2..24");
+  auto ms = pick_moduli(nm, top);
+  BZG_REQUIRE((int)ms.size() == nm, BZG_ERR_UNSUPPORTED, "fp64acc: not enough coprime moduli");
+  pl.mode = mode;
+  pl.nmod = nm;
+  pl.P = operand_bits(ms, k);
+  BZG_REQUIRE(pl.P >= 1, BZG_ERR_UNSUPPORTED, "fp64acc: too few moduli for this k");
+  pl.P = std::min(pl.P, 104);  // |A'| <= 2^104 keeps the residue split exact
+  BZG_REQUIRE(mode == 0 || k <= (1LL << 17), BZG_ERR_UNSUPPORTED, "fp64acc int8: k > 2^17");
+  build_crt(ms, pl.host);
+  return BZG_OK;
+}
+
+}  // namespace
+
+bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k, const double* A,
+                              int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                              int32_t moduli, int mode) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "fp64acc: null context");
+  BZG_REQUIRE(m >= 0 && n >= 0 && k >= 0, BZG_ERR_INVALID_ARGUMENT, "fp64acc: negative size");
+  BZG_REQUIRE(m < (1LL << 31) && n < (1LL << 31) && k < (1LL << 31), BZG_ERR_UNSUPPORTED,
+              "fp64acc: dimension >= 2^31");
+  BZG_REQUIRE(lda >= (k > 0 ? k : 1) && ldb >= (n > 0 ? n : 1) && ldc >= (n > 0 ? n : 1),
+              BZG_ERR_INVALID_ARGUMENT, "fp64acc: leading dimension smaller than row length");
+  if (m == 0 || n == 0) return BZG_OK;
+  if (k == 0) return bzg_dgemm(ctx, m, n, 0, 1.0, A, lda, B, ldb, 0.0, C, ldc);
+  BZG_REQUIRE(A && B && C, BZG_ERR_INVALID_ARGUMENT, "fp64acc: null pointer");
+  Fp64AccPlan pl;
+  bzg_status st = plan_fp64acc(k, moduli, mode, pl);
+  if (st) return st;
+  const int esz = mode == 0 ? 2 : 1;
+  const int kq = RBYTES / esz;  // K elements per stage
+  const long mp = (m + RT - 1) / RT * RT, np = (n + RT - 1) / RT * RT;
+  const long kp = (k + kq - 1) / kq * kq;
+  const size_t a_bytes = (size_t)pl.nmod * mp * kp * esz;
+  const size_t b_bytes = (size_t)pl.nmod * np * kp * esz;
+  const size_t c_bytes = (size_t)pl.nmod * mp * np * sizeof(int16_t);
+  const size_t misc_bytes = 4096 + (size_t)(mp + np) * sizeof(int);
+  const size_t total = a_bytes + b_bytes + c_bytes + misc_bytes;
+  void* ws = nullptr;
+  st = workspace(ctx, 4, total, &ws);
+  if (st) return st;
+  char* base = static_cast<char*>(ws);
+  CrtConsts* dcc = reinterpret_cast<CrtConsts*>(base);
+  int* sA = reinterpret_cast<int*>(base + 4096);
+  int* tB = sA + mp;
+  char* Ares = base + misc_bytes;
+  char* Bres = Ares + a_bytes;
+  int16_t* Cres = reinterpret_cast<int16_t*>(Bres + b_bytes);
+  static_assert(sizeof(CrtConsts) <= 4096, "CRT constants block");
+  BZG_HIP_CHECK(hipMemcpyAsync(dcc, &pl.host, sizeof(CrtConsts), hipMemcpyHostToDevice, ctx->stream));
+
+  hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)((m * 64 + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (int)m, (int)k, A, (long)lda, pl.P, sA);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, ctx->stream,
+                     (int)k, (int)n, B, (long)ldb, pl.P, tB);
+  BZG_LAUNCH_CHECK();
+  const long na = mp * kp;
+  const int wide = pl.P > 56 ? 1 : 0;
+  if (mode == 0) {
+    hipLaunchKernelGGL(residues_a_kernel<0>, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)m, (int)k, A, (long)lda, sA, (int)mp, (int)kp, pl.nmod, wide, dcc,
+                       reinterpret_cast<__bf16*>(Ares));
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(residues_b_kernel<0>, dim3((unsigned)(kp / 64), (unsigned)(np / 64)), dim3(256), 0,
+                       ctx->stream, (int)k, (int)n, B, (long)ldb, tB, (int)np, (int)kp, pl.nmod, wide, dcc,
+                       reinterpret_cast<__bf16*>(Bres));
+    BZG_LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(residues_a_kernel<1>, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)m, (int)k, A, (long)lda, sA, (int)mp, (int)kp, pl.nmod, wide, dcc,
+                       reinterpret_cast<int8_t*>(Ares));
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(residues_b_kernel<1>, dim3((unsigned)(kp / 64), (unsigned)(np / 64)), dim3(256), 0,
+                       ctx->stream, (int)k, (int)n, B, (long)ldb, tB, (int)np, (int)kp, pl.nmod, wide, dcc,
+                       reinterpret_cast<int8_t*>(Bres));
+    BZG_LAUNCH_CHECK();
+  }
+  const size_t lds = 4 * RSTAGE;
+  static bool attr_set[2] = {false, false};
+  if (!attr_set[mode]) {
+    if (mode == 0)
+      BZG_HIP_CHECK(hipFuncSetAttribute((const void*)residue_gemm_kernel<0, 2>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    else
+      BZG_HIP_CHECK(hipFuncSetAttribute((const void*)residue_gemm_kernel<1, 4>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set[mode] = true;
+  }
+  if (mode == 0)
+    hipLaunchKernelGGL((residue_gemm_kernel<0, 2>), dim3((unsigned)((mp / RT) * (np / 128)), 1, pl.nmod),
+                       dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres, Cres, dcc);
+  else
+    hipLaunchKernelGGL((residue_gemm_kernel<1, 4>), dim3((unsigned)((mp / RT) * (np / RT)), 1, pl.nmod),
+                       dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres, Cres, dcc);
+  BZG_LAUNCH_CHECK();
+  const long mn = m * n;
+  hipLaunchKernelGGL(crt_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, ctx->stream, (int)m,
+                     (int)n, (int)mp, (int)np, pl.nmod, Cres, sA, tB, dcc, C, (long)ldc);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+}  // namespace bzg
+
+extern "C" {
+
+bzg_status bzg_dgemm_fp64acc(bzg_context* ctx, int64_t m, int64_t n, int64_t k, const double* A,
+                             int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                             int32_t moduli) {
+  return bzg::dgemm_fp64acc_impl(ctx, m, n, k, A, lda, B, ldb, C, ldc, moduli, 0);
+}
+
+bzg_status bzg_dgemm_fp64acc_i8(bzg_context* ctx, int64_t m, int64_t n, int64_t k, const double* A,
+                                int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                                int32_t moduli) {
+  return bzg::dgemm_fp64acc_impl(ctx, m, n, k, A, lda, B, ldb, C, ldc, moduli, 1);
+}
+
+int32_t bzg_fp64acc_slices(int64_t k, int32_t moduli) {
+  bzg::Fp64AccPlan pl;
+  if (bzg::plan_fp64acc(k, moduli, 0, pl) != BZG_OK) return -1;
+  return pl.nmod;
+}
+
+int32_t bzg_fp64acc_operand_bits(int64_t k, int32_t moduli, int32_t int8_slices) {
+  bzg::Fp64AccPlan pl;
+  if (bzg::plan_fp64acc(k, moduli, int8_slices ? 1 : 0, pl) != BZG_OK) return -1;
+  return pl.P;
+}
+
+}  // extern "C"

@@ -160,8 +160,16 @@ bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_h
 bzg_status bzg_dgemm_fp64acc(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                              const double* A, int64_t lda, const double* B, int64_t ldb,
                              double* C, int64_t ldc, int32_t moduli);
+/* Same contract and method with INT8 residues (moduli <= 255) on the INT8 matrix cores with
+ * exact INT32 accumulation (k <= 2^17); roughly twice the slice-GEMM rate of BF16. */
+bzg_status bzg_dgemm_fp64acc_i8(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
+                                const double* A, int64_t lda, const double* B, int64_t ldb,
+                                double* C, int64_t ldc, int32_t moduli);
 /* Number of BF16 slice GEMMs that bzg_dgemm_fp64acc will run for this k / moduli choice. */
 int32_t bzg_fp64acc_slices(int64_t k, int32_t moduli);
+/* Bits P kept per scaled operand (|A'|,|B'| <= 2^P) for this k / moduli choice; the
+ * elementwise error is bounded by k * 2^-P * max|A_i,:| * max|B_:,j|.  -1 if invalid. */
+int32_t bzg_fp64acc_operand_bits(int64_t k, int32_t moduli, int32_t int8_slices);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,99 @@
+"""GPU parity: FP64-accurate GEMM from BF16 / INT8 residue slices (Ozaki scheme II).
+
+No reference function exists for this path; its oracle is the exact product, computed on the
+CPU in double-double (oracle.gemm_dd_entries: TwoProduct/TwoSum, ~106 bits) and, for integer
+inputs, exactly.  Bars:
+  * integer-valued inputs: bit-exact (C' is exact, the final rounding is exact);
+  * uniform[-1,1) inputs (sigma = 0): relative Frobenius error <= 1e-14 against the exact
+    product (the configured target), on sampled entries;
+  * graded exponents (sigma = 4, 10): every entry within the scheme's bound
+    k * 2^-P * max|A_i,:| * max|B_:,j| + |C_ij| 2^-52 (P from bzg_fp64acc_operand_bits).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import blaze_amd as bz
+import oracle
+from tests.gpu_util import ctx, dev, full, have_gpu, host, p
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not have_gpu(), reason="no GPU")]
+
+FNS = ["bzg_dgemm_fp64acc", "bzg_dgemm_fp64acc_i8"]
+
+
+def gpu_fp64acc(fn, A, B, moduli=0):
+    m, k = A.shape
+    n = B.shape[1]
+    dA, dB = dev(A), dev(B)
+    dC = full((m, n), np.float64, float("nan"))
+    ctx().call(fn, m, n, k, p(dA), k, p(dB), n, p(dC), n, moduli)
+    return host(dC)
+
+
+def graded(rng, shape, sigma):
+    u = rng.uniform(-1, 1, shape)
+    e = np.rint(sigma * rng.standard_normal(shape)) if sigma > 0 else 0
+    return np.ldexp(u, np.asarray(e, dtype=np.int64))
+
+
+@pytest.mark.parametrize("fn", FNS)
+def test_integer_inputs_bit_exact(fn):
+    rng = np.random.default_rng(50)
+    for (m, n, k) in [(1, 1, 1), (7, 5, 3), (300, 200, 129), (256, 256, 256), (513, 257, 1000)]:
+        A = rng.integers(-1000, 1001, (m, k)).astype(np.float64)
+        B = rng.integers(-1000, 1001, (k, n)).astype(np.float64)
+        assert np.array_equal(gpu_fp64acc(fn, A, B), A @ B), (fn, m, n, k)
+
+
+@pytest.mark.parametrize("fn", FNS)
+def test_uniform_relative_frobenius_1e14(fn):
+    rng = np.random.default_rng(51)
+    for (m, n, k) in [(512, 512, 512), (1024, 768, 2048), (300, 700, 4097)]:
+        A = rng.uniform(-1, 1, (m, k)); B = rng.uniform(-1, 1, (k, n))
+        Cg = gpu_fp64acc(fn, A, B)
+        rows = rng.integers(0, m, 3000); cols = rng.integers(0, n, 3000)
+        hi, lo = oracle.gemm_dd_entries(A, B, rows, cols)
+        err = np.sqrt(np.sum((Cg[rows, cols] - hi - lo) ** 2) / np.sum(hi ** 2))
+        assert err <= 1e-14, (fn, m, n, k, err)
+
+
+@pytest.mark.parametrize("fn", FNS)
+@pytest.mark.parametrize("sigma", [0, 4, 10])
+def test_graded_exponents_within_bound(fn, sigma):
+    rng = np.random.default_rng(52 + sigma)
+    m, n, k = 384, 320, 1536
+    A = graded(rng, (m, k), sigma); B = graded(rng, (k, n), sigma)
+    Cg = gpu_fp64acc(fn, A, B)
+    P = bz.lib().bzg_fp64acc_operand_bits(k, 0, 1 if fn.endswith("i8") else 0)
+    assert P >= 48
+    rows = rng.integers(0, m, 2000); cols = rng.integers(0, n, 2000)
+    hi, lo = oracle.gemm_dd_entries(A, B, rows, cols)
+    ra = np.abs(A).max(axis=1)[rows]; cb = np.abs(B).max(axis=0)[cols]
+    bound = k * 2.0 ** -P * ra * cb + np.abs(hi) * 2.0 ** -52
+    assert np.all(np.abs(Cg[rows, cols] - hi) <= bound + np.abs(lo)), (fn, sigma)
+
+
+@pytest.mark.parametrize("fn", FNS)
+def test_zero_rows_and_columns(fn):
+    rng = np.random.default_rng(53)
+    A = rng.uniform(-1, 1, (70, 90)); B = rng.uniform(-1, 1, (90, 40))
+    A[5] = 0; B[:, 7] = 0
+    Cg = gpu_fp64acc(fn, A, B)
+    assert np.all(Cg[5] == 0) and np.all(Cg[:, 7] == 0)
+    assert np.abs(Cg - A @ B).max() <= 1e-14 * np.abs(A @ B).max() * 10
+
+
+def test_more_moduli_tighten_error_for_wide_exponents():
+    rng = np.random.default_rng(54)
+    m, n, k = 256, 256, 512
+    A = graded(rng, (m, k), 10); B = graded(rng, (k, n), 10)
+    rows = rng.integers(0, m, 2000); cols = rng.integers(0, n, 2000)
+    hi, lo = oracle.gemm_dd_entries(A, B, rows, cols)
+    errs = []
+    for nm in (0, 20):
+        Cg = gpu_fp64acc("bzg_dgemm_fp64acc", A, B, moduli=nm)
+        errs.append(np.sqrt(np.sum((Cg[rows, cols] - hi) ** 2) / np.sum(hi ** 2)))
+    assert errs[1] <= errs[0]
+    assert errs[1] <= 1e-14, errs
