@@ -29,6 +29,8 @@ SIGNATURES = {
     "bzg_synchronize": (i32, [vp]),
     "bzg_status_string": (C.c_char_p, [i32]),
     "bzg_last_error": (C.c_char_p, []),
+    "bzg_profile_enable": (i32, [vp, i32]),
+    "bzg_profile_read": (i32, [vp, C.c_char_p, C.POINTER(f64), C.POINTER(i64)]),
     "bzg_malloc": (i32, [C.POINTER(vp), C.c_size_t]),
     "bzg_free": (i32, [vp]),
     "bzg_memcpy": (i32, [vp, vp, vp, C.c_size_t, C.c_int]),
@@ -110,6 +112,15 @@ class Context:
 
     def call(self, name: str, *args) -> int:
         return check(name, getattr(lib(), name)(self.handle, *args))
+
+    def profile(self, on: bool = True) -> None:
+        check("bzg_profile_enable", lib().bzg_profile_enable(self.handle, int(on)))
+
+    def profile_read(self, kernel: str):
+        """(total device ms, launches) of the named kernel since the last read."""
+        ms, n = f64(), i64()
+        check("bzg_profile_read", lib().bzg_profile_read(self.handle, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def sync(self) -> None:
         check("bzg_synchronize", lib().bzg_synchronize(self.handle))

@@ -3,7 +3,10 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <utility>
+#include <vector>
 #include "blaze_gpu.h"
 
 namespace bzg {
@@ -45,9 +48,32 @@ struct bzg_context {
   int num_cus = 256;
   // Cached scratch for kernels that need it (split GEMM slices, solver staging).
   bzg::Workspace ws[8];
+  // Optional HIP-event timing of named kernels (bzg_profile_enable / bzg_profile_read).
+  bool profile = false;
+  std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
 };
 
 namespace bzg {
+// Brackets the kernels launched in its scope with HIP events on the context stream when
+// profiling is enabled (the events measure device time of exactly those launches).
+struct ProfScope {
+  bzg_context* ctx;
+  const char* name;
+  hipEvent_t b = nullptr, e = nullptr;
+  ProfScope(bzg_context* c, const char* n) : ctx(c), name(n) {
+    if (ctx && ctx->profile && hipEventCreate(&b) == hipSuccess && hipEventCreate(&e) == hipSuccess)
+      hipEventRecord(b, ctx->stream);
+    else
+      b = e = nullptr;
+  }
+  ~ProfScope() {
+    if (b && e) {
+      hipEventRecord(e, ctx->stream);
+      ctx->prof[name].push_back({b, e});
+    }
+  }
+};
+
 // Returns a cached device buffer of at least `bytes` (slot-indexed); grows on demand.
 bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out);
 

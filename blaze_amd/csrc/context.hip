@@ -58,6 +58,11 @@ bzg_status bzg_create(bzg_context** out) {
 bzg_status bzg_destroy(bzg_context* ctx) {
   if (!ctx) return BZG_OK;
   hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->prof)
+    for (auto& pr : kv.second) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
   for (auto& w : ctx->ws)
     if (w.ptr) hipFree(w.ptr);
   if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
@@ -98,6 +103,32 @@ const char* bzg_status_string(bzg_status s) {
 }
 
 const char* bzg_last_error(void) { return bzg::g_last_error.c_str(); }
+
+bzg_status bzg_profile_enable(bzg_context* ctx, int32_t on) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "bzg_profile_enable: null context");
+  ctx->profile = on != 0;
+  return BZG_OK;
+}
+
+bzg_status bzg_profile_read(bzg_context* ctx, const char* kernel, double* total_ms, int64_t* launches) {
+  BZG_REQUIRE(ctx && kernel && total_ms && launches, BZG_ERR_INVALID_ARGUMENT,
+              "bzg_profile_read: null argument");
+  *total_ms = 0.0;
+  *launches = 0;
+  auto it = ctx->prof.find(kernel);
+  if (it == ctx->prof.end()) return BZG_OK;
+  for (auto& pr : it->second) {
+    BZG_HIP_CHECK(hipEventSynchronize(pr.second));
+    float ms = 0.f;
+    BZG_HIP_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    *total_ms += ms;
+    *launches += 1;
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  it->second.clear();
+  return BZG_OK;
+}
 
 bzg_status bzg_malloc(void** dptr, size_t bytes) {
   BZG_REQUIRE(dptr != nullptr, BZG_ERR_INVALID_ARGUMENT, "bzg_malloc: null output pointer");

@@ -293,6 +293,7 @@ bzg_status launch_tile(bzg_context* ctx, int m, int n, int k, T alpha, const T* 
                        const T* B, long ldb, T beta, T* C, long ldc) {
   const bool vec = ((uintptr_t)A % (2 * sizeof(T)) == 0) && ((uintptr_t)B % (2 * sizeof(T)) == 0) &&
                    (lda % 2 == 0) && (ldb % 2 == 0);
+  ProfScope prof(ctx, sizeof(T) == 8 ? "dgemm" : "sgemm");
   const long tiles = (long)((m + BM - 1) / BM) * ((n + BN - 1) / BN);
   BZG_REQUIRE(tiles < (1L << 31), BZG_ERR_UNSUPPORTED, "gemm: too many tiles");
   dim3 grid((unsigned)tiles), block(kThreads);

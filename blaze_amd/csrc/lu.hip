@@ -639,6 +639,7 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, 
   bzg_status st = bzg::workspace(ctx, 0, 256, &scratch);
   if (st) return st;
   int* counter = static_cast<int*>(scratch);
+  bzg::ProfScope prof(ctx, "dgesv");
   hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counter, 4);
   BZG_LAUNCH_CHECK();
   const int grid = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
@@ -669,6 +670,7 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
   if (!st) st = bzg::workspace(ctx, 3, (size_t)count * sizeof(int32_t), &lst);
   if (st) return st;
   int* counters = static_cast<int*>(cnt);
+  bzg::ProfScope prof(ctx, "dsgesv");
   hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 4);
   BZG_LAUNCH_CHECK();
   hipLaunchKernelGGL(bzg::dsgesv_batched_kernel, dim3(grid32), dim3(bzg::NT), 0, ctx->stream,

@@ -519,6 +519,7 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   Fp64AccPlan pl;
   bzg_status st = plan_fp64acc(k, moduli, mode, pl);
   if (st) return st;
+  ProfScope prof_total(ctx, "fp64acc_total");
   const int esz = mode == 0 ? 2 : 1;
   const int kq = RBYTES / esz;  // K elements per stage
   const long mp = (m + RT - 1) / RT * RT, np = (n + RT - 1) / RT * RT;
@@ -579,6 +580,8 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set[mode] = true;
   }
+  {
+  ProfScope prof_gemm(ctx, "fp64acc_gemm");
   if (mode == 0)
     hipLaunchKernelGGL((residue_gemm_kernel<0, 2>), dim3((unsigned)((mp / RT) * (np / 128)), 1, pl.nmod),
                        dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres, Cres, dcc);
@@ -586,6 +589,7 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
     hipLaunchKernelGGL((residue_gemm_kernel<1, 4>), dim3((unsigned)((mp / RT) * (np / RT)), 1, pl.nmod),
                        dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres, Cres, dcc);
   BZG_LAUNCH_CHECK();
+  }
   const long mn = m * n;
   hipLaunchKernelGGL(crt_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, ctx->stream, (int)m,
                      (int)n, (int)mp, (int)np, pl.nmod, Cres, sA, tB, dcc, C, (long)ldc);

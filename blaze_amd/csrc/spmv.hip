@@ -127,6 +127,7 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
   if (plan->n_rows == 0) return BZG_OK;
   BZG_REQUIRE(rowptr && y, BZG_ERR_INVALID_ARGUMENT, "csrmv: null rowptr or y");
   if (plan->nnz > 0) BZG_REQUIRE(colidx && val && x, BZG_ERR_INVALID_ARGUMENT, "csrmv: null CSR arrays or x");
+  ProfScope prof(ctx, "csrmv");
   if (plan->n_short > 0) {
     const int G = plan->lanes;
     const int64_t blocks = (plan->n_short * G + 255) / 256;

@@ -66,6 +66,14 @@ const char* bzg_status_string(bzg_status s);
 /* Text of the last error detected on this thread (HIP error name, failing argument ...). */
 const char* bzg_last_error(void);
 
+/* Device-time measurement of named kernels with HIP events recorded on the context stream
+ * around their launches: "dgemm", "fp64acc_gemm" (the residue slice GEMMs), "fp64acc_total",
+ * "csrmv", "dgesv", "dsgesv".  bzg_profile_read waits for the recorded events, returns the
+ * summed milliseconds and the number of recorded launches, and clears them. */
+bzg_status bzg_profile_enable(bzg_context* ctx, int32_t on);
+bzg_status bzg_profile_read(bzg_context* ctx, const char* kernel, double* total_ms,
+                            int64_t* launches);
+
 /* Device memory (hipMalloc/hipFree) for hosts that do not include HIP headers. */
 bzg_status bzg_malloc(void** dptr, size_t bytes);
 bzg_status bzg_free(void* dptr);
