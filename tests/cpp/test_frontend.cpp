@@ -1,0 +1,269 @@
+// C++ front-end test: the reference's expression syntax on blaze::gpu containers.
+//
+// Mirrors the reference's own tests: the DynamicMatrix multiplication-assignment known answer
+// (blazetest/src/mathtest/matrices/dynamicmatrix/ClassTest2.cpp:1244-1273), aliasing
+// (operations/dmatdmatmult/AliasingTest.cpp:1803-1850: A = A * B equals the non-aliased
+// result), CompressedMatrix products (matrices/compressedmatrix/ClassTest1.cpp), the LSE test
+// (operations/lse/DenseTest.h:140-200: A*x == b for N = 1..12, relaxed equality 1e-8) and the
+// exception types/messages of operator* and solve().  Every check prints PASS/FAIL; exit code
+// is the number of failures.
+#include <blaze/gpu/Math.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <random>
+#include <string>
+#include <vector>
+
+using namespace blaze::gpu;
+
+static int failures = 0;
+static void check(bool ok, const std::string& name) {
+  std::printf("%s %s\n", ok ? "PASS" : "FAIL", name.c_str());
+  std::fflush(stdout);
+  if (!ok) ++failures;
+}
+template <typename E>
+static bool throws(const std::function<void()>& f, const std::string& msg) {
+  try {
+    f();
+  } catch (const E& e) {
+    return msg.empty() || std::string(e.what()) == msg;
+  } catch (...) {
+    return false;
+  }
+  return false;
+}
+static bool relaxed(double a, double b, double eps = 1e-8) {  // blaze::equal (Equal.h:214-224)
+  return std::fabs(a - b) <= eps * std::max(1.0, std::fabs(b));
+}
+template <typename T>
+static std::vector<T> host_gemm(const std::vector<T>& A, const std::vector<T>& B, size_t m, size_t n, size_t k) {
+  std::vector<T> C(m * n, T(0));
+  for (size_t i = 0; i < m; ++i)
+    for (size_t p = 0; p < k; ++p)
+      for (size_t j = 0; j < n; ++j) C[i * n + j] += A[i * k + p] * B[p * n + j];
+  return C;
+}
+
+int main() {
+  std::mt19937_64 rng(128753984);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  std::uniform_int_distribution<int> I(-10, 10);
+
+  // ClassTest2.cpp:1244-1273
+  {
+    DynamicMatrix<int> mat1{{0, 2, 0, 0}, {1, 3, 0, 4}, {0, 0, 0, 5}};
+    DynamicMatrix<int> mat2{{1, 0, 2}, {0, 3, 0}, {4, 0, 5}};
+    mat2 *= mat1;
+    std::vector<int> got = mat2.toVector(), want{0, 2, 0, 10, 3, 9, 0, 12, 0, 8, 0, 25};
+    check(mat2.rows() == 3 && mat2.columns() == 4 && got == want && mat2.nonZeros() == 7,
+          "DynamicMatrix<int> multiplication assignment known answer");
+  }
+  // AliasingTest: A = A * B equals the non-aliased product
+  {
+    DynamicMatrix<int> A{{-1, 0, -2, 0}, {0, 2, -3, 1}, {0, 1, 2, 2}};
+    DynamicMatrix<int> B{{1, 0, -3}, {0, -1, 0}, {0, 2, 1}, {2, 1, -2}};
+    DynamicMatrix<int> C{{1, 0, 2}, {0, 3, -1}, {-1, 0, 2}};
+    DynamicMatrix<int> ref;
+    ref = A * B;
+    DynamicMatrix<int> A2(A);
+    A2 = A2 * B;  // aliased left operand
+    check(A2.toVector() == ref.toVector(), "aliased C = C * B (int)");
+    DynamicMatrix<int> ref2;
+    ref2 = C * C;
+    C = C * C;
+    check(C.toVector() == ref2.toVector(), "aliased C = C * C (int)");
+  }
+  // random double / float products against a host triple loop
+  for (size_t m : {1, 7, 37, 64, 130})
+    for (size_t k : {0, 3, 63, 200})
+      for (size_t n : {1, 16, 65}) {
+        std::vector<double> a(m * k), b(k * n);
+        for (auto& v : a) v = U(rng);
+        for (auto& v : b) v = U(rng);
+        DynamicMatrix<double> A(m, k, a.data()), B(k, n, b.data()), C(m, n, 5.0);
+        C = A * B;
+        auto want = host_gemm(a, b, m, n, k);
+        auto got = C.toVector();
+        bool ok = C.rows() == m && C.columns() == n;
+        for (size_t q = 0; q < m * n && ok; ++q) ok = relaxed(got[q], want[q]);
+        check(ok, "C = A * B double " + std::to_string(m) + "x" + std::to_string(n) + "x" + std::to_string(k));
+        std::vector<float> af(a.begin(), a.end()), bf(b.begin(), b.end());
+        DynamicMatrix<float> Af(m, k, af.data()), Bf(k, n, bf.data()), Cf;
+        Cf = Af * Bf;
+        auto gf = Cf.toVector();
+        auto wf = host_gemm(af, bf, m, n, k);
+        ok = true;
+        for (size_t q = 0; q < m * n && ok; ++q) ok = std::fabs(gf[q] - wf[q]) <= 1e-5f * (1 + std::fabs(wf[q]));
+        check(ok, "C = A * B float " + std::to_string(m) + "x" + std::to_string(n) + "x" + std::to_string(k));
+      }
+  // compound and scaled forms
+  {
+    const size_t m = 40, n = 30, k = 20;
+    std::vector<double> a(m * k), b(k * n), c(m * n);
+    for (auto& v : a) v = U(rng);
+    for (auto& v : b) v = U(rng);
+    for (auto& v : c) v = U(rng);
+    DynamicMatrix<double> A(m, k, a.data()), B(k, n, b.data()), C(m, n, c.data()), D(m, n, c.data()), E;
+    C += A * B;
+    D -= A * B;
+    E = 2.0 * (A * B);
+    auto p = host_gemm(a, b, m, n, k);
+    auto gc = C.toVector(), gd = D.toVector(), ge = E.toVector();
+    bool ok = true;
+    for (size_t q = 0; q < m * n; ++q)
+      ok = ok && relaxed(gc[q], c[q] + p[q]) && relaxed(gd[q], c[q] - p[q]) && relaxed(ge[q], 2 * p[q]);
+    check(ok, "C += A*B, C -= A*B, C = 2*(A*B)");
+  }
+  // K == 0 resets the target (DMatDMatMultExpr.h:527-530)
+  {
+    DynamicMatrix<double> A(3, 0), B(0, 4), C(3, 4, 7.0);
+    C = A * B;
+    bool ok = C.rows() == 3 && C.columns() == 4;
+    for (double v : C.toVector()) ok = ok && v == 0.0;
+    check(ok, "K == 0 product resets the target");
+  }
+  // exceptions of operator*
+  {
+    DynamicMatrix<double> A(3, 4), B(5, 2);
+    DynamicVector<double> x(7);
+    check(throws<std::invalid_argument>([&] { DynamicMatrix<double> C; C = A * B; }, "Matrix sizes do not match"),
+          "A * B size mismatch throws invalid_argument");
+    check(throws<std::invalid_argument>([&] { DynamicVector<double> y; y = A * x; },
+                                        "Matrix and vector sizes do not match"),
+          "A * x size mismatch throws invalid_argument");
+  }
+  // GEMV (MDaVDa.cpp shapes) and aliasing x = A * x
+  for (auto mn : std::vector<std::pair<size_t, size_t>>{{0, 0}, {1, 1}, {6, 5}, {67, 127}, {127, 67}, {64, 128}}) {
+    const size_t m = mn.first, n = mn.second;
+    std::vector<int> a(m * n), x(n);
+    for (auto& v : a) v = I(rng);
+    for (auto& v : x) v = I(rng);
+    DynamicMatrix<int> A(m, n, a.data());
+    DynamicVector<int> X(n, x.data()), Y;
+    Y = A * X;
+    auto want = host_gemm(a, x, m, 1, n);
+    check(Y.size() == m && Y.toVector() == want, "y = A * x int " + std::to_string(m) + "x" + std::to_string(n));
+  }
+  {
+    const size_t n = 50;
+    std::vector<double> a(n * n), x(n);
+    for (auto& v : a) v = U(rng);
+    for (auto& v : x) v = U(rng);
+    DynamicMatrix<double> A(n, n, a.data());
+    DynamicVector<double> X(n, x.data());
+    X = A * X;  // aliased
+    auto want = host_gemm(a, x, n, 1, n);
+    auto got = X.toVector();
+    bool ok = true;
+    for (size_t i = 0; i < n; ++i) ok = ok && relaxed(got[i], want[i]);
+    check(ok, "aliased x = A * x");
+  }
+  // CompressedMatrix: append / finalize, then y = S * x
+  {
+    // ( 1 0 2 0 )   ( 1 )   ( 7 )
+    // ( 0 0 0 0 ) * ( 2 ) = ( 0 )
+    // ( 0 3 0 4 )   ( 3 )   ( 22)
+    //               ( 4 )
+    CompressedMatrix<int> S(3, 4, 4);
+    S.append(0, 0, 1); S.append(0, 2, 2); S.finalize(0);
+    S.finalize(1);
+    S.append(2, 1, 3); S.append(2, 3, 4); S.finalize(2);
+    DynamicVector<int> x{1, 2, 3, 4}, y;
+    y = S * x;
+    check(S.nonZeros() == 4 && y.toVector() == std::vector<int>({7, 0, 22}), "CompressedMatrix<int> * vector known answer");
+    check(throws<std::logic_error>([&] { CompressedMatrix<int> T(2, 2); T.append(0, 1, 1); T.append(0, 0, 1); }, ""),
+          "append with non-increasing index throws");
+    DynamicVector<int> z(3);
+    check(throws<std::invalid_argument>([&] { DynamicVector<int> w; w = S * z; }, "Matrix and vector sizes do not match"),
+          "S * x size mismatch throws invalid_argument");
+  }
+  {
+    const size_t m = 300, n = 280;
+    CompressedMatrix<double> S(m, n);
+    std::vector<double> dense(m * n, 0.0), x(n);
+    for (size_t i = 0; i < m; ++i) {
+      for (size_t j = 0; j < n; ++j)
+        if ((i * 7 + j * 3) % 11 == 0 || (i % 50 == 0 && j % 2 == 0)) {
+          const double v = U(rng);
+          S.append(i, j, v);
+          dense[i * n + j] = v;
+        }
+      S.finalize(i);
+    }
+    for (auto& v : x) v = U(rng);
+    DynamicVector<double> X(n, x.data()), Y;
+    Y = S * X;
+    auto got = Y.toVector();
+    auto want = host_gemm(dense, x, m, 1, n);
+    bool ok = true;
+    for (size_t i = 0; i < m; ++i) ok = ok && relaxed(got[i], want[i], 1e-12);
+    check(ok, "CompressedMatrix<double> * vector (mixed short and long rows)");
+  }
+  // solve(): lse/DenseTest.h -- random A with det != 0, A*x == b, N = 0..12 (+ larger)
+  for (size_t n : {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 33, 100, 256}) {
+    std::vector<double> a(n * n), b(n);
+    for (auto& v : a) v = U(rng);
+    for (size_t i = 0; i < n; ++i) a[i * n + i] += 2.0;
+    for (auto& v : b) v = U(rng);
+    DynamicMatrix<double> A(n, n, a.data());
+    DynamicVector<double> B(n, b.data()), x1, x2;
+    solve(A, x1, B);
+    x2 = solve(A, B);
+    DynamicVector<double> Ax;
+    Ax = A * x1;
+    auto ax = Ax.toVector(), v1 = x1.toVector(), v2 = x2.toVector();
+    bool ok = v1 == v2;
+    for (size_t i = 0; i < n; ++i) ok = ok && relaxed(ax[i], b[i]);
+    check(ok, "solve(A,x,b) and x = solve(A,b), N = " + std::to_string(n));
+  }
+  {
+    const size_t n = 20, r = 5;
+    std::vector<double> a(n * n), b(n * r);
+    for (auto& v : a) v = U(rng);
+    for (size_t i = 0; i < n; ++i) a[i * n + i] += 3.0;
+    for (auto& v : b) v = U(rng);
+    DynamicMatrix<double> A(n, n, a.data()), B(n, r, b.data()), X, AX;
+    solve(A, X, B);
+    AX = A * X;
+    auto g = AX.toVector();
+    bool ok = X.rows() == n && X.columns() == r;
+    for (size_t q = 0; q < n * r; ++q) ok = ok && relaxed(g[q], b[q]);
+    check(ok, "solve(A,X,B) with 5 right-hand sides");
+  }
+  {
+    DynamicMatrix<double> S3{{1, 2, 3}, {2, 4, 6}, {0, 1, 1}};
+    DynamicMatrix<double> S9(9, 9, 1.0);
+    DynamicMatrix<double> R(3, 4);
+    DynamicVector<double> b3(3, 1.0), b9(9, 1.0), b4(4, 1.0), x;
+    check(throws<std::runtime_error>([&] { solve(S3, x, b3); }, "Solving LSE with singular system matrix failed"),
+          "singular 3x3 throws runtime_error (closed-form message)");
+    check(throws<std::runtime_error>([&] { solve(S9, x, b9); }, "Inversion of singular matrix failed"),
+          "singular 9x9 throws runtime_error (LAPACK message)");
+    check(throws<std::invalid_argument>([&] { solve(R, x, b3); }, "Invalid non-square system matrix provided"),
+          "non-square system throws invalid_argument");
+    check(throws<std::invalid_argument>([&] { solve(S3, x, b4); }, "Invalid right-hand side vector provided"),
+          "rhs size mismatch throws invalid_argument");
+  }
+  // FP64-accurate GEMM routing
+  {
+    const size_t m = 300, n = 200, k = 700;
+    std::vector<double> a(m * k), b(k * n);
+    for (auto& v : a) v = U(rng);
+    for (auto& v : b) v = U(rng);
+    DynamicMatrix<double> A(m, k, a.data()), B(k, n, b.data()), C1, C2;
+    C1 = A * B;
+    setFp64AccurateGemm(true);
+    C2 = A * B;
+    setFp64AccurateGemm(false);
+    auto g1 = C1.toVector(), g2 = C2.toVector();
+    double num = 0, den = 0;
+    for (size_t q = 0; q < m * n; ++q) { num += (g1[q] - g2[q]) * (g1[q] - g2[q]); den += g1[q] * g1[q]; }
+    check(std::sqrt(num / den) < 1e-14, "FP64-accurate split GEMM agrees with native FP64 (rel. Frobenius < 1e-14)");
+  }
+  std::printf("%d failure(s)\n", failures);
+  return failures;
+}
