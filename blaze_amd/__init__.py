@@ -43,6 +43,7 @@ SIGNATURES = {
     "bzg_igemv": (i32, [vp, i64, i64, i32, vp, i64, vp, i32, vp]),
     "bzg_csr_plan_create": (i32, [vp, i64, i64, vp, C.POINTER(vp)]),
     "bzg_csr_plan_destroy": (i32, [vp]),
+    "bzg_csr_plan_tune": (i32, [vp, i32, i32, i32]),
     "bzg_csr_plan_describe": (i32, [vp, C.c_char_p, C.c_size_t]),
     "bzg_dcsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_scsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),

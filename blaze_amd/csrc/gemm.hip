@@ -76,7 +76,7 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, i
 }
 
 template <typename T, int BM, int BN, int WM, int WN, bool VEC>
-__global__ __launch_bounds__(kThreads) void gemm_mfma_kernel(int M, int N, int K, T alpha,
+__global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, int K, T alpha,
                                                              const T* __restrict__ A, long lda,
                                                              const T* __restrict__ B, long ldb,
                                                              T beta, T* __restrict__ C,

@@ -13,12 +13,13 @@
 //  * wavefront-per-row: rows longer than the threshold get a whole wavefront; lanes stride
 //    by 64 and the 64 partials are combined by a fixed butterfly (deterministic; within the
 //    usual gamma_n bound of the sequential sum).
-// Nonzero values and column indices are streamed with non-temporal loads so that x stays in
-// L2 / MALL.
+// Nonzero values and column indices use ordinary cached loads: a row's chunks and neighbouring
+// rows share cache lines, which non-temporal loads would fetch from HBM once per access.
 #include "common.hpp"
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 // Every product is rounded before it is added: never contract v*x + s into an FMA here.
@@ -27,18 +28,36 @@
 struct bzg_csr_plan {
   int64_t n_rows = 0, n_cols = 0, nnz = 0;
   int lanes = 1;            // lanes per row in the vector kernel
-  int long_threshold = 0;   // rows with more nonzeros go to the wavefront kernel
+  int rows_per_group = 1;   // rows per lane group in the vector kernel
+  int cap = 8;              // nonzeros per lane per batch in the vector kernel
+  int mode = 0;             // short rows: 0 = row blocks through LDS, 1 = lane groups
+  int4* blocks = nullptr;   // device: (row0, row1, nz0, nz1) of each LDS row block
+  int64_t n_blocks = 0;
+  int block_nnz = 2048;     // nonzeros per row block (2048 or 4096)
+  int block_nt = 0;         // non-temporal loads of values / column indices
+  int long_threshold = 0;   // rows with more nonzeros leave the sequential (short-row) path
   int32_t* short_rows = nullptr;  // device row list; nullptr = all rows, contiguous
   int64_t n_short = 0;
-  int32_t* long_rows = nullptr;
+  int32_t* long_rows = nullptr;   // rows handled by one wavefront each
   int64_t n_long = 0;
+  int32_t* huge_rows = nullptr;   // rows handled by one workgroup each
+  int64_t n_huge = 0;
   int64_t max_row = 0;
   double mean_row = 0;
+  std::vector<int32_t> rowptr_host;  // kept for re-planning by the autotuner
   char desc[256] = {0};
 };
 
 namespace bzg {
 namespace {
+
+// Workgroup ids are dispatched round-robin over the 8 XCDs.  Remap them so each XCD walks a
+// contiguous eighth of the rows: neighbouring rows (and, for stencil-like matrices, the x
+// entries they gather) then stay in one XCD's L2.  The grid is padded to a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t b = blockIdx.x, nb = gridDim.x;
+  return (b % 8) * (nb / 8) + b / 8;
+}
 
 // Products and sums with explicit rounding per operation (no contraction into FMA).
 // (Plain operators compiled under `fp contract(off)` above.)
@@ -59,7 +78,13 @@ __device__ __forceinline__ double neg_zero<double>() { return -0.0; }
 template <>
 __device__ __forceinline__ float neg_zero<float>() { return -0.0f; }
 
-template <typename T, int G>
+// G lanes per row, R rows per lane group, CAP nonzeros per lane per batch.  A batch first
+// issues the loads of all R*CAP (value, column) pairs of a lane, then all their x gathers,
+// and only then sums, so a row costs one dependent round trip per G*CAP nonzeros (three in
+// total for a short row: rowptr -> column -> x) instead of one per element.  The products of
+// a batch are added in storage order (lane t of chunk c holds element c*G + t), which keeps
+// every row's sum equal to the reference's sequential sum.
+template <typename T, int G, int R, int CAP>
 __global__ __launch_bounds__(256) void csr_vector_kernel(int64_t n_rows, const int32_t* __restrict__ rows,
                                                          const int32_t* __restrict__ rowptr,
                                                          const int32_t* __restrict__ colidx,
@@ -68,56 +93,197 @@ __global__ __launch_bounds__(256) void csr_vector_kernel(int64_t n_rows, const i
                                                          T* __restrict__ y) {
   const int lig = threadIdx.x % G;
   const int group_base = (threadIdx.x % 64) - lig;  // first lane of this group in the wave
-  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  if (g >= n_rows) return;  // whole groups exit together
-  const int64_t row = rows ? rows[g] : g;
-  const int32_t start = rowptr[row], end = rowptr[row + 1];
-  // Starting from -0 reproduces sp = p0 exactly (-0 + p == p for every p, also p = +-0).
-  T s = neg_zero<T>();
-  for (int32_t base = start; base < end; base += G) {
-    const int32_t e = base + lig;
-    T p = T(0);
-    if (e < end) {
-      const int32_t c = __builtin_nontemporal_load(colidx + e);
-      p = mul_rn(__builtin_nontemporal_load(val + e), x[c]);
-    }
-    const int cnt = min(G, end - base);
+  const int64_t g = (xcd_block() * blockDim.x + threadIdx.x) / G;
+  const int64_t first = g * R;
+  if (first >= n_rows) return;  // whole groups exit together
+  constexpr int BATCH = G * CAP;
+  int64_t row[R];
+  int32_t start[R], len[R];
+  int32_t batches = 0;
 #pragma unroll
-    for (int t = 0; t < G; ++t) {
-      const T q = (G == 1) ? p : __shfl(p, group_base + t, 64);
-      if (t < cnt) s = add_rn(s, q);
+  for (int r = 0; r < R; ++r) {
+    const int64_t q = first + r;
+    row[r] = -1;
+    start[r] = 0;
+    len[r] = 0;
+    if (q < n_rows) {
+      row[r] = rows ? rows[q] : q;
+      start[r] = rowptr[row[r]];
+      len[r] = rowptr[row[r] + 1] - start[r];
     }
+    batches = max(batches, (len[r] + BATCH - 1) / BATCH);
   }
-  if (lig == 0) y[row] = (end == start) ? T(0) : s;
+  // Starting from -0 reproduces sp = p0 exactly (-0 + p == p for every p, also p = +-0).
+  T s[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) s[r] = neg_zero<T>();
+  for (int32_t b = 0; b < batches; ++b) {
+    T v[R][CAP];
+    int32_t c[R][CAP];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < CAP; ++k) {
+        const int32_t o = b * BATCH + k * G + lig;
+        const bool ok = o < len[r];
+        v[r][k] = ok ? val[start[r] + o] : T(0);
+        c[r][k] = ok ? colidx[start[r] + o] : 0;
+      }
+    T p[R][CAP];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < CAP; ++k) p[r][k] = mul_rn(v[r][k], x[c[r][k]]);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < CAP; ++k) {
+        const int32_t base = b * BATCH + k * G;
+        const int cnt = min(G, len[r] - base);
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+          const T q = (G == 1) ? p[r][k] : __shfl(p[r][k], group_base + t, 64);
+          if (t < cnt) s[r] = add_rn(s[r], q);
+        }
+      }
+  }
+  if (lig == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (row[r] >= 0) y[row[r]] = (len[r] == 0) ? T(0) : s[r];
+  }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void csr_wave_kernel(int64_t n_rows, const int32_t* __restrict__ rows,
+// Row blocks through LDS (for short rows): a block is a run of consecutive short rows holding at
+// most BNNZ nonzeros (at most 512 rows); its nonzeros form one contiguous range, read with
+// fully coalesced loads.  The products go to LDS and thread t then adds the products of rows
+// t and t+256 in storage order: the reference's sequential sum, bit for bit.  A block
+// descriptor holds (first row, end row, first nonzero, end nonzero).
+// Persistent and software-pipelined: each workgroup walks a contiguous range of blocks (ranges
+// laid out XCD by XCD) and issues the value/column loads of block i+1 before it gathers x for
+// block i, so the gather latency overlaps the stream.
+constexpr int kBlockRowsMax = 512;
+constexpr int kShortRow = 32;  // rows up to this length are summed sequentially
+
+template <typename T, int BNNZ, bool NT>
+__global__ __launch_bounds__(256) void csr_block_kernel(int64_t n_blocks, const int4* __restrict__ blocks,
+                                                        const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colidx,
+                                                        const T* __restrict__ val,
+                                                        const T* __restrict__ x, T* __restrict__ y) {
+  __shared__ T prod[2][BNNZ];
+  constexpr int PER = BNNZ / 256;
+  const int tid = threadIdx.x;
+  const int64_t G = gridDim.x;
+  const int64_t per = (n_blocks + G - 1) / G;
+  int64_t b = xcd_block() * per;
+  const int64_t bend = min(n_blocks, b + per);
+  if (b >= bend) return;  // uniform over the workgroup
+  T v[PER];
+  int32_t c[PER];
+  auto load = [&](const int4& d) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + 256 * i;
+      const bool ok = e < d.w - d.z;
+      if (NT) {
+        v[i] = ok ? __builtin_nontemporal_load(val + d.z + e) : T(0);
+        c[i] = ok ? __builtin_nontemporal_load(colidx + d.z + e) : 0;
+      } else {
+        v[i] = ok ? val[d.z + e] : T(0);
+        c[i] = ok ? colidx[d.z + e] : 0;
+      }
+    }
+  };
+  int4 d = blocks[b];
+  load(d);
+  int buf = 0;
+  for (; b < bend; ++b) {
+    const int4 cur = d;
+    T p[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) p[i] = mul_rn(v[i], x[c[i]]);
+    const int nrow = cur.y - cur.x;
+    int32_t rs[2], re[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = tid + 256 * h;
+      rs[h] = re[h] = 0;
+      if (t < nrow) {
+        rs[h] = rowptr[cur.x + t] - cur.z;
+        re[h] = rowptr[cur.x + t + 1] - cur.z;
+      }
+    }
+    if (b + 1 < bend) {
+      d = blocks[b + 1];
+      load(d);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + 256 * i;
+      if (e < cur.w - cur.z) prod[buf][e] = p[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = tid + 256 * h;
+      if (t < nrow) {
+        T s = neg_zero<T>();
+        for (int32_t k = rs[h]; k < re[h]; ++k) s = add_rn(s, prod[buf][k]);
+        y[cur.x + t] = (re[h] == rs[h]) ? T(0) : s;
+      }
+    }
+    buf ^= 1;  // the other buffer is rewritten only after the next barrier
+  }
+}
+
+// Long rows: LANES threads per row (64 = one wavefront, 256 = one workgroup).  Each thread
+// issues the loads of up to PER nonzeros at once, then their x gathers, keeping PER partial
+// sums; the partials are combined in a fixed order (lane butterfly, then the 4 wavefronts in
+// order), so the result is deterministic and within gamma_len * sum|v||x| of the sequential
+// sum.
+template <typename T, int LANES, int PER>
+__global__ __launch_bounds__(256) void csr_rows_kernel(int64_t n_rows, const int32_t* __restrict__ rows,
                                                        const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ colidx,
                                                        const T* __restrict__ val,
                                                        const T* __restrict__ x,
                                                        T* __restrict__ y) {
-  const int lane = threadIdx.x % 64;
-  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  if (w >= n_rows) return;
+  __shared__ T part[4];
+  const int tid = threadIdx.x;
+  const int t = tid % LANES;
+  const int64_t w = (xcd_block() * blockDim.x + tid) / LANES;
+  if (w >= n_rows) return;  // uniform per row group (LANES = 256: per workgroup)
   const int64_t row = rows ? rows[w] : w;
   const int32_t start = rowptr[row], end = rowptr[row + 1];
-  T s0 = T(0), s1 = T(0);
-  int32_t e = start + lane;
-  for (; e + 64 < end; e += 128) {
-    const int32_t c0 = __builtin_nontemporal_load(colidx + e);
-    const int32_t c1 = __builtin_nontemporal_load(colidx + e + 64);
-    const T v0 = __builtin_nontemporal_load(val + e);
-    const T v1 = __builtin_nontemporal_load(val + e + 64);
-    s0 = add_rn(s0, mul_rn(v0, x[c0]));
-    s1 = add_rn(s1, mul_rn(v1, x[c1]));
+  T acc[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) acc[k] = T(0);
+  for (int32_t base = start; base < end; base += LANES * PER) {
+    T v[PER];
+    int32_t c[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int32_t e = base + t + LANES * k;
+      const bool ok = e < end;
+      v[k] = ok ? val[e] : T(0);
+      c[k] = ok ? colidx[e] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) acc[k] = add_rn(acc[k], mul_rn(v[k], x[c[k]]));
   }
-  if (e < end) s0 = add_rn(s0, mul_rn(__builtin_nontemporal_load(val + e), x[__builtin_nontemporal_load(colidx + e)]));
-  T s = add_rn(s0, s1);
+  T s = acc[0];
+#pragma unroll
+  for (int k = 1; k < PER; ++k) s = add_rn(s, acc[k]);
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) s = add_rn(s, __shfl_xor(s, off, 64));
-  if (lane == 0) y[row] = s;
+  if (LANES == 64) {
+    if (t == 0) y[row] = s;
+  } else {
+    if ((tid & 63) == 0) part[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) y[row] = add_rn(add_rn(part[0], part[1]), add_rn(part[2], part[3]));
+  }
 }
 
 template <typename T>
@@ -128,35 +294,104 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
   BZG_REQUIRE(rowptr && y, BZG_ERR_INVALID_ARGUMENT, "csrmv: null rowptr or y");
   if (plan->nnz > 0) BZG_REQUIRE(colidx && val && x, BZG_ERR_INVALID_ARGUMENT, "csrmv: null CSR arrays or x");
   ProfScope prof(ctx, "csrmv");
-  if (plan->n_short > 0) {
+  if (plan->n_short > 0 && plan->mode == 0) {
+    // persistent grid: 8 workgroups per CU (multiple of the 8 XCDs), at most one per block
+    const int64_t blocks = std::min<int64_t>((int64_t)ctx->num_cus * 8, (plan->n_blocks + 7) / 8 * 8);
+    BZG_REQUIRE(blocks < (1LL << 31), BZG_ERR_UNSUPPORTED, "csrmv: too many row blocks");
+    if (plan->block_nnz == 4096) {
+      if (plan->block_nt)
+        hipLaunchKernelGGL((csr_block_kernel<T, 4096, true>), dim3((unsigned)blocks), dim3(256), 0,
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+      else
+        hipLaunchKernelGGL((csr_block_kernel<T, 4096, false>), dim3((unsigned)blocks), dim3(256), 0,
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+    } else {
+      if (plan->block_nt)
+        hipLaunchKernelGGL((csr_block_kernel<T, 2048, true>), dim3((unsigned)blocks), dim3(256), 0,
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+      else
+        hipLaunchKernelGGL((csr_block_kernel<T, 2048, false>), dim3((unsigned)blocks), dim3(256), 0,
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+    }
+    BZG_LAUNCH_CHECK();
+  } else if (plan->n_short > 0) {
     const int G = plan->lanes;
-    const int64_t blocks = (plan->n_short * G + 255) / 256;
+    const int R = plan->rows_per_group;
+    const int CP = plan->cap;
+    const int64_t groups = (plan->n_short + R - 1) / R;
+    const int64_t blocks = ((groups * G + 255) / 256 + 7) / 8 * 8;
     BZG_REQUIRE(blocks < (1LL << 31), BZG_ERR_UNSUPPORTED, "csrmv: too many rows");
     dim3 grid((unsigned)blocks), block(256);
-#define BZG_VEC(GG)                                                                        \
-  case GG:                                                                                 \
-    hipLaunchKernelGGL((csr_vector_kernel<T, GG>), grid, block, 0, ctx->stream, plan->n_short, \
-                       plan->short_rows, rowptr, colidx, val, x, y);                       \
-    break;
-    switch (G) {
-      BZG_VEC(1) BZG_VEC(2) BZG_VEC(4) BZG_VEC(8) BZG_VEC(16) BZG_VEC(32)
-      default:
-        set_error("csrmv: bad lanes per row in plan");
-        return BZG_ERR_INVALID_ARGUMENT;
-    }
+#define BZG_VEC(GG, RR, CC)                                                                    \
+  if (G == GG && R == RR && CP == CC) {                                                        \
+    hipLaunchKernelGGL((csr_vector_kernel<T, GG, RR, CC>), grid, block, 0, ctx->stream,         \
+                       plan->n_short, plan->short_rows, rowptr, colidx, val, x, y);             \
+    launched = true;                                                                           \
+  }
+#define BZG_VEC_C(GG, RR) BZG_VEC(GG, RR, 1) BZG_VEC(GG, RR, 2) BZG_VEC(GG, RR, 4) BZG_VEC(GG, RR, 8)
+#define BZG_VEC_R(GG) BZG_VEC_C(GG, 1) BZG_VEC_C(GG, 2)
+    bool launched = false;
+    BZG_VEC_R(1) BZG_VEC_R(2) BZG_VEC_R(4) BZG_VEC_R(8) BZG_VEC_R(16) BZG_VEC_R(32)
+#undef BZG_VEC_R
+#undef BZG_VEC_C
 #undef BZG_VEC
+    if (!launched) {
+      set_error("csrmv: bad lanes/rows per group in plan");
+      return BZG_ERR_INVALID_ARGUMENT;
+    }
     BZG_LAUNCH_CHECK();
   }
   if (plan->n_long > 0) {
-    const int64_t blocks = (plan->n_long * 64 + 255) / 256;
-    hipLaunchKernelGGL((csr_wave_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+    const int64_t blocks = ((plan->n_long * 64 + 255) / 256 + 7) / 8 * 8;
+    hipLaunchKernelGGL((csr_rows_kernel<T, 64, 4>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
                        plan->n_long, plan->long_rows, rowptr, colidx, val, x, y);
+    BZG_LAUNCH_CHECK();
+  }
+  if (plan->n_huge > 0) {
+    const int64_t blocks = (plan->n_huge + 7) / 8 * 8;
+    hipLaunchKernelGGL((csr_rows_kernel<T, 256, 8>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       plan->n_huge, plan->huge_rows, rowptr, colidx, val, x, y);
     BZG_LAUNCH_CHECK();
   }
   return BZG_OK;
 }
 
 }  // namespace
+}  // namespace bzg
+
+namespace bzg {
+// Groups consecutive short rows into LDS row blocks of at most `bnnz` nonzeros and `brows`
+// rows (a long row ends the current block) and uploads the descriptors.
+static bzg_status build_row_blocks(bzg_context* ctx, bzg_csr_plan* p, const int32_t* rowptr_host,
+                                   int bnnz, int brows) {
+  std::vector<int4> bl;
+  const int64_t n_rows = p->n_rows, long_thr = p->long_threshold;
+  int64_t r = 0;
+  while (r < n_rows) {
+    const int64_t len = (int64_t)rowptr_host[r + 1] - rowptr_host[r];
+    if (len > long_thr) { ++r; continue; }
+    const int64_t r0 = r;
+    int64_t nz = 0;
+    while (r < n_rows && r - r0 < brows) {
+      const int64_t l = (int64_t)rowptr_host[r + 1] - rowptr_host[r];
+      if (l > long_thr || nz + l > bnnz) break;
+      nz += l;
+      ++r;
+    }
+    bl.push_back(make_int4((int)r0, (int)r, rowptr_host[r0], rowptr_host[r]));
+  }
+  if (p->blocks) hipFree(p->blocks);
+  p->blocks = nullptr;
+  p->n_blocks = (int64_t)bl.size();
+  p->block_nnz = bnnz;
+  if (p->n_blocks) {
+    BZG_HIP_CHECK(hipMalloc(&p->blocks, bl.size() * sizeof(int4)));
+    hipStream_t st = ctx ? ctx->stream : nullptr;
+    BZG_HIP_CHECK(hipMemcpyAsync(p->blocks, bl.data(), bl.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    BZG_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  return BZG_OK;
+}
 }  // namespace bzg
 
 extern "C" {
@@ -199,55 +434,115 @@ bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
   }
   p->nnz = total;
   p->max_row = max_len;
+  p->rowptr_host.assign(rowptr_host, rowptr_host + n_rows + 1);
   p->mean_row = (double)total / (double)n_rows;
-  // Rows longer than 4 wavefront-widths of work for a 32-lane group go to the wavefront
-  // kernel; the remaining rows pick the lane count nearest their mean length.
-  const int64_t long_thr = 64;
-  int64_t n_long = 0, short_nnz = 0;
-  for (int b = 0; b <= 32; ++b)
-    if ((1LL << b) > long_thr) n_long += hist[b];
+  // Bins by row length: short rows (<= kShortRow) are summed sequentially (bit-identical to the
+  // reference order) in LDS row blocks or lane groups; longer rows get a wavefront each, rows
+  // beyond 256 nonzeros a whole workgroup each (fixed-order tree sums).
+  const int64_t long_thr = bzg::kShortRow, huge_thr = 256;
+  int64_t n_long = 0, n_huge = 0;
+  std::vector<int32_t> sr, lr, hr;
   for (int64_t r = 0; r < n_rows; ++r) {
     const int64_t len = (int64_t)rowptr_host[r + 1] - rowptr_host[r];
-    if (len <= long_thr) short_nnz += len;
+    if (len > huge_thr) hr.push_back((int32_t)r);
+    else if (len > long_thr) lr.push_back((int32_t)r);
+    else sr.push_back((int32_t)r);
   }
-  const int64_t n_short = n_rows - n_long;
-  const double mean_short = n_short ? (double)short_nnz / (double)n_short : 0.0;
+  n_long = (int64_t)lr.size();
+  n_huge = (int64_t)hr.size();
+  const int64_t n_short = (int64_t)sr.size();
+  // 90th percentile of the short rows' lengths (power-of-two histogram bins)
+  int64_t acc90 = 0, L90 = 1;
+  for (int b = 0; b <= 32 && (1LL << b) <= long_thr; ++b) {
+    acc90 += hist[b];
+    if (acc90 * 10 >= n_short * 9) { L90 = 1LL << b; break; }
+  }
+  // lane-group alternative (selectable by the autotuner): G * 8 covers the typical short row
   int G = 1;
-  while (G < 32 && G < mean_short) G *= 2;
+  while (G < 32 && G * 8 < L90) G *= 2;
+  int CP = 1;
+  while (CP < 8 && G * CP < L90) CP *= 2;
   p->lanes = G;
+  p->cap = CP;
+  p->rows_per_group = 1;
   p->long_threshold = (int)long_thr;
   p->n_short = n_short;
   p->n_long = n_long;
-  if (n_long > 0) {
-    std::vector<int32_t> sr, lr;
-    sr.reserve(n_short);
-    lr.reserve(n_long);
-    for (int64_t r = 0; r < n_rows; ++r) {
-      const int64_t len = (int64_t)rowptr_host[r + 1] - rowptr_host[r];
-      (len > long_thr ? lr : sr).push_back((int32_t)r);
-    }
-    if (n_short) BZG_HIP_CHECK(hipMalloc(&p->short_rows, n_short * sizeof(int32_t)));
-    BZG_HIP_CHECK(hipMalloc(&p->long_rows, n_long * sizeof(int32_t)));
-    if (n_short)
-      BZG_HIP_CHECK(hipMemcpyAsync(p->short_rows, sr.data(), n_short * sizeof(int32_t),
-                                   hipMemcpyHostToDevice, ctx->stream));
-    BZG_HIP_CHECK(hipMemcpyAsync(p->long_rows, lr.data(), n_long * sizeof(int32_t),
-                                 hipMemcpyHostToDevice, ctx->stream));
-    BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  p->n_huge = n_huge;
+  {
+    bzg_status bst = bzg::build_row_blocks(ctx, p, rowptr_host, 2048, 256);
+    if (bst) { delete p; return bst; }
   }
+  // Row blocks pay off when short rows come in long runs (a stencil); when long rows break the
+  // short ones into short runs (power-law rows), lane groups over the short-row list do better.
+  p->mode = (p->n_blocks == 0 || n_short >= 32 * p->n_blocks) ? 0 : 1;
+  auto upload_list = [&](const std::vector<int32_t>& v, int32_t** dst) -> bzg_status {
+    BZG_HIP_CHECK(hipMalloc(dst, v.size() * sizeof(int32_t)));
+    BZG_HIP_CHECK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    return BZG_OK;
+  };
+  bzg_status ust = BZG_OK;
+  if (!ust && n_long + n_huge > 0 && n_short > 0) ust = upload_list(sr, &p->short_rows);
+  if (!ust && n_long > 0) ust = upload_list(lr, &p->long_rows);
+  if (!ust && n_huge > 0) ust = upload_list(hr, &p->huge_rows);
+  if (ust) {
+    bzg_csr_plan_destroy(p);
+    return ust;
+  }
+  BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   std::snprintf(p->desc, sizeof p->desc,
-                "rows=%lld nnz=%lld mean=%.2f max=%lld vector-csr lanes=%d rows=%lld; "
-                "wavefront-per-row rows=%lld (len>%lld)",
-                (long long)n_rows, (long long)total, p->mean_row, (long long)max_len, G,
-                (long long)n_short, (long long)n_long, (long long)long_thr);
+                "rows=%lld nnz=%lld mean=%.2f max=%lld %s rows=%lld; "
+                "wavefront-per-row rows=%lld (%lld<len<=%lld); workgroup-per-row rows=%lld (len>%lld)",
+                (long long)n_rows, (long long)total, p->mean_row, (long long)max_len,
+                p->mode == 0 ? (std::string("row-blocks=") + std::to_string(p->n_blocks) + " (lds)").c_str()
+                             : (std::string("vector-csr lanes=") + std::to_string(G) + " rows/group=1 cap=" +
+                                std::to_string(CP)).c_str(),
+                (long long)n_short, (long long)n_long, (long long)long_thr, (long long)huge_thr,
+                (long long)n_huge, (long long)huge_thr);
   *out = p;
+  return BZG_OK;
+}
+
+bzg_status bzg_csr_plan_tune(bzg_csr_plan* p, int32_t lanes, int32_t rows_per_group, int32_t cap) {
+  BZG_REQUIRE(p != nullptr, BZG_ERR_INVALID_ARGUMENT, "csr_plan_tune: null plan");
+  if (lanes == 0) {  // row blocks through LDS: rows_per_group 1 -> 2048 nnz / 256 rows,
+                     // 2 -> 4096 / 512; cap != 0 -> non-temporal nonzero loads
+    BZG_REQUIRE(p->rowptr_host.size() == (size_t)p->n_rows + 1, BZG_ERR_INVALID_ARGUMENT,
+                "csr_plan_tune: plan has no host row pointers");
+    p->mode = 0;
+    p->block_nt = cap != 0;
+    return bzg::build_row_blocks(nullptr, p, p->rowptr_host.data(), rows_per_group == 2 ? 4096 : 2048,
+                            rows_per_group == 2 ? 512 : 256);
+  }
+  p->mode = 1;
+  BZG_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32,
+              BZG_ERR_INVALID_ARGUMENT, "csr_plan_tune: lanes must be 1, 2, 4, 8, 16 or 32");
+  BZG_REQUIRE(rows_per_group == 1 || rows_per_group == 2, BZG_ERR_INVALID_ARGUMENT,
+              "csr_plan_tune: rows_per_group must be 1 or 2");
+  BZG_REQUIRE(cap == 1 || cap == 2 || cap == 4 || cap == 8, BZG_ERR_INVALID_ARGUMENT,
+              "csr_plan_tune: cap must be 1, 2, 4 or 8");
+  p->lanes = lanes;
+  p->rows_per_group = rows_per_group;
+  p->cap = cap;
+  const char* mark = std::strstr(p->desc, " row-blocks=");
+  if (mark) {
+    char tail[256];
+    const char* semi = std::strchr(mark, ';');
+    std::snprintf(tail, sizeof tail, "%s", semi ? semi : "");
+    std::snprintf(p->desc + (mark - p->desc), sizeof p->desc - (mark - p->desc),
+                  " vector-csr lanes=%d rows/group=%d cap=%d rows=%lld%s", lanes, rows_per_group,
+                  cap, (long long)p->n_short, tail);
+  }
   return BZG_OK;
 }
 
 bzg_status bzg_csr_plan_destroy(bzg_csr_plan* p) {
   if (!p) return BZG_OK;
   if (p->short_rows) hipFree(p->short_rows);
+  if (p->blocks) hipFree(p->blocks);
   if (p->long_rows) hipFree(p->long_rows);
+  if (p->huge_rows) hipFree(p->huge_rows);
   delete p;
   return BZG_OK;
 }

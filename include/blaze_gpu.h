@@ -117,6 +117,11 @@ typedef struct bzg_csr_plan bzg_csr_plan;
 bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
                                const int32_t* rowptr_host, bzg_csr_plan** plan);
 bzg_status bzg_csr_plan_destroy(bzg_csr_plan* plan);
+/* Overrides the vector-CSR shape chosen from the histogram: lanes per row in {1,2,4,8,16,32},
+ * rows per lane group in {1,2}, nonzeros loaded per lane per batch in {1,2,4,8}; used by the
+ * autotuner.  Results stay bit-identical (every row is still summed in storage order). */
+bzg_status bzg_csr_plan_tune(bzg_csr_plan* plan, int32_t lanes, int32_t rows_per_group,
+                             int32_t cap);
 /* Describes the plan: writes a short text (kernel choice, lanes per row, blocks) to buf. */
 bzg_status bzg_csr_plan_describe(const bzg_csr_plan* plan, char* buf, size_t len);
 bzg_status bzg_dcsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,

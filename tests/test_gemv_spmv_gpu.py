@@ -90,7 +90,7 @@ def gpu_csrmv(rowptr, col, val, x, n_cols):
     return host(dy)[:n], plan.describe()
 
 
-LONG_ROW = 64  # rows longer than this run on the wavefront-per-row kernel
+LONG_ROW = 32  # rows longer than this use the tree-sum (wavefront / workgroup per row) kernels
 
 
 def check_spmv(got, rp, ci, va, x):
@@ -167,7 +167,7 @@ def test_spmv_laplacian_bit_exact():
         rp, ci, va = laplace(nx)
         x = rng.uniform(-1, 1, nx ** 3)
         got, desc = gpu_csrmv(rp, ci, va, x, nx ** 3)
-        assert "lanes=8" in desc
+        assert "wavefront-per-row rows=0" in desc
         assert np.array_equal(got, oracle.csrmv(rp, ci, va, x)), nx
         # fp32 variant
         got32, _ = gpu_csrmv(rp, ci, va.astype(np.float32), x.astype(np.float32), nx ** 3)
