@@ -28,27 +28,35 @@
 namespace bzg {
 namespace {
 
-constexpr int NT = 256;       // threads per workgroup
-constexpr int NB = 16;        // panel width
+constexpr int NT = 64;        // threads per workgroup: one wavefront per system
+constexpr int NW = NT / 64;   // wavefronts per workgroup
+constexpr int NB = 16;        // largest panel width
+// Panel width per precision: the panel lives in the registers of one wavefront (4 rows x
+// kNB values per lane), 8 doubles or 16 floats keep it at 64 registers.
+template <typename T>
+constexpr int kNB = sizeof(T) == 8 ? 8 : 16;
 constexpr int NMAX = 256;     // largest system order
 constexpr int RMAX = 8;       // largest number of right-hand sides
 constexpr int UPAD = 16;      // row padding of the U12 buffer (conflict-free MFMA B reads)
 
 template <typename T>
 struct FactorShm {
-  T panel[NMAX][NB + 1];      // L11 / L21 of the current panel (rows beyond n are zero)
-  T ublk[NB][NMAX + UPAD];    // U12 of the current panel
+  T panel[NMAX][kNB<T> + 1];      // L11 / L21 of the current panel (rows beyond n are zero)
+  T ublk[kNB<T>][NMAX + UPAD];    // U12 of the current panel
 };
 
 template <typename T>
 struct SolveShm {
   T x[NMAX * RMAX];           // right-hand sides / solution, row-major, ld = RMAX
+  T diag[16 * 17];            // diagonal block being solved
 };
 
 struct Misc {
-  double pv[4];
-  int pr[4];
+  double pv[2][4];            // per-wave best |a| of the column (double-buffered by step parity)
+  int pr[2][4];               // ... and its row
   int piv[NB];                // absolute pivot rows of the current panel (0-based)
+  int swsrc[2 * NB];          // interchange sources: rows j0.. (first NB), pivot rows (last NB)
+  int swdst[2 * NB];          // ... and their destinations (-1: none)
   int perm[NMAX];             // row i of the factored system = original row perm[i]
   int sys;
   unsigned long long cmax[RMAX * 2];  // column maxima (bit patterns of non-negative doubles)
@@ -77,14 +85,26 @@ struct MfmaLU<float> {
 template <typename T>
 __device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
 
+// Value of lane `src` (wave-uniform) broadcast to the wavefront.
+__device__ __forceinline__ double bcast(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
 // LU factorisation of the n x n row-major matrix A (leading dimension lda) in place.
 // Writes 1-based pivots to ipiv (may be null) and the row permutation to misc.perm.
 // Returns info (0 or 1-based index of the first exactly-zero pivot).
 template <typename T>
-__device__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict__ ipiv,
+__device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict__ ipiv,
                          FactorShm<T>& sh, Misc& misc) {
   using MF = MfmaLU<T>;
   using acc_t = typename MF::acc_t;
+  constexpr int NB = kNB<T>;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int info = 0;
   for (int i = tid; i < n; i += NT) misc.perm[i] = i;
@@ -93,119 +113,143 @@ __device__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict_
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int m = n - j0;
     const int nbe = min(NB, m);
-    const bool rin = tid < m;
-    T prow[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-      prow[c] = (rin && c < nbe) ? A[(long)(j0 + tid) * lda + j0 + c] : T(0);
 
-    // ---- panel factorisation (columns j0 .. j0+nbe-1) ----
+    // ---- panel factorisation (columns j0 .. j0+nbe-1) by wavefront 0 alone ----
+    // Lane l holds panel rows l, l+64, l+128, l+192 in registers.  Rows never move between
+    // registers: each keeps its logical position lpos (its row index after the interchanges
+    // so far), an interchange just swaps two lpos values, and the rows are written to their
+    // final positions at the end.  A column step is a per-lane max over its 4 rows, a
+    // butterfly over the wavefront and v_readlane broadcasts of the pivot row: no LDS traffic
+    // and no barrier per column.
+    if (wave == 0) {
+      T P[4][NB];
+      int lpos[4];
 #pragma unroll
-    for (int jj = 0; jj < NB; ++jj) {
-      if (jj >= nbe) break;
-      // Candidates: rows jj..m-1; others -1.  A NaN entry ranks below every number (as in
-      // idamax, where NaN never compares greater) but above the non-candidates, so the
-      // chosen pivot row is always a row of the matrix.
-      double v = (rin && tid >= jj) ? (double)tabs(prow[jj]) : -1.0;
-      if (v != v) v = -0.5;
-      int r = tid;
+      for (int q = 0; q < 4; ++q) {
+        const int rr = lane + 64 * q;
+        lpos[q] = (rr < m) ? rr : (1 << 20);  // rows beyond the matrix never become candidates
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(v, off, 64);
-        const int orr = __shfl_xor(r, off, 64);
-        if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; }
+        for (int c = 0; c < NB; ++c)
+          P[q][c] = (rr < m && c < nbe) ? A[(long)(j0 + rr) * lda + j0 + c] : T(0);
       }
-      if (lane == 0) { misc.pv[wave] = v; misc.pr[wave] = r; }
-      __syncthreads();
-      double bv = misc.pv[0];
-      int p = misc.pr[0];
 #pragma unroll
-      for (int w = 1; w < 4; ++w)
-        if (misc.pv[w] > bv) { bv = misc.pv[w]; p = misc.pr[w]; }
-      T* rowP = &sh.ublk[0][0];          // scratch rows (ublk is free during the panel)
-      T* rowJ = &sh.ublk[1][0];
-      if (tid == p) {
+      for (int jj = 0; jj < NB; ++jj) {
+        if (jj < nbe) {
+          // Candidates: logical rows jj..m-1.  A NaN entry ranks below every number (as in
+          // idamax, where NaN never compares greater) but above the non-candidates, so the
+          // pivot is always a row of the matrix.  Ties: the lower logical row.
+          double v = -1.0;
+          int r = 1 << 30, wq = 0;
 #pragma unroll
-        for (int c = 0; c < NB; ++c) rowP[c] = prow[c];
-      }
-      if (tid == jj && p != jj) {
+          for (int q = 0; q < 4; ++q) {
+            double a = (lpos[q] >= jj && lpos[q] < m) ? (double)tabs(P[q][jj]) : -1.0;
+            if (a != a) a = -0.5;
+            if (a > v || (a == v && lpos[q] < r)) { v = a; r = lpos[q]; wq = q; }
+          }
+          int wl = lane;
 #pragma unroll
-        for (int c = 0; c < NB; ++c) rowJ[c] = prow[c];
-      }
-      __syncthreads();
-      const T piv = rowP[jj];
-      if (piv != T(0)) {
-        if (tid == 0) {
-          misc.piv[jj] = j0 + p;
-          const int t = misc.perm[j0 + jj];
-          misc.perm[j0 + jj] = misc.perm[j0 + p];
-          misc.perm[j0 + p] = t;
-        }
-        if (p != jj) {
-          if (tid == jj) {
+          for (int off = 32; off >= 1; off >>= 1) {
+            const double ov = __shfl_xor(v, off, 64);
+            const int orr = __shfl_xor(r, off, 64);
+            const int owl = __shfl_xor(wl, off, 64), owq = __shfl_xor(wq, off, 64);
+            if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; wl = owl; wq = owq; }
+          }
+          const int p = __builtin_amdgcn_readfirstlane(r);
+          const int pl = __builtin_amdgcn_readfirstlane(wl), pq = __builtin_amdgcn_readfirstlane(wq);
+          // pivot row values (physical row (pl, pq)), kept in scalar registers
+          T pr[NB];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) prow[c] = rowP[c];
-          } else if (tid == p) {
+          for (int c = 0; c < NB; ++c) {
+            T cand = P[0][c];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) prow[c] = rowJ[c];
+            for (int q = 1; q < 4; ++q) cand = (pq == q) ? P[q][c] : cand;
+            pr[c] = bcast(cand, pl);
+          }
+          const T piv = pr[jj];
+          if (piv != T(0)) {
+            if (lane == 0) {
+              misc.piv[jj] = j0 + p;
+              const int t = misc.perm[j0 + jj];
+              misc.perm[j0 + jj] = misc.perm[j0 + p];
+              misc.perm[j0 + p] = t;
+            }
+            // interchange logical rows jj and p
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              lpos[q] = (lpos[q] == jj) ? p : ((lpos[q] == p) ? jj : lpos[q]);
+            const T rinv = T(1) / piv;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (lpos[q] > jj && lpos[q] < m) {
+                const T l = P[q][jj] * rinv;
+                P[q][jj] = l;
+                // constant trip count (c > jj folds after the jj loop is unrolled): a loop
+                // starting at jj + 1 would be unrolled first with a run-time trip count and
+                // index P dynamically
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                  if (c > jj) P[q][c] -= l * pr[c];
+              }
+            }
+          } else {
+            // Exactly-zero pivot: no interchange, no scaling (dgetf2); the column below is
+            // zero, so the rank-1 update is a no-op.
+            if (lane == 0) misc.piv[jj] = j0 + jj;
+            if (info == 0) info = j0 + jj + 1;
           }
         }
-        if (rin && tid > jj) {
-          const T l = prow[jj] * (T(1) / piv);
-          prow[jj] = l;
-#pragma unroll
-          for (int c = jj + 1; c < NB; ++c) prow[c] -= l * rowP[c];
-        }
-      } else {
-        // Exactly-zero pivot: no interchange, no scaling (dgetf2); the column below is zero,
-        // so the rank-1 update is a no-op.
-        if (tid == 0) misc.piv[jj] = j0 + jj;
-        if (info == 0) info = j0 + jj + 1;
       }
-      // rowP/rowJ are rewritten only after the next step's first barrier: safe.
+      // ---- panel to HBM and LDS, each row at its final (logical) position ----
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = lane + 64 * q;
+        const int lr = lpos[q];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          if (rr < m) {
+            if (c < nbe) A[(long)(j0 + lr) * lda + j0 + c] = P[q][c];
+            sh.panel[lr][c] = P[q][c];
+          } else {
+            sh.panel[rr][c] = T(0);
+          }
+        }
+      }
+      if (lane == 0) misc.flag = info;
     }
     __syncthreads();
+    info = misc.flag;  // wavefront 0 tracked the first zero pivot
     if (ipiv && tid < nbe) ipiv[j0 + tid] = misc.piv[tid] + 1;
 
-    // ---- panel to HBM and LDS ----
-    if (rin) {
-#pragma unroll
-      for (int c = 0; c < NB; ++c) {
-        if (c < nbe) A[(long)(j0 + tid) * lda + j0 + c] = prow[c];
-        sh.panel[tid][c] = prow[c];
+    // ---- interchanges on the other columns, then U12 = L11^-1 A12 ----
+    // Source row of every position the panel's interchanges touch (rows j0..j0+nbe-1 and the
+    // pivot rows below them), found by following the swaps backwards; computed once by 32
+    // threads, then every column gathers with one batch of loads.
+    if (tid < 2 * NB) {
+      const int q = tid % NB;
+      int pos = -1;
+      if (q < nbe) pos = (tid < NB) ? j0 + q : misc.piv[q];
+      if (tid >= NB && pos < j0 + nbe) pos = -1;  // low list: only pivot rows below the block
+      const int dst = pos;
+      if (pos >= 0) {
+        for (int jj = nbe - 1; jj >= 0; --jj) {
+          const int pj = misc.piv[jj];
+          if (pos == j0 + jj) pos = pj;
+          else if (pos == pj) pos = j0 + jj;
+        }
       }
-    } else if (tid < NMAX) {
-#pragma unroll
-      for (int c = 0; c < NB; ++c) sh.panel[tid][c] = T(0);
+      misc.swsrc[tid] = pos;
+      misc.swdst[tid] = dst;
     }
     __syncthreads();
-
-    // ---- interchanges on the other columns, then U12 = L11^-1 A12 ----
     const int other = n - nbe;
-    if (tid < other) {
-      const int c = tid < j0 ? tid : tid + nbe;
-      // Source of each touched position: follow the panel's swaps backwards.
-      int pv[NB];
-#pragma unroll
-      for (int jj = 0; jj < NB; ++jj) pv[jj] = (jj < nbe) ? misc.piv[jj] : (j0 + jj);
-      auto src_of = [&](int pos) {
-#pragma unroll
-        for (int jj = NB - 1; jj >= 0; --jj) {
-          if (jj >= nbe) continue;
-          if (pos == j0 + jj) pos = pv[jj];
-          else if (pos == pv[jj]) pos = j0 + jj;
-        }
-        return pos;
-      };
+    for (int oc = tid; oc < other; oc += NT) {
+      const int c = oc < j0 ? oc : oc + nbe;
       T top[NB], low[NB];
 #pragma unroll
       for (int jj = 0; jj < NB; ++jj) {
-        top[jj] = T(0);
-        low[jj] = T(0);
-        if (jj < nbe) {
-          top[jj] = A[(long)src_of(j0 + jj) * lda + c];
-          if (pv[jj] >= j0 + nbe) low[jj] = A[(long)src_of(pv[jj]) * lda + c];
-        }
+        const int st = misc.swsrc[jj], sl = misc.swsrc[NB + jj];
+        top[jj] = (st >= 0) ? A[(long)st * lda + c] : T(0);
+        low[jj] = (sl >= 0) ? A[(long)sl * lda + c] : T(0);
       }
       if (c > j0) {  // right of the panel: U12 column
 #pragma unroll
@@ -218,10 +262,9 @@ __device__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict_
       }
 #pragma unroll
       for (int jj = 0; jj < NB; ++jj) {
-        if (jj < nbe) {
-          A[(long)(j0 + jj) * lda + c] = top[jj];
-          if (pv[jj] >= j0 + nbe) A[(long)pv[jj] * lda + c] = low[jj];
-        }
+        if (jj < nbe) A[(long)(j0 + jj) * lda + c] = top[jj];
+        const int dl = misc.swdst[NB + jj];
+        if (dl >= 0) A[(long)dl * lda + c] = low[jj];
       }
     }
     // zero the unused tail of the U12 buffer (columns beyond n) for the MFMA tiles
@@ -236,27 +279,47 @@ __device__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict_
     __syncthreads();
 
     // ---- trailing update A22 -= L21 * U12 (MFMA 16x16x4) ----
+    // Each wavefront takes its tiles in batches of TB: all loads of a batch are issued before
+    // the MFMAs, so a batch costs one HBM/L2 round trip instead of one per tile.
     if (rem > 0) {
+      constexpr int TB = sizeof(T) == 8 ? 4 : 8;
       const int tiles = (rem + 15) / 16;
       const int r0 = j0 + NB;
-      for (int t = wave; t < tiles * tiles; t += 4) {
-        const int ti = t / tiles, tj = t % tiles;
-        acc_t acc;
+      const int total = tiles * tiles;
+      for (int t0 = wave; t0 < total; t0 += NW * TB) {
+        acc_t acc[TB];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int gr = r0 + ti * 16 + MF::row(i, lane), gc = r0 + tj * 16 + lane % 16;
-          acc[i] = (gr < n && gc < n) ? A[(long)gr * lda + gc] : T(0);
+        for (int q = 0; q < TB; ++q) {
+          const int t = t0 + NW * q;
+          const int ti = t / tiles, tj = t % tiles;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int gr = r0 + ti * 16 + MF::row(i, lane), gc = r0 + tj * 16 + lane % 16;
+            acc[q][i] = (t < total && gr < n && gc < n) ? A[(long)gr * lda + gc] : T(0);
+          }
         }
 #pragma unroll
-        for (int kk = 0; kk < NB; kk += 4) {
-          const T a = -sh.panel[NB + ti * 16 + lane % 16][kk + lane / 16];
-          const T b = sh.ublk[kk + lane / 16][tj * 16 + lane % 16];
-          acc = MF::mma(a, b, acc);
+        for (int q = 0; q < TB; ++q) {
+          const int t = t0 + NW * q;
+          if (t < total) {
+            const int ti = t / tiles, tj = t % tiles;
+#pragma unroll
+            for (int kk = 0; kk < NB; kk += 4) {
+              const T a = -sh.panel[NB + ti * 16 + lane % 16][kk + lane / 16];
+              const T b = sh.ublk[kk + lane / 16][tj * 16 + lane % 16];
+              acc[q] = MF::mma(a, b, acc[q]);
+            }
+          }
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int gr = r0 + ti * 16 + MF::row(i, lane), gc = r0 + tj * 16 + lane % 16;
-          if (gr < n && gc < n) A[(long)gr * lda + gc] = acc[i];
+        for (int q = 0; q < TB; ++q) {
+          const int t = t0 + NW * q;
+          const int ti = t / tiles, tj = t % tiles;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int gr = r0 + ti * 16 + MF::row(i, lane), gc = r0 + tj * 16 + lane % 16;
+            if (t < total && gr < n && gc < n) A[(long)gr * lda + gc] = acc[q][i];
+          }
         }
       }
     }
@@ -265,61 +328,103 @@ __device__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict_
   return info;
 }
 
-// Dot products of 16 consecutive rows of M (row0 .. row0+nrows-1) with the columns of the
-// LDS matrix X (ld RMAX) over k in [kb, ke):  out(r, c) = sum_k M(row0+r, k) * X(k, c).
-// Thread t handles row t/16 and k = kb + t%16 (mod 16); the 16 partials are combined by a
-// fixed butterfly.  Returns the sum in every lane of the row's 16-lane group.
+// Dot products of up to 16 consecutive rows of M (row0 .. row0+nrows-1) with the columns of
+// the LDS matrix X (ld RMAX) over k in [kb, ke):  out(r, c) = sum_k M(row0+r, k) * X(k, c).
+// Thread t handles rows t/16 + 4j (j < 4) and k = kb + t%16 (mod 16): the loads of all 16
+// rows are in flight together.  The 16 partials of a row are combined by a fixed butterfly;
+// every lane of a 16-lane group ends with the row's sums.
 template <typename TM, typename TX, typename TA>
 __device__ __forceinline__ void rows_dot(const TM* __restrict__ M, int lda, int row0, int nrows,
                                          int kb, int ke, const TX* __restrict__ X, int nrhs,
-                                         TA (&acc)[RMAX]) {
+                                         TA (&acc)[4][RMAX]) {
   const int tid = threadIdx.x;
-  const int r = tid / 16, ks = tid % 16;
+  const int r0 = tid / 16, ks = tid % 16;
 #pragma unroll
-  for (int c = 0; c < RMAX; ++c) acc[c] = TA(0);
-  if (r < nrows) {
-    const TM* row = M + (long)(row0 + r) * lda;
-    for (int k = kb + ks; k < ke; k += 16) {
-      const TA a = (TA)row[k];
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int c = 0; c < RMAX; ++c)
-        if (c < nrhs) acc[c] += a * (TA)X[k * RMAX + c];
+    for (int c = 0; c < RMAX; ++c) acc[j][c] = TA(0);
+  // four k-steps (16 rows x 64 values) of loads in flight before the multiply-adds
+  for (int k0 = kb + ks; k0 < ke; k0 += 64) {
+    TA a[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = r0 + 4 * j, k = k0 + 16 * u;
+        a[u][j] = (r < nrows && k < ke) ? (TA)M[(long)(row0 + r) * lda + k] : TA(0);
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 16 * u;
+      if (k < ke) {
+#pragma unroll
+        for (int c = 0; c < RMAX; ++c) {
+          if (c < nrhs) {
+            const TA xv = (TA)X[k * RMAX + c];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[j][c] += a[u][j] * xv;
+          }
+        }
+      }
     }
   }
 #pragma unroll
-  for (int c = 0; c < RMAX; ++c) {
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
-  }
+    for (int c = 0; c < RMAX; ++c) {
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) acc[j][c] += __shfl_xor(acc[j][c], off, 64);
+    }
 }
 
 // Solves (L U) X = X in place for the factored n x n matrix LU (row-major, unit-lower L),
-// X in LDS (n x nrhs, ld RMAX), already permuted (X = P * B).
+// X in LDS (n x nrhs, ld RMAX), already permuted (X = P * B).  By 16-row blocks: the loads of
+// the off-diagonal dot products and of the 16 x 16 diagonal block are issued together (one
+// round trip per block), the diagonal block goes to LDS (`diag`, 16 x 17), and one thread per
+// right-hand side then solves the block from registers.
 template <typename T>
-__device__ void lu_solve(const T* __restrict__ LU, int n, int lda, T* X, int nrhs) {
+__device__ __forceinline__ void lu_solve(const T* __restrict__ LU, int n, int lda, T* X, int nrhs, T* diag) {
   const int tid = threadIdx.x;
-  const int r = tid / 16, ks = tid % 16;
+  const int r0 = tid / 16, ks = tid % 16;
+  auto stage_block = [&](int b0, int nr, int kb, int ke) {
+    T dv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + 4 * j;
+      dv[j] = (r < nr && ks < nr) ? LU[(long)(b0 + r) * lda + b0 + ks] : T(0);
+    }
+    T acc[4][RMAX];
+    rows_dot<T, T, T>(LU, lda, b0, nr, kb, ke, X, nrhs, acc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + 4 * j;
+      diag[r * 17 + ks] = dv[j];
+      if (r < nr && ks == 0 && ke > kb) {
+#pragma unroll
+        for (int c = 0; c < RMAX; ++c)
+          if (c < nrhs) X[(b0 + r) * RMAX + c] -= acc[j][c];
+      }
+    }
+    __syncthreads();
+  };
   // forward: L y = b
   for (int b0 = 0; b0 < n; b0 += 16) {
     const int nr = min(16, n - b0);
-    if (b0 > 0) {
-      T acc[RMAX];
-      rows_dot<T, T, T>(LU, lda, b0, nr, 0, b0, X, nrhs, acc);
-      if (r < nr && ks == 0) {
-#pragma unroll
-        for (int c = 0; c < RMAX; ++c)
-          if (c < nrhs) X[(b0 + r) * RMAX + c] -= acc[c];
-      }
-      __syncthreads();
-    }
+    stage_block(b0, nr, 0, b0);
     if (tid < nrhs) {
-      const int c = tid;
-      for (int i = 1; i < nr; ++i) {
-        const T* row = LU + (long)(b0 + i) * lda + b0;
-        T s = X[(b0 + i) * RMAX + c];
-        for (int k = 0; k < i; ++k) s -= row[k] * X[(b0 + k) * RMAX + c];
-        X[(b0 + i) * RMAX + c] = s;
+      T xb[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) xb[i] = (i < nr) ? X[(b0 + i) * RMAX + tid] : T(0);
+#pragma unroll
+      for (int i = 1; i < 16; ++i) {
+        T s = xb[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= diag[i * 17 + k] * xb[k];
+        xb[i] = s;
       }
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < nr) X[(b0 + i) * RMAX + tid] = xb[i];
     }
     __syncthreads();
   }
@@ -327,24 +432,24 @@ __device__ void lu_solve(const T* __restrict__ LU, int n, int lda, T* X, int nrh
   const int last = ((n - 1) / 16) * 16;
   for (int b0 = last; b0 >= 0; b0 -= 16) {
     const int nr = min(16, n - b0);
-    if (b0 + 16 < n) {
-      T acc[RMAX];
-      rows_dot<T, T, T>(LU, lda, b0, nr, b0 + 16, n, X, nrhs, acc);
-      if (r < nr && ks == 0) {
-#pragma unroll
-        for (int c = 0; c < RMAX; ++c)
-          if (c < nrhs) X[(b0 + r) * RMAX + c] -= acc[c];
-      }
-      __syncthreads();
-    }
+    stage_block(b0, nr, b0 + 16, n);
     if (tid < nrhs) {
-      const int c = tid;
-      for (int i = nr - 1; i >= 0; --i) {
-        const T* row = LU + (long)(b0 + i) * lda + b0;
-        T s = X[(b0 + i) * RMAX + c];
-        for (int k = i + 1; k < nr; ++k) s -= row[k] * X[(b0 + k) * RMAX + c];
-        X[(b0 + i) * RMAX + c] = s / row[i];
+      T xb[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) xb[i] = (i < nr) ? X[(b0 + i) * RMAX + tid] : T(0);
+#pragma unroll
+      for (int i = 15; i >= 0; --i) {
+        if (i < nr) {
+          T s = xb[i];
+#pragma unroll
+          for (int k = i + 1; k < 16; ++k)
+            if (k < nr) s -= diag[i * 17 + k] * xb[k];
+          xb[i] = s / diag[i * 17 + i];
+        }
       }
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < nr) X[(b0 + i) * RMAX + tid] = xb[i];
     }
     __syncthreads();
   }
@@ -395,7 +500,7 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32
         sh.s.x[e] = (c < nrhs) ? b[(long)misc.perm[i] * nrhs + c] : T(0);
       }
       __syncthreads();
-      lu_solve<T>(a, n, n, sh.s.x, nrhs);
+      lu_solve<T>(a, n, n, sh.s.x, nrhs, sh.s.diag);
       for (int e = tid; e < n * nrhs; e += NT) b[e] = sh.s.x[(e / nrhs) * RMAX + e % nrhs];
     }
     __syncthreads();
@@ -407,6 +512,7 @@ struct IRShm {
   double x[NMAX * RMAX];
   double r[NMAX * RMAX];
   float d[NMAX * RMAX];
+  float diag[16 * 17];
 };
 
 __device__ __forceinline__ void col_max_reset(Misc& misc) {
@@ -443,19 +549,37 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
     if (tid < 2 * RMAX) misc.cmax[tid] = 0ull;
     __syncthreads();
     {
-      const int r = tid / 16, ks = tid % 16;
-      for (int rb = 0; rb < n; rb += 16) {
-        double rs = 0.0;
-        if (rb + r < n)
-          for (int k = ks; k < n; k += 16) {
-            const double v = a[(long)(rb + r) * n + k];
-            W[(long)(rb + r) * n + k] = (float)v;
-            rs += fabs(v);
+      // 16 lanes per row, 4 rows per step, up to 16 loads per lane in flight; the row
+      // sums of |a| use a fixed order (lane-strided partial sums, then a butterfly) so
+      // ||A||_inf and with it the stopping test are reproducible run to run
+      const int r0 = tid / 16, ks = tid % 16;
+      double mx = 0.0;
+      for (int rb = 0; rb < n; rb += NT / 16) {
+        const int r = rb + r0;
+        double sum = 0.0;
+        if (r < n) {
+          const double* ar = a + (long)r * n;
+          float* wr = W + (long)r * n;
+          double v[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            const int k = ks + 16 * u;
+            v[u] = (k < n) ? ar[k] : 0.0;
           }
 #pragma unroll
-        for (int off = 8; off >= 1; off >>= 1) rs += __shfl_xor(rs, off, 64);
-        if (ks == 0 && rb + r < n) atomicMax(&misc.cmax[2 * RMAX - 1], (unsigned long long)__double_as_longlong(rs));
+          for (int u = 0; u < 16; ++u) {
+            const int k = ks + 16 * u;
+            if (k < n) wr[k] = (float)v[u];
+            sum += fabs(v[u]);
+          }
+        }
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        mx = fmax(mx, sum);
       }
+#pragma unroll
+      for (int off = 32; off >= 16; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+      if (tid == 0) misc.cmax[2 * RMAX - 1] = (unsigned long long)__double_as_longlong(mx);
     }
     __syncthreads();
     const double anrm = __longlong_as_double((long long)misc.cmax[2 * RMAX - 1]);
@@ -471,7 +595,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
         sh.ir.d[e] = (c < nrhs) ? (float)b[(long)misc.perm[i] * nrhs + c] : 0.0f;
       }
       __syncthreads();
-      lu_solve<float>(W, n, n, sh.ir.d, nrhs);
+      lu_solve<float>(W, n, n, sh.ir.d, nrhs, sh.ir.diag);
       for (int e = tid; e < n * RMAX; e += NT) sh.ir.x[e] = (double)sh.ir.d[e];
       __syncthreads();
       double prev = 1e308;
@@ -483,20 +607,24 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
         __syncthreads();
         for (int rb = 0; rb < n; rb += 16) {
           const int nr = min(16, n - rb);
-          double acc[RMAX];
+          double acc[4][RMAX];
           rows_dot<double, double, double>(a, n, rb, nr, 0, n, sh.ir.x, nrhs, acc);
-          const int r = tid / 16, ks = tid % 16;
-          if (r < nr && ks < nrhs) {
-            // lane ks of the row group writes column ks (all lanes hold every sum)
-            double v = acc[0];
+          const int r0 = tid / 16, ks = tid % 16;
 #pragma unroll
-            for (int c = 1; c < RMAX; ++c)
-              if (c == ks) v = acc[c];
-            const double rv2 = b[(long)(rb + r) * nrhs + ks] - v;
-            sh.ir.r[(rb + r) * RMAX + ks] = rv2;
-            atomicMax(&misc.cmax[ks], (unsigned long long)__double_as_longlong(fabs(rv2)));
-            atomicMax(&misc.cmax[RMAX + ks],
-                      (unsigned long long)__double_as_longlong(fabs(sh.ir.x[(rb + r) * RMAX + ks])));
+          for (int j = 0; j < 4; ++j) {
+            const int r = r0 + 4 * j;
+            if (r < nr && ks < nrhs) {
+              // lane ks of the row group writes column ks (all lanes hold every sum)
+              double v = acc[j][0];
+#pragma unroll
+              for (int c = 1; c < RMAX; ++c)
+                if (c == ks) v = acc[j][c];
+              const double rv2 = b[(long)(rb + r) * nrhs + ks] - v;
+              sh.ir.r[(rb + r) * RMAX + ks] = rv2;
+              atomicMax(&misc.cmax[ks], (unsigned long long)__double_as_longlong(fabs(rv2)));
+              atomicMax(&misc.cmax[RMAX + ks],
+                        (unsigned long long)__double_as_longlong(fabs(sh.ir.x[(rb + r) * RMAX + ks])));
+            }
           }
         }
         __syncthreads();
@@ -523,7 +651,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
           sh.ir.d[e] = (c < nrhs) ? (float)sh.ir.r[misc.perm[i] * RMAX + c] : 0.0f;
         }
         __syncthreads();
-        lu_solve<float>(W, n, n, sh.ir.d, nrhs);
+        lu_solve<float>(W, n, n, sh.ir.d, nrhs, sh.ir.diag);
         for (int e = tid; e < n * RMAX; e += NT) sh.ir.x[e] += (double)sh.ir.d[e];
         __syncthreads();
       }
@@ -572,7 +700,7 @@ __global__ __launch_bounds__(NT) void gesv_fallback_kernel(
         sh.s.x[e] = (c < nrhs) ? b[(long)misc.perm[i] * nrhs + c] : 0.0;
       }
       __syncthreads();
-      lu_solve<double>(W, n, n, sh.s.x, nrhs);
+      lu_solve<double>(W, n, n, sh.s.x, nrhs, sh.s.diag);
       double* x = X + b_off[s];
       for (int e = tid; e < n * nrhs; e += NT) x[e] = sh.s.x[(e / nrhs) * RMAX + e % nrhs];
     }
@@ -588,8 +716,8 @@ __global__ void reset_counters_kernel(int* c, int n) {
 
 // Occupancy of the persistent grids (workgroups per CU); chosen by the autotuner.
 struct SolverTuning {
-  int wg_per_cu_fp64 = 2;
-  int wg_per_cu_mixed = 2;
+  int wg_per_cu_fp64 = 8;
+  int wg_per_cu_mixed = 8;
 };
 static SolverTuning g_tuning;
 
