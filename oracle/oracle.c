@@ -6,23 +6,22 @@
  * benchmark load this file's library; the product path never does.  Each function names the
  * reference file:line it restates (paths relative to the Blaze source tree).
  *
- * Pinning (what checks each function independently of this code; see tests/test_oracle.py):
- *   orc_gemm_*    Blaze known answers (ClassTest2.cpp:1244-1273, AliasingTest.cpp:1803-1850),
- *                 the compiled Blaze kernels in oracle/_ref, exact integer arithmetic,
- *                 numpy matmul, identity/permutation products.
- *   orc_gemv_*    compiled Blaze DMatDVecMultExpr (oracle/_ref), exact integers, numpy.
- *   orc_csrmv_*   compiled Blaze SMatDVecMultExpr (oracle/_ref) bit-for-bit (same sequential
- *                 order, no contraction), exact integer Laplacian, dense product of the
- *                 same matrix.
- *   orc_getrf/getrs/gesv   scipy.linalg.lu_factor (LAPACK dgetrf, same pivot rule), exact
- *                 rational solves of small systems (fractions), P*A == L*U reconstruction.
- *   orc_solve_closed   compiled Blaze solve1x1..solve6x6 (oracle/_ref), and numpy.solve.
- *   orc_dsgesv    backward error <= 1e-12 against the FP64 solve, iteration counts against
- *                 the convergence theory (kappa * 2^-24 < 1 converges; kappa = 1e8 falls back).
- *   orc_gemm_dd_entry  exact rational products of small integers scaled by powers of two
- *                 (fractions), and the identity C = A * I.
- *   orc_fp64acc   the double-double oracle above (error <= 1e-14 * ||C||_F) and exact
- *                 integer products (bit-exact).
+ * Pinning (what checks each function independently of this code; tests/test_oracle.py):
+ *   orc_gemm_*    the reference's known answers (ClassTest2.cpp:1244-1273,
+ *                 AliasingTest.cpp:1803-1850); golden fixtures from the compiled reference
+ *                 kernels (int32 bit-exact, double within 2 gamma_k |A||B|); exact integer
+ *                 arithmetic; permutation products (a transposed operand fails).
+ *   orc_gemm_dd_entries  exact rational sums of dyadic products (fractions), to 2^-100.
+ *   orc_gemv_*    compiled reference DMatDVecMultExpr (oracle/_ref), exact integers.
+ *   orc_csrmv_*   golden fixtures from the compiled reference SMatDVecMultExpr, bit-for-bit
+ *                 (same sequential order, no contraction); the dense product of the same
+ *                 integer matrix; empty rows.
+ *   orc_getrf/getrs/gesv   LAPACK dgetrf factors and pivots (scipy.linalg.lu_factor fixture),
+ *                 exact rational solves (fractions), P*A == L*U, info of a zero column.
+ *   orc_solve_closed   golden fixtures from the reference's solve1x1..solve6x6.
+ *   orc_dsgesv    FP64 solve agreement, zero refinement steps when the FP32 solve is exact,
+ *                 fallback (-3) at kappa = 1e13 where FP32 refinement cannot converge.
+ *   orc_backward_error   a hand-computed case.
  */
 #include <math.h>
 #include <stdint.h>

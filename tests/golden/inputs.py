@@ -1,0 +1,87 @@
+"""Seeded inputs shared by the golden-fixture generator and the tests that read the fixtures.
+
+Every input is regenerated from a stored seed by the package's generator library
+(blaze_amd/csrc/gen.c: MT19937-64 and SplitMix64 counter streams), so the fixtures hold
+only seeds and expected outputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+import blaze_amd as bz
+
+HERE = Path(__file__).resolve().parent
+vp, u64, i64, i32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_int32
+
+
+def _p(a):
+    return a.ctypes.data_as(vp)
+
+
+def ints(seed, stream, n, lo=-10, hi=10):
+    out = np.empty(n, np.int32)
+    bz.gen().bzg_gen_int_ctr(u64(seed), u64(stream), i64(n), i32(lo), i32(hi), _p(out))
+    return out
+
+
+def uniform(seed, stream, n):
+    out = np.empty(n, np.float64)
+    bz.gen().bzg_gen_uniform_ctr(u64(seed), u64(stream), i64(n), _p(out))
+    return out
+
+
+def uniform_mt(seed, *shapes):
+    """Matrices filled one after the other from a single MT19937-64 stream (A first, then B)."""
+    outs = [np.empty(s) for s in shapes]
+    if len(outs) == 1:
+        bz.gen().bzg_gen_uniform_mt(u64(seed), i64(outs[0].size), _p(outs[0]), i64(0), vp(0))
+    else:
+        bz.gen().bzg_gen_uniform_mt(u64(seed), i64(outs[0].size), _p(outs[0]), i64(outs[1].size), _p(outs[1]))
+    return outs
+
+
+def laplace3d(nx):
+    nnz = bz.gen().bzg_gen_laplace3d_nnz(i64(nx))
+    n = nx ** 3
+    rp = np.empty(n + 1, np.int32); ci = np.empty(nnz, np.int32); v = np.empty(nnz, np.float64)
+    bz.gen().bzg_gen_laplace3d(i64(nx), _p(rp), _p(ci), _p(v))
+    return rp, ci, v
+
+
+def zipf(seed, n, s, kmax):
+    rp = np.empty(n + 1, np.int32)
+    bz.gen().bzg_gen_zipf_rowptr(u64(seed), i64(n), C.c_double(s), i32(kmax), _p(rp))
+    nnz = int(rp[-1])
+    ci = np.empty(nnz, np.int32); v = np.empty(nnz, np.float64)
+    bz.gen().bzg_gen_zipf_fill(u64(seed), i64(n), i64(n), _p(rp), _p(ci), _p(v))
+    return rp, ci, v
+
+
+GEMM_SHAPES = [(m, n, k) for m in range(7) for n in range(7) for k in range(7)] + [
+    (15, 37, 63), (37, 15, 63), (16, 32, 64), (64, 64, 64), (63, 65, 67), (128, 128, 128)]
+
+
+def gemm_case(seed, idx, dtype):
+    """Operands of case idx of the size grid: int32 in [-10, 10] or double uniform[-1, 1)."""
+    m, n, k = GEMM_SHAPES[idx]
+    if dtype == np.int32:
+        return ints(seed, 2 * idx, m * k).reshape(m, k), ints(seed, 2 * idx + 1, k * n).reshape(k, n)
+    return (uniform(seed, 2 * idx, m * k).reshape(m, k), uniform(seed, 2 * idx + 1, k * n).reshape(k, n))
+
+
+def solve_small_case(n, t):
+    """System t (0..3) of size n of the closed-form fixture; t == 3 is exactly singular
+    (a zero last row, so every term of a determinant expansion is an exact zero)."""
+    A = uniform(0x501E, 100 * n + 2 * t, n * n).reshape(n, n) + n * np.eye(n)
+    if t == 3:
+        A[-1] = 0.0
+    b = uniform(0x501E, 100 * n + 2 * t + 1, n)
+    return A, b
+
+
+def load(name):
+    with np.load(HERE / name, allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}

@@ -1,0 +1,114 @@
+"""Writes the golden fixtures of this directory (run in a tree where the reference is present).
+
+Expected outputs come from the reference's own kernels, compiled from its headers into
+oracle/_ref/libblazeref.so (oracle/Makefile), except lu_scipy.npz, whose factors come from
+scipy.linalg.lu_factor (LAPACK dgetrf): the reference hands n >= 7 solves to an external,
+unpinned LAPACK (blaze/math/lapack/clapack/gesv.h:207), so an independent LAPACK build is
+the only available witness there.
+
+Inputs are regenerated from the seeds stored in each file by the package's generator
+library (MT19937-64 and SplitMix64 counter streams, blaze_amd/csrc/gen.c); only small inputs
+are stored verbatim.
+
+    python tests/golden/make_golden.py      # rewrites tests/golden/*.npz
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(HERE))
+
+import oracle  # noqa: E402
+
+from inputs import (GEMM_SHAPES, gemm_case, ints, laplace3d, solve_small_case, uniform,  # noqa: E402
+                    uniform_mt, zipf)
+
+GEMM_SEED = 0x0B1A2E
+
+
+def gemm_fixtures():
+    # int32 (bit-exact) and double (relaxed) products of the blazetest size grid
+    # (blazetest/src/mathtest/dmatdmatmult/*.cpp run m, n, k in 0..6 plus 15/37/63, 16/32/64).
+    shapes = np.array(GEMM_SHAPES, np.int64)
+    ci, cd = [], []
+    for idx, (m, n, k) in enumerate(GEMM_SHAPES):
+        A, B = gemm_case(GEMM_SEED, idx, np.int32)
+        ci.append(oracle.ref_gemm(A, B).ravel())
+        Ad, Bd = gemm_case(GEMM_SEED, idx, np.float64)
+        cd.append(oracle.ref_gemm(Ad, Bd).ravel())
+    np.savez_compressed(HERE / "gemm_blazetest.npz", seed=np.uint64(GEMM_SEED), shapes=shapes,
+                        c_int32=np.concatenate(ci), c_double=np.concatenate(cd))
+
+    # 512^3: A then B drawn from one MT19937-64 stream; 4096 sampled entries + ||C||_F
+    seed, n = 128753984, 512
+    A, B = uniform_mt(seed, (n, n), (n, n))
+    Cb = oracle.ref_gemm(A, B)
+    rng = np.random.default_rng(7)
+    rows = rng.integers(0, n, 4096); cols = rng.integers(0, n, 4096)
+    np.savez_compressed(HERE / "gemm_512_mt.npz", seed=np.uint64(seed), n=np.int64(n), rows=rows,
+                        cols=cols, c_sample=Cb[rows, cols], c_frob=np.linalg.norm(Cb))
+
+
+def spmv_fixtures():
+    rp, ci, v = laplace3d(16)
+    x = uniform(0x5A11, 0, len(rp) - 1)
+    y = oracle.ref_csrmv(rp, ci, v, x, len(rp) - 1)
+    xi = ints(0x5A11, 1, len(rp) - 1, -100, 100)
+    yi = oracle.ref_csrmv(rp, ci, v.astype(np.int32), xi, len(rp) - 1)
+    np.savez_compressed(HERE / "spmv_laplace16.npz", nx=np.int64(16), x_seed=np.uint64(0x5A11),
+                        y=y, y_int32=yi)
+
+    n, s, kmax, seed = 10000, 1.5, 2000, 0x21FF
+    rp, ci, v = zipf(seed, n, s, kmax)
+    x = uniform(seed, 99, n)
+    y = oracle.ref_csrmv(rp, ci, v, x, n)
+    y32 = oracle.ref_csrmv(rp, ci, v.astype(np.float32), x.astype(np.float32), n)
+    np.savez_compressed(HERE / "spmv_zipf1e4.npz", n=np.int64(n), s=np.float64(s),
+                        kmax=np.int64(kmax), seed=np.uint64(seed), x_stream=np.int64(99),
+                        nnz=np.int64(rp[-1]), y=y, y_f32=y32)
+
+
+def solve_fixtures():
+    # closed-form solves of the reference (LSE.h solve1x1 .. solve6x6), well-conditioned
+    # systems plus one singular system per size
+    As, bs, xs, rcs, ns = [], [], [], [], []
+    for n in range(1, 7):
+        for t in range(4):
+            A, b = solve_small_case(n, t)
+            x, rc = oracle.ref_solve_small(A, b)
+            As.append(A.ravel()); bs.append(b); xs.append(x); rcs.append(rc); ns.append(n)
+    np.savez_compressed(HERE / "solve_small.npz", n=np.array(ns, np.int64), A=np.concatenate(As),
+                        b=np.concatenate(bs), x=np.concatenate(xs), rc=np.array(rcs, np.int64))
+
+    import scipy.linalg as sl
+    out = {}
+    for n in (7, 16, 64):
+        A = uniform(0x1F, n, n * n).reshape(n, n)
+        lu, piv = sl.lu_factor(A)
+        out[f"A{n}"] = A
+        out[f"LU{n}"] = lu
+        out[f"piv{n}"] = piv.astype(np.int32)
+    np.savez_compressed(HERE / "lu_scipy.npz", **out)
+
+
+def main() -> None:
+    if not oracle.have_ref():
+        oracle.build()
+    if not oracle.have_ref():
+        raise SystemExit("oracle/_ref/libblazeref.so is missing (needs the reference tree)")
+    oracle.ref().ref_set_threads(1)
+    gemm_fixtures()
+    spmv_fixtures()
+    solve_fixtures()
+    for f in sorted(HERE.glob("*.npz")):
+        print(f"{f.name}: {f.stat().st_size} bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,306 @@
+"""CPU checks that pin the oracle (oracle/oracle.c) before it is trusted as the parity checker.
+
+Each oracle function is checked against something other than its own formula: the
+reference's known answers, the golden fixtures written from the reference's compiled kernels
+(tests/golden, make_golden.py), exact rational arithmetic, an independent LAPACK
+(scipy.linalg.lu_factor), and mathematical invariants chosen so that a dropped term, a wrong
+index or a transposed operand fails.  No GPU is used here.
+"""
+from __future__ import annotations
+
+import sys
+from fractions import Fraction
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+
+sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+from inputs import (GEMM_SHAPES, gemm_case, laplace3d, load, solve_small_case, uniform,  # noqa: E402
+                    uniform_mt, zipf)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    oracle.build()
+
+
+# ---- generators ------------------------------------------------------------------------
+def test_mt19937_64_published_value():
+    # ISO C++ [rand.predef]: the 10000th output of a default-seeded (5489) mt19937_64
+    import ctypes as C
+    import blaze_amd as bz
+    out = np.empty(10000, np.uint64)
+    bz.gen().bzg_gen_mt64_raw(C.c_uint64(5489), C.c_int64(10000), out.ctypes.data_as(C.c_void_p))
+    assert int(out[-1]) == 9981545732273789042
+
+
+def test_uniform_mt_maps_top_53_bits():
+    # u = (x >> 11) * 2^-53 * 2 - 1 lies in [-1, 1) and takes 2^53 values
+    (a,) = uniform_mt(1, (1 << 14,))
+    assert a.min() >= -1.0 and a.max() < 1.0
+    assert np.all(np.ldexp(a + 1.0, 52) == np.round(np.ldexp(a + 1.0, 52)))
+
+
+# ---- GEMM ------------------------------------------------------------------------------
+def test_gemm_known_answer_classtest2():
+    # blazetest/src/mathtest/dynamicmatrix/ClassTest2.cpp:1244-1273
+    A = np.array([[1, 0, 2], [0, 3, 0], [4, 0, 5]])
+    B = np.array([[0, 2, 0, 0], [1, 3, 0, 4], [0, 0, 0, 5]])
+    expect = np.array([[0, 2, 0, 10], [3, 9, 0, 12], [0, 8, 0, 25]])
+    for dt in (np.int32, np.float64, np.float32):
+        assert np.array_equal(oracle.gemm(A.astype(dt), B.astype(dt)), expect.astype(dt))
+
+
+def test_gemm_aliasing_fixture():
+    # blazetest/src/mathtest/dmatdmatmult/AliasingTest.cpp:1803-1850 (dA3x4 * dB4x3), by hand
+    A = np.array([[-1, 0, -2, 0], [0, 2, -3, 1], [0, 1, 2, 2]], np.int32)
+    B = np.array([[1, 0, -3], [0, -1, 0], [0, 2, 1], [2, 1, -2]], np.int32)
+    expect = np.array([[-1, -4, 1], [2, -7, -5], [4, 5, -2]], np.int32)
+    assert np.array_equal(oracle.gemm(A, B), expect)
+
+
+def test_gemm_int32_golden_bit_exact():
+    g = load("gemm_blazetest.npz")
+    off = 0
+    for idx, (m, n, k) in enumerate(GEMM_SHAPES):
+        A, B = gemm_case(int(g["seed"]), idx, np.int32)
+        want = g["c_int32"][off:off + m * n].reshape(m, n)
+        assert np.array_equal(oracle.gemm(A, B), want), (m, n, k)
+        off += m * n
+    assert off == g["c_int32"].size
+
+
+def test_gemm_double_golden_relaxed():
+    # the reference's kernel order (SIMD lanes, FMA) is not the oracle's, so the comparison is
+    # Higham's bound |C' - C| <= 2 gamma_k |A||B| between any two summation orders
+    g = load("gemm_blazetest.npz")
+    off = 0
+    for idx, (m, n, k) in enumerate(GEMM_SHAPES):
+        A, B = gemm_case(int(g["seed"]), idx, np.float64)
+        want = g["c_double"][off:off + m * n].reshape(m, n)
+        bound = 2 * k * 2.0 ** -53 / (1 - k * 2.0 ** -53) * (np.abs(A) @ np.abs(B))
+        for fused in (True, False):
+            assert np.all(np.abs(oracle.gemm(A, B, fused) - want) <= bound + 1e-300), (m, n, k)
+        off += m * n
+
+
+def test_gemm_512_mt_golden():
+    g = load("gemm_512_mt.npz")
+    n = int(g["n"])
+    A, B = uniform_mt(int(g["seed"]), (n, n), (n, n))
+    Cm = oracle.gemm(A, B)
+    got = Cm[g["rows"], g["cols"]]
+    bound = 2 * n * 2.0 ** -53 * (np.abs(A) @ np.abs(B))[g["rows"], g["cols"]]
+    assert np.all(np.abs(got - g["c_sample"]) <= bound)
+    assert abs(np.linalg.norm(Cm) - g["c_frob"]) <= 1e-12 * g["c_frob"]
+
+
+def test_gemm_integer_valued_doubles_any_order_exact():
+    # integer entries in [-10, 10], k = 300: every partial sum < 2^53, so fused, unfused and
+    # numpy's blocked order all give the exact product
+    rng = np.random.default_rng(3)
+    A = rng.integers(-10, 11, (33, 300)).astype(np.float64)
+    B = rng.integers(-10, 11, (300, 17)).astype(np.float64)
+    exact = A.astype(np.int64) @ B.astype(np.int64)
+    assert np.array_equal(oracle.gemm(A, B, True), exact)
+    assert np.array_equal(oracle.gemm(A, B, False), exact)
+
+
+def test_gemm_permutation_and_transpose_sensitivity():
+    # P A permutes rows; a transposed operand or a wrong index would permute columns instead
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((6, 5))
+    P = np.eye(6)[[3, 0, 5, 1, 4, 2]]
+    assert np.array_equal(oracle.gemm(P, A), A[[3, 0, 5, 1, 4, 2]])
+    Q = np.eye(5)[[1, 2, 3, 4, 0]]
+    assert np.array_equal(oracle.gemm(A, Q), A @ Q)
+
+
+def test_gemm_dd_entries_exact_rationals():
+    # entries m * 2^e with small integers m: the exact product is a dyadic rational; the
+    # double-double result must hold it to ~2^-104 relative
+    rng = np.random.default_rng(5)
+    k = 200
+    A = np.ldexp(rng.integers(-(1 << 20), 1 << 20, (4, k)).astype(np.float64), rng.integers(-30, 30, (4, k)))
+    B = np.ldexp(rng.integers(-(1 << 20), 1 << 20, (k, 3)).astype(np.float64), rng.integers(-30, 30, (k, 3)))
+    rows = np.repeat(np.arange(4), 3); cols = np.tile(np.arange(3), 4)
+    hi, lo = oracle.gemm_dd_entries(A, B, rows, cols)
+    for t, (i, j) in enumerate(zip(rows, cols)):
+        exact = sum(Fraction(float(A[i, q])) * Fraction(float(B[q, j])) for q in range(k))
+        got = Fraction(float(hi[t])) + Fraction(float(lo[t]))
+        scale = sum(abs(Fraction(float(A[i, q])) * Fraction(float(B[q, j]))) for q in range(k))
+        assert abs(got - exact) <= scale * Fraction(1, 2 ** 100)
+
+
+# ---- GEMV ------------------------------------------------------------------------------
+def test_gemv_against_reference_and_exact():
+    rng = np.random.default_rng(6)
+    A = rng.integers(-50, 51, (37, 129)).astype(np.int32)
+    x = rng.integers(-50, 51, 129).astype(np.int32)
+    assert np.array_equal(oracle.gemv(A, x), A.astype(np.int64) @ x)
+    if oracle.have_ref():
+        assert np.array_equal(oracle.gemv(A, x), oracle.ref_gemv(A, x))
+        Ad = rng.uniform(-1, 1, (37, 129)); xd = rng.uniform(-1, 1, 129)
+        bound = 2 * 129 * 2.0 ** -53 * (np.abs(Ad) @ np.abs(xd))
+        assert np.all(np.abs(oracle.gemv(Ad, xd) - oracle.ref_gemv(Ad, xd)) <= bound)
+
+
+# ---- SpMV ------------------------------------------------------------------------------
+def test_spmv_laplace_golden_bit_exact():
+    g = load("spmv_laplace16.npz")
+    rp, ci, v = laplace3d(int(g["nx"]))
+    n = len(rp) - 1
+    x = uniform(int(g["x_seed"]), 0, n)
+    assert np.array_equal(oracle.csrmv(rp, ci, v, x), g["y"])
+    from inputs import ints
+    xi = ints(int(g["x_seed"]), 1, n, -100, 100)
+    assert np.array_equal(oracle.csrmv(rp, ci, v.astype(np.int32), xi), g["y_int32"])
+
+
+def test_spmv_zipf_golden_bit_exact():
+    g = load("spmv_zipf1e4.npz")
+    rp, ci, v = zipf(int(g["seed"]), int(g["n"]), float(g["s"]), int(g["kmax"]))
+    assert rp[-1] == g["nnz"]
+    x = uniform(int(g["seed"]), int(g["x_stream"]), int(g["n"]))
+    assert np.array_equal(oracle.csrmv(rp, ci, v, x), g["y"])
+    assert np.array_equal(oracle.csrmv(rp, ci, v.astype(np.float32), x.astype(np.float32)), g["y_f32"])
+
+
+def test_spmv_equals_dense_product_of_same_matrix():
+    # the CSR loop against the dense matrix it encodes, integer values: exact
+    rp, ci, v = laplace3d(5)
+    n = len(rp) - 1
+    D = np.zeros((n, n), np.int64)
+    for i in range(n):
+        D[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]].astype(np.int64)
+    assert np.all(np.diff(rp) <= 7) and np.all(D.diagonal() == 6)
+    x = np.arange(n, dtype=np.int32) % 17 - 8
+    assert np.array_equal(oracle.csrmv(rp, ci, v.astype(np.int32), x), D @ x)
+
+
+def test_spmv_empty_rows():
+    rp = np.array([0, 0, 2, 2, 3], np.int32)
+    ci = np.array([0, 3, 1], np.int32)
+    v = np.array([2.0, -1.0, 4.0])
+    x = np.array([1.0, 2.0, 3.0, 4.0])
+    assert np.array_equal(oracle.csrmv(rp, ci, v, x), [0.0, -2.0, 0.0, 8.0])
+
+
+# ---- LU and solves ---------------------------------------------------------------------
+def test_getrf_matches_lapack_factors_and_pivots():
+    g = load("lu_scipy.npz")
+    for n in (7, 16, 64):
+        LU, ip, info = oracle.getrf(g[f"A{n}"])
+        assert info == 0
+        assert np.array_equal(ip - 1, g[f"piv{n}"]), n   # 1-based vs 0-based
+        assert np.allclose(LU, g[f"LU{n}"], rtol=0, atol=1e-12 * n)
+
+
+def test_getrf_reconstruction_and_singular_info():
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((20, 20))
+    LU, ip, info = oracle.getrf(A)
+    L = np.tril(LU, -1) + np.eye(20); U = np.triu(LU)
+    PA = A.copy()
+    for i, p in enumerate(ip - 1):
+        PA[[i, p]] = PA[[p, i]]
+    assert np.allclose(L @ U, PA, atol=1e-13)
+    # an exactly zero column 5 stops at U(5,5) == 0: info = 6 (1-based, LAPACK convention)
+    A[:, 5] = 0.0
+    assert oracle.getrf(A)[2] == 6
+
+
+def test_gesv_exact_rational_small_systems():
+    rng = np.random.default_rng(9)
+    for n in range(1, 6):
+        A = rng.integers(-9, 10, (n, n)).astype(np.float64) + 10 * np.eye(n)
+        b = rng.integers(-9, 10, (n, 1)).astype(np.float64)
+        X, _, _, info = oracle.gesv(A, b)
+        assert info == 0
+        # exact solution by Gauss-Jordan over the rationals
+        M = [[Fraction(int(A[i, j])) for j in range(n)] + [Fraction(int(b[i, 0]))] for i in range(n)]
+        for c in range(n):
+            p = next(r for r in range(c, n) if M[r][c] != 0)
+            M[c], M[p] = M[p], M[c]
+            for r in range(n):
+                if r != c and M[r][c] != 0:
+                    f = M[r][c] / M[c][c]
+                    M[r] = [a - f * bb for a, bb in zip(M[r], M[c])]
+        exact = [float(M[i][n] / M[i][i]) for i in range(n)]
+        assert np.allclose(X[:, 0], exact, rtol=1e-13, atol=1e-15)
+
+
+def test_solve_closed_forms_match_reference_golden():
+    g = load("solve_small.npz")
+    offA = offb = 0
+    for t, n in enumerate(g["n"]):
+        n = int(n)
+        A, b = solve_small_case(n, t % 4)
+        assert np.array_equal(A.ravel(), g["A"][offA:offA + n * n])
+        want = g["x"][offb:offb + n]
+        if g["rc"][t] != 0:
+            # singular: the reference throws, the LU oracle reports info > 0
+            assert oracle.gesv(A, b[:, None])[3] > 0
+        else:
+            X, _, _, info = oracle.gesv(A, b[:, None])
+            assert info == 0 and np.allclose(X[:, 0], want, rtol=1e-12, atol=1e-14)
+            if n <= 3:
+                xc, rc = oracle.solve_closed(A, b)
+                assert rc == 0 and np.allclose(xc, want, rtol=1e-13, atol=1e-15)
+        offA += n * n; offb += n
+
+
+def test_backward_error_definition():
+    # ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), per column maximum
+    A = np.array([[2.0, 1.0], [0.0, 4.0]])
+    x = np.array([[1.0], [1.0]])
+    b = A @ x
+    assert oracle.backward_error(A, b, x) == 0.0
+    xb = x + np.array([[0.0], [1e-3]])
+    want = 4e-3 / (4.0 * (1.0 + 1e-3) + 4.0)   # ||A||_inf = 4 (row 2), ||r||_inf = 4e-3
+    assert abs(oracle.backward_error(A, b, xb) - want) <= 1e-12 * want
+
+
+def test_dsgesv_converges_and_falls_back():
+    rng = np.random.default_rng(10)
+    n = 60
+    A = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
+    B = rng.uniform(-1, 1, (n, 3))
+    X, it, info = oracle.dsgesv(A, B)
+    assert 0 <= it <= 5 and info == 0
+    assert oracle.backward_error(A, B, X) <= 1e-15
+    Xd = oracle.gesv(A, B)[0]
+    assert np.allclose(X, Xd, rtol=1e-13, atol=1e-15)
+    # diagonal powers of two and FP32-representable b: the FP32 solve is exact, zero steps
+    D = np.diag(np.ldexp(1.0, rng.integers(-5, 5, n)))
+    b = rng.integers(-100, 100, (n, 1)).astype(np.float64)
+    X0, it0, _ = oracle.dsgesv(D, b)
+    assert it0 == 0 and np.array_equal(X0, b / np.diag(D)[:, None])
+    # kappa = 1e13 >> 2^24: FP32 refinement cannot converge; result comes from the FP64 LU
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Ak = Q1 @ np.diag(np.logspace(0, -13, n)) @ Q2.T
+    Xk, itk, infok = oracle.dsgesv(Ak, B)
+    assert itk == -3 and infok == 0
+    assert oracle.backward_error(Ak, B, Xk) <= 1e-15
+
+
+# ---- compiled reference (when present) -------------------------------------------------
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built (reference tree absent)")
+def test_oracle_matches_compiled_reference_kernels():
+    rng = np.random.default_rng(11)
+    A = rng.integers(-10, 11, (45, 70)).astype(np.int32)
+    B = rng.integers(-10, 11, (70, 33)).astype(np.int32)
+    assert np.array_equal(oracle.gemm(A, B), oracle.ref_gemm(A, B))
+    rp, ci, v = zipf(77, 3000, 1.5, 500)
+    x = rng.uniform(-1, 1, 3000)
+    assert np.array_equal(oracle.csrmv(rp, ci, v, x), oracle.ref_csrmv(rp, ci, v, x, 3000))
+    for n in range(1, 4):
+        M = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
+        b = rng.uniform(-1, 1, n)
+        xr, rcr = oracle.ref_solve_small(M, b)
+        xo, rco = oracle.solve_closed(M, b)
+        assert rcr == rco == 0 and np.allclose(xo, xr, rtol=1e-14, atol=1e-16)
