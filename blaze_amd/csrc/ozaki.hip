@@ -181,21 +181,39 @@ __global__ __launch_bounds__(256) void row_scale_kernel(int m, int k, const doub
   if (lane == 0) s[wave] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
 }
 
-// Column exponents of B (k x n, ldb): t_j = P - 1 - ilogb(max_i |b_ij|).
-__global__ __launch_bounds__(256) void col_scale_kernel(int k, int n, const double* __restrict__ B,
-                                                        long ldb, int P, int* __restrict__ t) {
-  // 64 columns per block, 4 row phases; partial maxima combined in LDS
+// Column maxima of |B| (k x n, ldb) as FP64 bit patterns (non-negative doubles order like
+// their bit patterns): a (n / 64) x (k / 256) grid of 64-column x 256-row blocks (y strides over K), combined with
+// one atomicMax per column and block.  cmax must be zero on entry.
+__global__ __launch_bounds__(256) void col_max_kernel(int k, int n, const double* __restrict__ B,
+                                                      long ldb, unsigned long long* __restrict__ cmax) {
   __shared__ double part[4][64];
   const int c = blockIdx.x * 64 + threadIdx.x % 64, ph = threadIdx.x / 64;
   double mx = 0.0;
   if (c < n)
-    for (int i = ph; i < k; i += 4) mx = fmax(mx, fabs(B[(long)i * ldb + c]));
+    for (int r0 = blockIdx.y * 256; r0 < k; r0 += gridDim.y * 256) {
+      const int r1 = min(k, r0 + 256);
+      for (int i = r0 + ph; i < r1; i += 4) mx = fmax(mx, fabs(B[(long)i * ldb + c]));
+    }
   part[ph][threadIdx.x % 64] = mx;
   __syncthreads();
   if (ph == 0 && c < n) {
     mx = fmax(fmax(part[0][threadIdx.x], part[1][threadIdx.x]), fmax(part[2][threadIdx.x], part[3][threadIdx.x]));
-    t[c] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
+    atomicMax(&cmax[c], (unsigned long long)__double_as_longlong(mx));
   }
+}
+
+__global__ __launch_bounds__(256) void zero_u64_kernel(int n, unsigned long long* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// t_j = P - 1 - ilogb(max_i |b_ij|)  (0 for zero columns).
+__global__ __launch_bounds__(256) void col_scale_kernel(int n, const unsigned long long* __restrict__ cmax,
+                                                        int P, int* __restrict__ t) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double mx = __longlong_as_double((long long)cmax[j]);
+  t[j] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
 }
 
 // Symmetric residue of the integer-valued double v modulo m (|result| <= (m-1)/2).
@@ -219,17 +237,32 @@ __device__ __forceinline__ double sym_mod_wide(double v, double m, double minv, 
 template <int MODE>
 struct SliceOf;
 template <>
-struct SliceOf<0> {  // BF16
+struct SliceOf<0> {  // BF16: 4 slices packed in 8 bytes
   using T = __bf16;
-  __device__ static T cvt(double r) { return (__bf16)(float)r; }
+  using Pack = uint2;
+  __device__ static uint32_t bits(double r) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (__bf16)(float)r);
+  }
+  __device__ static Pack pack(const double* r) {
+    return make_uint2(bits(r[0]) | (bits(r[1]) << 16), bits(r[2]) | (bits(r[3]) << 16));
+  }
 };
 template <>
-struct SliceOf<1> {  // INT8
+struct SliceOf<1> {  // INT8: 4 slices packed in 4 bytes
   using T = int8_t;
-  __device__ static T cvt(double r) { return (int8_t)(int)r; }
+  using Pack = uint32_t;
+  __device__ static uint32_t bits(double r) { return (uint32_t)(uint8_t)(int8_t)(int)r; }
+  __device__ static Pack pack(const double* r) {
+    return bits(r[0]) | (bits(r[1]) << 8) | (bits(r[2]) << 16) | (bits(r[3]) << 24);
+  }
 };
 
-// A residues: Ares[l][i][kk] (rows padded to mp, k padded to kp, zeros in the padding).
+__device__ __forceinline__ double residue(double v, int wide, const CrtConsts* __restrict__ cc, int l) {
+  return wide ? sym_mod_wide(v, cc->m[l], cc->minv[l], cc->c30[l]) : sym_mod(v, cc->m[l], cc->minv[l]);
+}
+
+// A residues: Ares[l][i][kk] (rows padded to mp, k padded to kp (a multiple of 4), zeros in the
+// padding).  Grid: x covers kp in 1024-element chunks (4 per thread), y strides over rows.
 template <int MODE>
 __global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const double* __restrict__ A,
                                                          long lda, const int* __restrict__ s, int mp,
@@ -237,21 +270,29 @@ __global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const dou
                                                          const CrtConsts* __restrict__ cc,
                                                          typename SliceOf<MODE>::T* __restrict__ out) {
   using S = SliceOf<MODE>;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = (long)mp * kp;
-  if (idx >= total) return;
-  const int i = (int)(idx / kp), kk = (int)(idx % kp);
-  double v = 0.0;
-  if (i < m && kk < k) v = rint(ldexp(A[(long)i * lda + kk], s[i]));
-  if (wide) {
-    for (int l = 0; l < nmod; ++l)
-      out[(long)l * total + idx] = S::cvt(sym_mod_wide(v, cc->m[l], cc->minv[l], cc->c30[l]));
-  } else {
-    for (int l = 0; l < nmod; ++l) out[(long)l * total + idx] = S::cvt(sym_mod(v, cc->m[l], cc->minv[l]));
+  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (k0 >= kp) return;
+  const long plane = (long)mp * kp;
+  for (int i = blockIdx.y; i < mp; i += gridDim.y) {
+    double v[4];
+    const int si = i < m ? s[i] : 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = k0 + u;
+      v[u] = (i < m && kk < k) ? rint(ldexp(A[(long)i * lda + kk], si)) : 0.0;
+    }
+    typename S::T* o = out + (long)i * kp + k0;
+    for (int l = 0; l < nmod; ++l) {
+      double r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r[u] = residue(v[u], wide, cc, l);
+      *reinterpret_cast<typename S::Pack*>(o + l * plane) = S::pack(r);
+    }
   }
 }
 
-// B residues, transposed: Bres[l][j][kk] (n rows padded to np, k padded to kp).
+// B residues, transposed: Bres[l][j][kk] (n rows padded to np, k padded to kp).  64 x 64 tiles:
+// coalesced reads along j into LDS, then 4 consecutive kk per thread, packed stores along kk.
 template <int MODE>
 __global__ __launch_bounds__(256) void residues_b_kernel(int k, int n, const double* __restrict__ B,
                                                          long ldb, const int* __restrict__ t, int np,
@@ -262,21 +303,28 @@ __global__ __launch_bounds__(256) void residues_b_kernel(int k, int n, const dou
   __shared__ double tile[64][65];
   const int kb = blockIdx.x * 64, jb = blockIdx.y * 64;
   const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  const int tj = (jb + tx < n) ? t[jb + tx] : 0;
   for (int r = ty; r < 64; r += 4) {  // read B[kb+r][jb+tx]
     const int kk = kb + r, j = jb + tx;
     double v = 0.0;
-    if (kk < k && j < n) v = rint(ldexp(B[(long)kk * ldb + j], t[j]));
+    if (kk < k && j < n) v = rint(ldexp(B[(long)kk * ldb + j], tj));
     tile[r][tx] = v;
   }
   __syncthreads();
-  const long total = (long)np * kp;
-  for (int r = ty; r < 64; r += 4) {  // write Bres[l][jb+r][kb+tx]
-    const int j = jb + r, kk = kb + tx;
-    if (j < np && kk < kp) {
-      const double v = tile[tx][r];
-      for (int l = 0; l < nmod; ++l)
-        out[(long)l * total + (long)j * kp + kk] =
-            S::cvt(wide ? sym_mod_wide(v, cc->m[l], cc->minv[l], cc->c30[l]) : sym_mod(v, cc->m[l], cc->minv[l]));
+  const long plane = (long)np * kp;
+  const int jj = threadIdx.x / 16, g = threadIdx.x % 16;  // 16 rows of j per pass, 4 kk each
+  for (int jr = jj; jr < 64; jr += 16) {
+    const int j = jb + jr, kk = kb + 4 * g;
+    if (j >= np || kk >= kp) continue;
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = tile[4 * g + u][jr];
+    typename S::T* o = out + (long)j * kp + kk;
+    for (int l = 0; l < nmod; ++l) {
+      double r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r[u] = residue(v[u], wide, cc, l);
+      *reinterpret_cast<typename S::Pack*>(o + l * plane) = S::pack(r);
     }
   }
 }
@@ -288,19 +336,33 @@ constexpr int RT = 256;        // tile rows / cols
 constexpr int RBYTES = 128;    // bytes of K per LDS stage and row
 constexpr int RSTAGE = RT * RBYTES;  // bytes per operand per stage
 
-// 16-byte chunk c of row r lives at chunk c ^ (r & 7): conflict-free ds_read_b128 of 32 rows.
-__device__ __forceinline__ int swz(int r, int c) { return r * RBYTES + ((c ^ (r & 7)) << 4); }
+// 16-byte chunk c of row r lives at chunk c ^ ((r >> 1) & 7).  The LDS serves a ds_read_b128
+// 16 lanes at a time over 64 four-byte banks (256 bytes): 16 consecutive rows read at one
+// logical chunk then land on 16 distinct 16-byte bank groups (row parity picks the half,
+// (r >> 1) & 7 the group), and a row-major store of 2 rows x 8 chunks is conflict-free too.
+__device__ __forceinline__ int swz(int r, int c) { return r * RBYTES + ((c ^ ((r >> 1) & 7)) << 4); }
 
 // FN = 32-column MFMA blocks per wavefront: 4 -> 256 x 256 workgroup tile (INT8), 2 -> 256 x 128
 // (BF16: its FP32 accumulators are reduced on the VALU every 256 K, which needs them in VGPRs).
+// Output residues are symmetric: int8 for INT8 slices (m <= 255 -> |r| <= 127), int16 for BF16.
+//
+// Pipeline per K stage (128 bytes of K, 4 MFMA k-steps): the global loads of stage st+1 are
+// issued at the top; the fragments of k-step ks+1 are read from LDS into a second register set
+// while the 4 x FN MFMAs of k-step ks run; the staged registers of st+1 are written into the
+// other LDS buffer in the middle of the stage (that buffer was last read before the previous
+// barrier); one barrier closes the stage.
+template <int MODE>
+using ResOut = typename std::conditional<MODE == 0, int16_t, int8_t>::type;
+
 template <int MODE, int FN>
 __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, int kp,
                                                               const char* __restrict__ Ares,
                                                               const char* __restrict__ Bres,
-                                                              int16_t* __restrict__ Cres,
+                                                              ResOut<MODE>* __restrict__ Cres,
                                                               const CrtConsts* __restrict__ cc) {
   constexpr int ESZ = (MODE == 0) ? 2 : 1;          // bytes per element
   constexpr int KSTEP_B = 32;                        // bytes of K per MFMA (16 bf16 / 32 i8)
+  constexpr int NKS = RBYTES / KSTEP_B;              // MFMA k-steps per stage
   constexpr int BN = 64 * FN;                        // workgroup tile columns
   constexpr int NBCH = BN * 8 / 256;                 // B chunks staged per thread
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -366,61 +428,92 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
       *reinterpret_cast<v4i*>(Bs + buf * RSTAGE + swz(r, c)) = rb[e];
     }
   };
+  // MFMA operand fragments of k-step ks of LDS buffer buf
+  v4i fa[2][4], fb[2][FN];
+  auto lfrag = [&](int buf, int ks, v4i* a, v4i* b) {
+    const char* as = As + buf * RSTAGE;
+    const char* bs = Bs + buf * RSTAGE;
+    const int c = ks * 2 + (lane >> 5);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const v4i*>(as + swz(wm0 + i * 32 + (lane & 31), c));
+#pragma unroll
+    for (int j = 0; j < FN; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + swz(wn0 + j * 32 + (lane & 31), c));
+  };
+
+  auto kstep = [&](int cur) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        if constexpr (MODE == 0) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, fa[cur][i]),
+                                                               __builtin_bit_cast(bf8, fb[cur][j]),
+                                                               acc[i][j], 0, 0, 0);
+        } else {
+          acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[cur][i], fb[cur][j], acc[i][j], 0, 0, 0);
+        }
+      }
+  };
+  auto reduce_acc = [&]() {
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const float v = acc[i][j][r];
+            acc[i][j][r] = __builtin_fmaf(-mf, __builtin_rintf(v * minvf), v);
+          }
+    }
+  };
 
   const int nst = (int)(kbytes / RBYTES);
   // K elements per stage: 64 (bf16) / 128 (i8); bf16 reduces every 4 stages (256 K)
   gload(0);
   sstore(0);
   __syncthreads();
-  for (int st = 0; st < nst; ++st) {
+  lfrag(0, 0, fa[0], fb[0]);
+  for (int st = 0; st + 1 < nst; ++st) {
     const int buf = st & 1;
-    if (st + 1 < nst) gload(st + 1);
-    const char* as = As + buf * RSTAGE;
-    const char* bs = Bs + buf * RSTAGE;
+    gload(st + 1);
 #pragma unroll
-    for (int ks = 0; ks < RBYTES / KSTEP_B; ++ks) {
-      v4i a[4], b[FN];
-      const int c = ks * 2 + (lane >> 5);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const v4i*>(as + swz(wm0 + i * 32 + (lane & 31), c));
-#pragma unroll
-      for (int j = 0; j < FN; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + swz(wn0 + j * 32 + (lane & 31), c));
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          if constexpr (MODE == 0) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8, a[i]),
-                                                                 __builtin_bit_cast(bf8, b[j]),
-                                                                 acc[i][j], 0, 0, 0);
-          } else {
-            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
-          }
-        }
+    for (int ks = 0; ks < NKS; ++ks) {
+      const int cur = ks & 1;
+      if (ks + 1 < NKS) lfrag(buf, ks + 1, fa[cur ^ 1], fb[cur ^ 1]);
+      kstep(cur);
+      if (ks == NKS / 2) sstore(buf ^ 1);
     }
-    if constexpr (MODE == 0) {
-      // exact-integer FP32 accumulators: reduce mod m after every 256 K (4 stages)
-      if ((st & 3) == 3 && st + 1 < nst) {
+    // instruction order of the stage: each MFMA of k-step 0 is followed by one global load of
+    // the next stage, and the first 4 + FN MFMAs of k-steps 0..NKS-2 by one LDS fragment read
+    // of the following k-step; the LDS stores of the next stage ride on k-step NKS/2.
+    constexpr int NM = 4 * FN;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+    for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
-          for (int j = 0; j < FN; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              const float v = acc[i][j][r];
-              acc[i][j][r] = __builtin_fmaf(-mf, __builtin_rintf(v * minvf), v);
-            }
+      for (int t = 0; t < NM; ++t) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // MFMA
+        if (ks == 0 && t < 8 + NBCH) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+        if (ks + 1 < NKS && t < 4 + FN) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+        if (ks == NKS / 2 && t < 8 + NBCH) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
       }
-    }
-    if (st + 1 < nst) sstore(buf ^ 1);
+    if ((st & 3) == 3) reduce_acc();
     __syncthreads();
+    lfrag(buf ^ 1, 0, fa[0], fb[0]);
+  }
+#pragma unroll
+  for (int ks = 0; ks < NKS; ++ks) {
+    const int cur = ks & 1;
+    if (ks + 1 < NKS) lfrag((nst - 1) & 1, ks + 1, fa[cur ^ 1], fb[cur ^ 1]);
+    kstep(cur);
   }
 
-  // epilogue: symmetric residue mod m as int16
+  // epilogue: symmetric residue mod m
   const long cplane = (long)mp * np;
-  int16_t* Cg = Cres + l * cplane;
+  ResOut<MODE>* Cg = Cres + l * cplane;
   const int mi = (int)cc->m[l];
   const int half = (mi - 1) / 2;
+  const double mid = cc->m[l], minvd = cc->minv[l];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -432,46 +525,77 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
           const float f = acc[i][j][r];
           v = (int)__builtin_fmaf(-mf, __builtin_rintf(f * minvf), f);
         } else {
-          v = acc[i][j][r] % mi;
+          // |acc| < 2^31: the FP64 quotient is within 2^-20 of v / m, whose fractional part
+          // is 0 or >= 1/m, so floor() is exact except at multiples of m (result m -> 0 below)
+          const double d = (double)acc[i][j][r];
+          v = (int)fma(-mid, floor(d * minvd), d);
         }
         v = (v > half) ? v - mi : v;
         v = (v < -half) ? v + mi : v;
         const long gr = row0 + wm0 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const long gc = col0 + wn0 + j * 32 + (lane & 31);
-        Cg[gr * np + gc] = (int16_t)v;
+        Cg[gr * np + gc] = (ResOut<MODE>)v;
       }
 }
 
 // ------------------------------------------------------------------------------------------
 // device: CRT recombination and scaling
 // ------------------------------------------------------------------------------------------
+template <typename R>
+struct Quad;
+template <>
+struct Quad<int8_t> {
+  using V = char4;
+};
+template <>
+struct Quad<int16_t> {
+  using V = short4;
+};
+
+// 4 consecutive columns per thread (np is a multiple of 256); grid x covers np / 1024, y strides
+// over rows.
+template <typename R>
 __global__ __launch_bounds__(256) void crt_kernel(int m, int n, int mp, int np, int nmod,
-                                                  const int16_t* __restrict__ Cres,
+                                                  const R* __restrict__ Cres,
                                                   const int* __restrict__ s, const int* __restrict__ t,
                                                   const CrtConsts* __restrict__ cc, double* __restrict__ C,
                                                   long ldc) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)m * n) return;
-  const int i = (int)(idx / n), j = (int)(idx % n);
+  const int j0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (j0 >= n) return;
   const long plane = (long)mp * np;
-  const long off = (long)i * np + j;
-  double S1 = 0.0, S2 = 0.0, S3 = 0.0;
-  for (int l = 0; l < nmod; ++l) {
-    const double r = (double)Cres[l * plane + off];
-    S1 = fma(r, cc->w1[l], S1);  // exact: multiples of a common ulp, below 2^53 ulps
-    S2 = fma(r, cc->w2[l], S2);
-    S3 = fma(r, cc->w3[l], S3);
+  int tj[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) tj[u] = (j0 + u < n) ? t[j0 + u] : 0;
+  for (int i = blockIdx.y; i < m; i += gridDim.y) {
+    const R* src = Cres + (long)i * np + j0;
+    double S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, S3[4] = {0, 0, 0, 0};
+    for (int l = 0; l < nmod; ++l) {
+      const auto q4 = *reinterpret_cast<const typename Quad<R>::V*>(src + l * plane);
+      const double r[4] = {(double)q4.x, (double)q4.y, (double)q4.z, (double)q4.w};
+      const double w1 = cc->w1[l], w2 = cc->w2[l], w3 = cc->w3[l];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        S1[u] = fma(r[u], w1, S1[u]);  // exact: multiples of a common ulp, below 2^53 ulps
+        S2[u] = fma(r[u], w2, S2[u]);
+        S3[u] = fma(r[u], w3, S3[u]);
+      }
+    }
+    const int si = s[i];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (j0 + u >= n) break;
+      const double q = rint(S1[u] * cc->Minv);
+      const double T1 = fma(-q, cc->M1, S1[u]);  // exact
+      const double T2 = fma(-q, cc->M2, S2[u]);  // exact
+      const double T3 = fma(-q, cc->M3, S3[u]);
+      // T1 + T2 needs up to ~70 bits: add it error-free (TwoSum) so that C' is rounded only once.
+      const double sum = T1 + T2;
+      const double bb = sum - T1;
+      const double err = (T1 - (sum - bb)) + (T2 - bb);
+      const double Cp = sum + (err + T3);
+      C[(long)i * ldc + j0 + u] = ldexp(Cp, -(si + tj[u]));
+    }
   }
-  const double q = rint(S1 * cc->Minv);
-  const double T1 = fma(-q, cc->M1, S1);  // exact
-  const double T2 = fma(-q, cc->M2, S2);  // exact
-  const double T3 = fma(-q, cc->M3, S3);
-  // T1 + T2 needs up to ~70 bits: add it error-free (TwoSum) so that C' is rounded only once.
-  const double sum = T1 + T2;
-  const double bb = sum - T1;
-  const double err = (T1 - (sum - bb)) + (T2 - bb);
-  const double Cp = sum + (err + T3);
-  C[(long)i * ldc + j] = ldexp(Cp, -(s[i] + t[j]));
 }
 
 struct Fp64AccPlan {
@@ -526,8 +650,8 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   const long kp = (k + kq - 1) / kq * kq;
   const size_t a_bytes = (size_t)pl.nmod * mp * kp * esz;
   const size_t b_bytes = (size_t)pl.nmod * np * kp * esz;
-  const size_t c_bytes = (size_t)pl.nmod * mp * np * sizeof(int16_t);
-  const size_t misc_bytes = 4096 + (size_t)(mp + np) * sizeof(int);
+  const size_t c_bytes = (size_t)pl.nmod * mp * np * (mode == 0 ? sizeof(int16_t) : sizeof(int8_t));
+  const size_t misc_bytes = 4096 + (size_t)(mp + np) * sizeof(int) + (size_t)np * 8;
   const size_t total = a_bytes + b_bytes + c_bytes + misc_bytes;
   void* ws = nullptr;
   st = workspace(ctx, 4, total, &ws);
@@ -536,37 +660,41 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   CrtConsts* dcc = reinterpret_cast<CrtConsts*>(base);
   int* sA = reinterpret_cast<int*>(base + 4096);
   int* tB = sA + mp;
+  unsigned long long* cmaxB = reinterpret_cast<unsigned long long*>(base + 4096 + (size_t)(mp + np) * sizeof(int));
   char* Ares = base + misc_bytes;
   char* Bres = Ares + a_bytes;
-  int16_t* Cres = reinterpret_cast<int16_t*>(Bres + b_bytes);
+  char* Cres = Bres + b_bytes;
   static_assert(sizeof(CrtConsts) <= 4096, "CRT constants block");
   BZG_HIP_CHECK(hipMemcpyAsync(dcc, &pl.host, sizeof(CrtConsts), hipMemcpyHostToDevice, ctx->stream));
 
   hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)((m * 64 + 255) / 256)), dim3(256), 0, ctx->stream,
                      (int)m, (int)k, A, (long)lda, pl.P, sA);
   BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, ctx->stream,
-                     (int)k, (int)n, B, (long)ldb, pl.P, tB);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
+                     cmaxB);
   BZG_LAUNCH_CHECK();
-  const long na = mp * kp;
+  hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((n + 63) / 64), (unsigned)std::min<long>((k + 255) / 256, 16384)), dim3(256),
+                     0, ctx->stream, (int)k, (int)n, B, (long)ldb, cmaxB);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
+                     cmaxB, pl.P, tB);
+  BZG_LAUNCH_CHECK();
   const int wide = pl.P > 56 ? 1 : 0;
+  const dim3 ga((unsigned)((kp + 1023) / 1024), (unsigned)std::min<long>(mp, 16384));
+  const dim3 gb((unsigned)(kp / 64), (unsigned)(np / 64));
   if (mode == 0) {
-    hipLaunchKernelGGL(residues_a_kernel<0>, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (int)m, (int)k, A, (long)lda, sA, (int)mp, (int)kp, pl.nmod, wide, dcc,
-                       reinterpret_cast<__bf16*>(Ares));
+    hipLaunchKernelGGL(residues_a_kernel<0>, ga, dim3(256), 0, ctx->stream, (int)m, (int)k, A, (long)lda, sA,
+                       (int)mp, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<__bf16*>(Ares));
     BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(residues_b_kernel<0>, dim3((unsigned)(kp / 64), (unsigned)(np / 64)), dim3(256), 0,
-                       ctx->stream, (int)k, (int)n, B, (long)ldb, tB, (int)np, (int)kp, pl.nmod, wide, dcc,
-                       reinterpret_cast<__bf16*>(Bres));
+    hipLaunchKernelGGL(residues_b_kernel<0>, gb, dim3(256), 0, ctx->stream, (int)k, (int)n, B, (long)ldb, tB,
+                       (int)np, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<__bf16*>(Bres));
     BZG_LAUNCH_CHECK();
   } else {
-    hipLaunchKernelGGL(residues_a_kernel<1>, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (int)m, (int)k, A, (long)lda, sA, (int)mp, (int)kp, pl.nmod, wide, dcc,
-                       reinterpret_cast<int8_t*>(Ares));
+    hipLaunchKernelGGL(residues_a_kernel<1>, ga, dim3(256), 0, ctx->stream, (int)m, (int)k, A, (long)lda, sA,
+                       (int)mp, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<int8_t*>(Ares));
     BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(residues_b_kernel<1>, dim3((unsigned)(kp / 64), (unsigned)(np / 64)), dim3(256), 0,
-                       ctx->stream, (int)k, (int)n, B, (long)ldb, tB, (int)np, (int)kp, pl.nmod, wide, dcc,
-                       reinterpret_cast<int8_t*>(Bres));
+    hipLaunchKernelGGL(residues_b_kernel<1>, gb, dim3(256), 0, ctx->stream, (int)k, (int)n, B, (long)ldb, tB,
+                       (int)np, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<int8_t*>(Bres));
     BZG_LAUNCH_CHECK();
   }
   const size_t lds = 4 * RSTAGE;
@@ -584,15 +712,21 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   ProfScope prof_gemm(ctx, "fp64acc_gemm");
   if (mode == 0)
     hipLaunchKernelGGL((residue_gemm_kernel<0, 2>), dim3((unsigned)((mp / RT) * (np / 128)), 1, pl.nmod),
-                       dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres, Cres, dcc);
+                       dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres,
+                       reinterpret_cast<int16_t*>(Cres), dcc);
   else
     hipLaunchKernelGGL((residue_gemm_kernel<1, 4>), dim3((unsigned)((mp / RT) * (np / RT)), 1, pl.nmod),
-                       dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres, Cres, dcc);
+                       dim3(256), lds, ctx->stream, (int)mp, (int)np, (int)kp, Ares, Bres,
+                       reinterpret_cast<int8_t*>(Cres), dcc);
   BZG_LAUNCH_CHECK();
   }
-  const long mn = m * n;
-  hipLaunchKernelGGL(crt_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, ctx->stream, (int)m,
-                     (int)n, (int)mp, (int)np, pl.nmod, Cres, sA, tB, dcc, C, (long)ldc);
+  const dim3 gc((unsigned)((n + 1023) / 1024), (unsigned)std::min<long>(m, 16384));
+  if (mode == 0)
+    hipLaunchKernelGGL(crt_kernel<int16_t>, gc, dim3(256), 0, ctx->stream, (int)m, (int)n, (int)mp, (int)np,
+                       pl.nmod, reinterpret_cast<const int16_t*>(Cres), sA, tB, dcc, C, (long)ldc);
+  else
+    hipLaunchKernelGGL(crt_kernel<int8_t>, gc, dim3(256), 0, ctx->stream, (int)m, (int)n, (int)mp, (int)np,
+                       pl.nmod, reinterpret_cast<const int8_t*>(Cres), sA, tB, dcc, C, (long)ldc);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }
