@@ -349,7 +349,7 @@ __device__ __forceinline__ int swz(int r, int c) { return r * RBYTES + ((c ^ ((r
 // Pipeline per K stage (128 bytes of K, 4 MFMA k-steps): the global loads of stage st+1 are
 // issued at the top; the fragments of k-step ks+1 are read from LDS into a second register set
 // while the 4 x FN MFMAs of k-step ks run; the staged registers of st+1 are written into the
-// other LDS buffer in the middle of the stage (that buffer was last read before the previous
+// other LDS buffer during the last k-step (that buffer was last read before the previous
 // barrier); one barrier closes the stage.
 template <int MODE>
 using ResOut = typename std::conditional<MODE == 0, int16_t, int8_t>::type;
@@ -363,6 +363,8 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
   constexpr int ESZ = (MODE == 0) ? 2 : 1;          // bytes per element
   constexpr int KSTEP_B = 32;                        // bytes of K per MFMA (16 bf16 / 32 i8)
   constexpr int NKS = RBYTES / KSTEP_B;              // MFMA k-steps per stage
+  constexpr int kStoreStep = NKS - 1;  // k-step that carries the next stage's LDS stores (the
+                                       // last: the global loads get the whole stage to land)
   constexpr int BN = 64 * FN;                        // workgroup tile columns
   constexpr int NBCH = BN * 8 / 256;                 // B chunks staged per thread
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -482,12 +484,13 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
       const int cur = ks & 1;
       if (ks + 1 < NKS) lfrag(buf, ks + 1, fa[cur ^ 1], fb[cur ^ 1]);
       kstep(cur);
-      if (ks == NKS / 2) sstore(buf ^ 1);
+      if (ks == kStoreStep) sstore(buf ^ 1);
     }
     // instruction order of the stage: each MFMA of k-step 0 is followed by one global load of
     // the next stage, and the first 4 + FN MFMAs of k-steps 0..NKS-2 by one LDS fragment read
-    // of the following k-step; the LDS stores of the next stage ride on k-step NKS/2.
+    // of the following k-step; the LDS stores of the next stage ride on the last k-step.
     constexpr int NM = 4 * FN;
+    if constexpr (MODE == 1)  // (the BF16 tile measured faster with the compiler's own order)
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
@@ -495,7 +498,7 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
         __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // MFMA
         if (ks == 0 && t < 8 + NBCH) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
         if (ks + 1 < NKS && t < 4 + FN) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
-        if (ks == NKS / 2 && t < 8 + NBCH) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+        if (ks == kStoreStep && t < 8 + NBCH) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
       }
     if ((st & 3) == 3) reduce_acc();
     __syncthreads();
