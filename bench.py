@@ -339,26 +339,10 @@ class Bench:
                 "sample": f"oracle C restatement of Blaze's GEMM, {nn}^3, 1 core"}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--size", type=int, default=8192)
-    ap.add_argument("--systems", type=int, default=100_000)
-    ap.add_argument("--legs", default="gemm,solve,spmv,serve,cpu")
-    ap.add_argument("--variant", default="auto", choices=["auto", "bf16", "int8"])
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
-        sys.exit(2)
-    legs = set(args.legs.split(","))
-    b = Bench(args)
+def gemm_leg(b, rec, args):
+    """Headline: FP64-accurate GEMM vs native FP64 at args.size^3, error per sigma, rectangular case."""
     import blaze_amd as bz
     m = n = k = args.size
-    rec = {"metric": METRIC, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
     log(f"[gemm] generating {m}x{k} inputs")
     A = b.uniform((m, k), 1); B = b.uniform((k, n), 2)
     variants = [("fp64acc_bf16", "bzg_dgemm_fp64acc"), ("fp64acc_int8", "bzg_dgemm_fp64acc_i8")]
@@ -402,6 +386,31 @@ def main():
         rec["fp64acc_16384x1024x16384"] = b.gemm_case(mr, nr_, kr, Ar, Br, max(3, args.steps // 2), 1, variants)
         del Ar, Br
     del A, B
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=8192)
+    ap.add_argument("--systems", type=int, default=100_000)
+    ap.add_argument("--legs", default="gemm,solve,spmv,serve,cpu",
+                    help="comma list of gemm, solve, spmv, serve, cpu, copy (1 GiB copy for counter calibration)")
+    ap.add_argument("--variant", default="auto", choices=["auto", "bf16", "int8"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+    legs = set(args.legs.split(","))
+    b = Bench(args)
+    import blaze_amd as bz
+    m = n = k = args.size
+    rec = {"metric": METRIC, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    if "gemm" in legs:
+        gemm_leg(b, rec, args)
     if "solve" in legs:
         try:
             rec["solve"] = b.solve_leg(args.systems, max(3, args.steps // 2), 1)
@@ -426,6 +435,14 @@ def main():
             rec["serve"] = {"error": repr(e)}
     if "cpu" in legs:
         rec["cpu_baseline"] = b.cpu_baseline()
+    if "copy" in legs:
+        # counter calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE: one 1 GiB device copy
+        src = b.torch.ones(1 << 27, dtype=b.torch.float64, device="cuda")
+        dst = b.torch.empty_like(src)
+        b.sync()
+        dst.copy_(src)
+        b.sync()
+        rec["copy_bytes"] = 1 << 30
     print(json.dumps(rec), flush=True)
 
 

@@ -35,29 +35,33 @@ constexpr int NB = 16;        // largest panel width
 // kNB values per lane), 8 doubles or 16 floats keep it at 64 registers.
 template <typename T>
 constexpr int kNB = sizeof(T) == 8 ? 8 : 16;
-constexpr int NMAX = 256;     // largest system order
+constexpr int NMAX = 512;     // largest system order
+constexpr int NSMALL = 256;   // systems up to this order use the NM = 256 kernels
 constexpr int RMAX = 8;       // largest number of right-hand sides
 constexpr int UPAD = 16;      // row padding of the U12 buffer (conflict-free MFMA B reads)
 
-template <typename T>
+// NM: the largest order a kernel instance handles (256 or 512); it sizes the LDS buffers and
+// the register panel (NM / 64 rows per lane).
+template <typename T, int NM>
 struct FactorShm {
-  T panel[NMAX][kNB<T> + 1];      // L11 / L21 of the current panel (rows beyond n are zero)
-  T ublk[kNB<T>][NMAX + UPAD];    // U12 of the current panel
+  T panel[NM][kNB<T> + 1];      // L11 / L21 of the current panel (rows beyond n are zero)
+  T ublk[kNB<T>][NM + UPAD];    // U12 of the current panel
 };
 
-template <typename T>
+template <typename T, int NM>
 struct SolveShm {
-  T x[NMAX * RMAX];           // right-hand sides / solution, row-major, ld = RMAX
+  T x[NM * RMAX];             // right-hand sides / solution, row-major, ld = RMAX
   T diag[16 * 17];            // diagonal block being solved
 };
 
+template <int NM>
 struct Misc {
   double pv[2][4];            // per-wave best |a| of the column (double-buffered by step parity)
   int pr[2][4];               // ... and its row
   int piv[NB];                // absolute pivot rows of the current panel (0-based)
   int swsrc[2 * NB];          // interchange sources: rows j0.. (first NB), pivot rows (last NB)
   int swdst[2 * NB];          // ... and their destinations (-1: none)
-  int perm[NMAX];             // row i of the factored system = original row perm[i]
+  int perm[NM];               // row i of the factored system = original row perm[i]
   int sys;
   unsigned long long cmax[RMAX * 2];  // column maxima (bit patterns of non-negative doubles)
   int flag;
@@ -99,12 +103,13 @@ __device__ __forceinline__ float bcast(float v, int src) {
 // LU factorisation of the n x n row-major matrix A (leading dimension lda) in place.
 // Writes 1-based pivots to ipiv (may be null) and the row permutation to misc.perm.
 // Returns info (0 or 1-based index of the first exactly-zero pivot).
-template <typename T>
+template <typename T, int NM>
 __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int32_t* __restrict__ ipiv,
-                         FactorShm<T>& sh, Misc& misc) {
+                         FactorShm<T, NM>& sh, Misc<NM>& misc) {
   using MF = MfmaLU<T>;
   using acc_t = typename MF::acc_t;
   constexpr int NB = kNB<T>;
+  constexpr int Q = NM / 64;  // panel rows per lane
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int info = 0;
   for (int i = tid; i < n; i += NT) misc.perm[i] = i;
@@ -115,17 +120,17 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
     const int nbe = min(NB, m);
 
     // ---- panel factorisation (columns j0 .. j0+nbe-1) by wavefront 0 alone ----
-    // Lane l holds panel rows l, l+64, l+128, l+192 in registers.  Rows never move between
+    // Lane l holds panel rows l, l+64, l+128, ... (Q of them) in registers.  Rows never move between
     // registers: each keeps its logical position lpos (its row index after the interchanges
     // so far), an interchange just swaps two lpos values, and the rows are written to their
     // final positions at the end.  A column step is a per-lane max over its 4 rows, a
     // butterfly over the wavefront and v_readlane broadcasts of the pivot row: no LDS traffic
     // and no barrier per column.
     if (wave == 0) {
-      T P[4][NB];
-      int lpos[4];
+      T P[Q][NB];
+      int lpos[Q];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int rr = lane + 64 * q;
         lpos[q] = (rr < m) ? rr : (1 << 20);  // rows beyond the matrix never become candidates
 #pragma unroll
@@ -141,7 +146,7 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
           double v = -1.0;
           int r = 1 << 30, wq = 0;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < Q; ++q) {
             double a = (lpos[q] >= jj && lpos[q] < m) ? (double)tabs(P[q][jj]) : -1.0;
             if (a != a) a = -0.5;
             if (a > v || (a == v && lpos[q] < r)) { v = a; r = lpos[q]; wq = q; }
@@ -162,7 +167,7 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
           for (int c = 0; c < NB; ++c) {
             T cand = P[0][c];
 #pragma unroll
-            for (int q = 1; q < 4; ++q) cand = (pq == q) ? P[q][c] : cand;
+            for (int q = 1; q < Q; ++q) cand = (pq == q) ? P[q][c] : cand;
             pr[c] = bcast(cand, pl);
           }
           const T piv = pr[jj];
@@ -175,11 +180,11 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
             }
             // interchange logical rows jj and p
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < Q; ++q)
               lpos[q] = (lpos[q] == jj) ? p : ((lpos[q] == p) ? jj : lpos[q]);
             const T rinv = T(1) / piv;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < Q; ++q) {
               if (lpos[q] > jj && lpos[q] < m) {
                 const T l = P[q][jj] * rinv;
                 P[q][jj] = l;
@@ -201,7 +206,7 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
       }
       // ---- panel to HBM and LDS, each row at its final (logical) position ----
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int rr = lane + 64 * q;
         const int lr = lpos[q];
 #pragma unroll
@@ -455,7 +460,8 @@ __device__ __forceinline__ void lu_solve(const T* __restrict__ LU, int n, int ld
   }
 }
 
-__device__ __forceinline__ int next_system(int* counter, Misc& misc) {
+template <int NM>
+__device__ __forceinline__ int next_system(int* counter, Misc<NM>& misc) {
   if (threadIdx.x == 0) misc.sys = atomicAdd(counter, 1);
   __syncthreads();
   const int s = misc.sys;
@@ -464,7 +470,10 @@ __device__ __forceinline__ int next_system(int* counter, Misc& misc) {
 }
 
 // ---- plain FP64 (or FP32) dgesv: A and B in place --------------------------------------
-template <typename T>
+// NM = NSMALL: takes the systems of order <= NSMALL from the (ordered) index range and skips
+// the others; NM = NMAX: takes the systems listed in big_list (orders NSMALL+1 .. NMAX).
+// Systems outside 1..NMAX / 0..RMAX were marked info = -1 by classify_kernel.
+template <typename T, int NM>
 __global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32_t* __restrict__ nv,
                                                           const int32_t* __restrict__ rv,
                                                           const int64_t* __restrict__ a_off,
@@ -474,25 +483,25 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32
                                                           int32_t* __restrict__ ipiv,
                                                           int32_t* __restrict__ info,
                                                           const int32_t* __restrict__ order,
+                                                          const int32_t* __restrict__ big_list,
+                                                          const int* __restrict__ big_count,
                                                           int* counter) {
   __shared__ union {
-    FactorShm<T> f;
-    SolveShm<T> s;
+    FactorShm<T, NM> f;
+    SolveShm<T, NM> s;
   } sh;
-  __shared__ Misc misc;
+  __shared__ Misc<NM> misc;
   const int tid = threadIdx.x;
+  if (NM != NSMALL) count = *big_count;
   for (;;) {
     const int idx = next_system(counter, misc);
     if (idx >= count) break;
-    const int s = order ? order[idx] : idx;
+    const int s = (NM != NSMALL) ? big_list[idx] : (order ? order[idx] : idx);
     const int n = nv[s], nrhs = rv[s];
-    if (n < 1 || n > NMAX || nrhs < 0 || nrhs > RMAX) {  // outside the supported range
-      if (tid == 0) info[s] = -1;
-      continue;
-    }
+    if (NM == NSMALL && (n < 1 || n > NSMALL || nrhs < 0 || nrhs > RMAX)) continue;
     T* a = A + a_off[s];
     T* b = B + b_off[s];
-    const int inf = lu_factor<T>(a, n, n, ipiv ? ipiv + p_off[s] : nullptr, sh.f, misc);
+    const int inf = lu_factor<T, NM>(a, n, n, ipiv ? ipiv + p_off[s] : nullptr, sh.f, misc);
     if (tid == 0) info[s] = inf;
     if (inf == 0) {
       for (int e = tid; e < n * RMAX; e += NT) {
@@ -508,40 +517,39 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32
 }
 
 // ---- mixed precision: FP32 LU + FP64 refinement (dsgesv) --------------------------------
+template <int NM>
 struct IRShm {
-  double x[NMAX * RMAX];
-  double r[NMAX * RMAX];
-  float d[NMAX * RMAX];
+  double x[NM * RMAX];
+  double r[NM * RMAX];
+  float d[NM * RMAX];
   float diag[16 * 17];
 };
 
-__device__ __forceinline__ void col_max_reset(Misc& misc) {
-  if (threadIdx.x < 2 * RMAX) misc.cmax[threadIdx.x] = 0ull;
-}
-
+// Same work split as gesv_batched_kernel (NM = NSMALL: ordered range, skipping the others;
+// NM = NMAX: big_list).
+template <int NM>
 __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
     int count, const int32_t* __restrict__ nv, const int32_t* __restrict__ rv,
     const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
     const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ X,
     int32_t* __restrict__ iter, int32_t* __restrict__ info, const int32_t* __restrict__ order,
+    const int32_t* __restrict__ big_list, const int* __restrict__ big_count,
     int* counter, float* __restrict__ ws32, int32_t* __restrict__ fallback_list,
     int* fallback_count, double tol, int max_iter) {
   __shared__ union {
-    FactorShm<float> f;
-    IRShm ir;
+    FactorShm<float, NM> f;
+    IRShm<NM> ir;
   } sh;
-  __shared__ Misc misc;
+  __shared__ Misc<NM> misc;
   const int tid = threadIdx.x;
-  float* W = ws32 + (long)blockIdx.x * NMAX * NMAX;
+  float* W = ws32 + (long)blockIdx.x * NM * NM;
+  if (NM != NSMALL) count = *big_count;
   for (;;) {
     const int idx = next_system(counter, misc);
     if (idx >= count) break;
-    const int s = order ? order[idx] : idx;
+    const int s = (NM != NSMALL) ? big_list[idx] : (order ? order[idx] : idx);
     const int n = nv[s], nrhs = rv[s];
-    if (n < 1 || n > NMAX || nrhs < 0 || nrhs > RMAX) {
-      if (tid == 0) { info[s] = -1; iter[s] = 0; }
-      continue;
-    }
+    if (NM == NSMALL && (n < 1 || n > NSMALL || nrhs < 0 || nrhs > RMAX)) continue;
     const double* a = A + a_off[s];
     const double* b = B + b_off[s];
     double* x = X + b_off[s];
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
     if (tid < 2 * RMAX) misc.cmax[tid] = 0ull;
     __syncthreads();
     {
-      // 16 lanes per row, 4 rows per step, up to 16 loads per lane in flight; the row
+      // 16 lanes per row, 4 rows per step, 16 loads per lane in flight per 256 columns; the row
       // sums of |a| use a fixed order (lane-strided partial sums, then a butterfly) so
       // ||A||_inf and with it the stopping test are reproducible run to run
       const int r0 = tid / 16, ks = tid % 16;
@@ -560,17 +568,19 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
         if (r < n) {
           const double* ar = a + (long)r * n;
           float* wr = W + (long)r * n;
-          double v[16];
+          for (int k0 = 0; k0 < n; k0 += 256) {
+            double v[16];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) {
-            const int k = ks + 16 * u;
-            v[u] = (k < n) ? ar[k] : 0.0;
-          }
+            for (int u = 0; u < 16; ++u) {
+              const int k = k0 + ks + 16 * u;
+              v[u] = (k < n) ? ar[k] : 0.0;
+            }
 #pragma unroll
-          for (int u = 0; u < 16; ++u) {
-            const int k = ks + 16 * u;
-            if (k < n) wr[k] = (float)v[u];
-            sum += fabs(v[u]);
+            for (int u = 0; u < 16; ++u) {
+              const int k = k0 + ks + 16 * u;
+              if (k < n) wr[k] = (float)v[u];
+              sum += fabs(v[u]);
+            }
           }
         }
 #pragma unroll
@@ -586,7 +596,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
     const double eps = tol > 0 ? tol : 0x1.0p-53;
     const double cte = anrm * eps * sqrt((double)n);
     __syncthreads();
-    const int inf32 = lu_factor<float>(W, n, n, nullptr, sh.f, misc);
+    const int inf32 = lu_factor<float, NM>(W, n, n, nullptr, sh.f, misc);
     int it_out = -2;
     if (inf32 == 0) {
       // x = FP32 solve of P b
@@ -668,7 +678,9 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
 }
 
 // FP64 fallback for the systems listed by the mixed-precision kernel: copy A to the
-// workgroup's FP64 scratch, factor, solve B into X.
+// workgroup's FP64 scratch, factor, solve B into X.  One instance (NM = NMAX) serves every
+// order: fallbacks are rare.
+template <int NM>
 __global__ __launch_bounds__(NT) void gesv_fallback_kernel(
     const int32_t* __restrict__ nv, const int32_t* __restrict__ rv,
     const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
@@ -676,12 +688,12 @@ __global__ __launch_bounds__(NT) void gesv_fallback_kernel(
     int32_t* __restrict__ info, const int32_t* __restrict__ list, const int* list_count,
     int* counter, double* __restrict__ ws64) {
   __shared__ union {
-    FactorShm<double> f;
-    SolveShm<double> s;
+    FactorShm<double, NM> f;
+    SolveShm<double, NM> s;
   } sh;
-  __shared__ Misc misc;
+  __shared__ Misc<NM> misc;
   const int tid = threadIdx.x;
-  double* W = ws64 + (long)blockIdx.x * NMAX * NMAX;
+  double* W = ws64 + (long)blockIdx.x * NM * NM;
   const int count = *list_count;
   for (;;) {
     const int idx = next_system(counter, misc);
@@ -691,7 +703,7 @@ __global__ __launch_bounds__(NT) void gesv_fallback_kernel(
     const double* a = A + a_off[s];
     for (int e = tid; e < n * n; e += NT) W[e] = a[e];
     __syncthreads();
-    const int inf = lu_factor<double>(W, n, n, nullptr, sh.f, misc);
+    const int inf = lu_factor<double, NM>(W, n, n, nullptr, sh.f, misc);
     if (tid == 0) info[s] = inf;
     if (inf == 0) {
       const double* b = B + b_off[s];
@@ -710,6 +722,24 @@ __global__ __launch_bounds__(NT) void gesv_fallback_kernel(
 
 __global__ void reset_counters_kernel(int* c, int n) {
   if (threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+// Marks systems outside the supported range (info = -1, iter = 0 when iter != null) and lists
+// the systems of order NSMALL+1 .. NMAX for the NM = NMAX kernels.
+__global__ __launch_bounds__(256) void classify_kernel(int count, const int32_t* __restrict__ nv,
+                                                       const int32_t* __restrict__ rv,
+                                                       int32_t* __restrict__ info,
+                                                       int32_t* __restrict__ iter,
+                                                       int32_t* __restrict__ big_list, int* big_count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= count) return;
+  const int n = nv[s], nrhs = rv[s];
+  if (n < 1 || n > NMAX || nrhs < 0 || nrhs > RMAX) {
+    info[s] = -1;
+    if (iter) iter[s] = 0;
+  } else if (n > NSMALL) {
+    big_list[atomicAdd(big_count, 1)] = s;
+  }
 }
 
 }  // namespace
@@ -762,17 +792,30 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, 
               "dgesv_batched: null array");
   BZG_REQUIRE(ipiv == nullptr || p_off != nullptr, BZG_ERR_INVALID_ARGUMENT,
               "dgesv_batched: ipiv without p_off");
-  // Scratch: [0] work counter; [1..] (unused)
-  void* scratch = nullptr;
-  bzg_status st = bzg::workspace(ctx, 0, 256, &scratch);
+  // Scratch: slot 0 = counters [0] small-kernel work counter, [1] big count, [2] big-kernel
+  // work counter; slot 3 = list of the systems of order > NSMALL
+  void *cnt = nullptr, *lst = nullptr;
+  bzg_status st = bzg::workspace(ctx, 0, 256, &cnt);
+  if (!st) st = bzg::workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
   if (st) return st;
-  int* counter = static_cast<int*>(scratch);
+  int* counters = static_cast<int*>(cnt);
+  int32_t* big = static_cast<int32_t*>(lst);
   bzg::ProfScope prof(ctx, "dgesv");
-  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counter, 4);
+  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 8);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bzg::classify_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (int)count, n, nrhs, info, (int32_t*)nullptr, big, counters + 1);
+  BZG_LAUNCH_CHECK();
+  // the few systems of order > NSMALL first (their kernel holds 2 workgroups per CU), then
+  // the rest at full occupancy
+  hipLaunchKernelGGL((bzg::gesv_batched_kernel<double, bzg::NMAX>), dim3(ctx->num_cus * 2), dim3(bzg::NT), 0,
+                     ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
+                     (const int32_t*)big, (const int*)(counters + 1), counters + 2);
   BZG_LAUNCH_CHECK();
   const int grid = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
-  hipLaunchKernelGGL(bzg::gesv_batched_kernel<double>, dim3(grid), dim3(bzg::NT), 0, ctx->stream,
-                     (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order, counter);
+  hipLaunchKernelGGL((bzg::gesv_batched_kernel<double, bzg::NSMALL>), dim3(grid), dim3(bzg::NT), 0,
+                     ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
+                     (const int32_t*)big, (const int*)(counters + 1), counters + 0);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }
@@ -788,26 +831,40 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
               "dsgesv_batched: null array");
   BZG_REQUIRE(max_iter >= 0 && max_iter <= 1000, BZG_ERR_INVALID_ARGUMENT, "dsgesv_batched: bad max_iter");
   const int grid32 = ctx->num_cus * bzg::g_tuning.wg_per_cu_mixed;
-  const int grid64 = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
-  // scratch layout (slot 0): counters; slot 1: FP32 per-workgroup matrices; slot 2: FP64
-  // per-workgroup matrices; slot 3: fallback list
-  void *cnt = nullptr, *w32 = nullptr, *w64 = nullptr, *lst = nullptr;
+  const int grid_big = ctx->num_cus * 2;
+  // scratch: slot 0 counters ([0] small work, [1] fallback count, [2] fallback work,
+  // [3] big count, [4] big work); slot 1 FP32 matrices of the small kernel's workgroups;
+  // slot 5 FP32 matrices of the big kernel's; slot 2 FP64 matrices of the fallback kernel;
+  // slot 3 fallback list then big list
+  void *cnt = nullptr, *w32 = nullptr, *w32b = nullptr, *w64 = nullptr, *lst = nullptr;
+  const size_t nn_small = (size_t)bzg::NSMALL * bzg::NSMALL, nn_big = (size_t)bzg::NMAX * bzg::NMAX;
   bzg_status st = bzg::workspace(ctx, 0, 256, &cnt);
-  if (!st) st = bzg::workspace(ctx, 1, (size_t)grid32 * bzg::NMAX * bzg::NMAX * sizeof(float), &w32);
-  if (!st) st = bzg::workspace(ctx, 2, (size_t)grid64 * bzg::NMAX * bzg::NMAX * sizeof(double), &w64);
-  if (!st) st = bzg::workspace(ctx, 3, (size_t)count * sizeof(int32_t), &lst);
+  if (!st) st = bzg::workspace(ctx, 1, (size_t)grid32 * nn_small * sizeof(float), &w32);
+  if (!st) st = bzg::workspace(ctx, 5, (size_t)grid_big * nn_big * sizeof(float), &w32b);
+  if (!st) st = bzg::workspace(ctx, 2, (size_t)grid_big * nn_big * sizeof(double), &w64);
+  if (!st) st = bzg::workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
   if (st) return st;
   int* counters = static_cast<int*>(cnt);
+  int32_t* fb_list = static_cast<int32_t*>(lst);
+  int32_t* big = fb_list + count;
   bzg::ProfScope prof(ctx, "dsgesv");
-  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 4);
+  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 8);
   BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bzg::dsgesv_batched_kernel, dim3(grid32), dim3(bzg::NT), 0, ctx->stream,
-                     (int)count, n, nrhs, a_off, b_off, A, B, X, iter, info,
-                     order, counters + 0, static_cast<float*>(w32),
-                     static_cast<int32_t*>(lst), counters + 1, tol, max_iter);
+  hipLaunchKernelGGL(bzg::classify_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (int)count, n, nrhs, info, iter, big, counters + 3);
   BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bzg::gesv_fallback_kernel, dim3(grid64), dim3(bzg::NT), 0, ctx->stream, n, nrhs,
-                     a_off, b_off, A, B, X, info, static_cast<const int32_t*>(lst), counters + 1,
+  hipLaunchKernelGGL(bzg::dsgesv_batched_kernel<bzg::NMAX>, dim3(grid_big), dim3(bzg::NT), 0, ctx->stream,
+                     (int)count, n, nrhs, a_off, b_off, A, B, X, iter, info, order,
+                     (const int32_t*)big, (const int*)(counters + 3), counters + 4,
+                     static_cast<float*>(w32b), fb_list, counters + 1, tol, max_iter);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bzg::dsgesv_batched_kernel<bzg::NSMALL>, dim3(grid32), dim3(bzg::NT), 0, ctx->stream,
+                     (int)count, n, nrhs, a_off, b_off, A, B, X, iter, info, order,
+                     (const int32_t*)big, (const int*)(counters + 3), counters + 0,
+                     static_cast<float*>(w32), fb_list, counters + 1, tol, max_iter);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bzg::gesv_fallback_kernel<bzg::NMAX>, dim3(grid_big), dim3(bzg::NT), 0, ctx->stream, n,
+                     nrhs, a_off, b_off, A, B, X, info, static_cast<const int32_t*>(fb_list), counters + 1,
                      counters + 2, static_cast<double*>(w64));
   BZG_LAUNCH_CHECK();
   return BZG_OK;

@@ -671,8 +671,8 @@ inline void solve_into(const DynamicMatrix<double>& A, double* B_dev, size_t nrh
   const size_t n = A.rows();
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
   if (n == 0) return;
-  if (n > 256 || nrhs > 8)
-    throw std::invalid_argument("solve: this build supports n <= 256 and up to 8 right-hand sides");
+  if (n > 512 || nrhs > 8)
+    throw std::invalid_argument("solve: this build supports n <= 512 and up to 8 right-hand sides");
   DeviceBuffer<double> Acopy(n * n);
   d2d(Acopy.ptr, A.data(), n * n);
   const int32_t nn = (int32_t)n, rr = (int32_t)nrhs;

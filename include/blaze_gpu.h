@@ -139,7 +139,9 @@ bzg_status bzg_icsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t*
  * ipiv + p_off[i] (n[i] entries).  info[i] = 0 or the 1-based index of an exactly-zero
  * pivot.  Row-major storage means the factorisation is of A itself (not of A^T), with
  * partial pivoting by rows exactly as dgetf2: the first row of maximal |a|.
- * 1 <= n[i] <= 256, 0 <= nrhs[i] <= 8; a system outside that range gets info[i] = -1.
+ * 1 <= n[i] <= 512, 0 <= nrhs[i] <= 8; a system outside that range gets info[i] = -1.
+ * Orders up to 256 run at 8 workgroups per CU, orders 257..512 on a second kernel instance
+ * with twice the LDS (2 per CU).
  * `order` (device, may be NULL) lists the systems in the order the persistent workgroups
  * should take them -- bzg_batched_order() gives longest-first; NULL = index order.
  * ipiv may be NULL (then p_off is ignored). */

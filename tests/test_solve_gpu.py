@@ -111,3 +111,33 @@ def test_dsgesv_well_and_ill_conditioned():
         assert (it >= 0) == (it_o >= 0), (A.shape, it, it_o)
     assert (iters[:6] >= 0).all() and (iters[:6] <= 5).all()
     assert iters[8] == -3  # kappa = 1e13: refinement stalls -> FP64 fallback
+
+
+def test_orders_above_256_and_out_of_range():
+    # n in (256, 512] runs on the second kernel instance; both instances share one batch.
+    from blaze_amd import solve
+    rng = np.random.default_rng(44)
+    sizes = [257, 300, 5, 384, 511, 64, 512, 256]
+    As = [rng.uniform(-1, 1, (n, n)) for n in sizes]        # general: exercises pivoting
+    Bs = [rng.uniform(-1, 1, (n, 2)) for n in sizes]
+    Xs, infos, LUs, ipivs = solve.gesv_batched(ctx(), As, Bs, with_ipiv=True)
+    for A, B, X, info, LU, ip in zip(As, Bs, Xs, infos, LUs, ipivs):
+        n = A.shape[0]
+        Xo, LUo, ipo, info_o = oracle.gesv(A, B)
+        assert info == info_o == 0, n
+        assert np.array_equal(ip, ipo), n
+        assert np.abs(LU - LUo).max() <= 1e-12 * max(1, np.abs(LUo).max()) * n, n
+        assert oracle.backward_error(A, B, X) <= 1e-14 * n, n
+    # mixed precision on the same mix (diagonally dominant so FP32 refinement converges)
+    Ad = [A + A.shape[0] * np.eye(A.shape[0]) for A in As]
+    Xm, iters, infom = solve.dsgesv_batched(ctx(), Ad, Bs)
+    assert (infom == 0).all() and (iters >= 0).all()
+    for A, B, X in zip(Ad, Bs, Xm):
+        assert oracle.backward_error(A, B, X) <= 1e-12
+    # orders beyond 512 and more than 8 right-hand sides are rejected per system (info = -1)
+    bad_A = [np.eye(513), np.eye(3), np.eye(4)]
+    bad_B = [np.ones((513, 1)), np.ones((3, 9)), np.ones((4, 1))]
+    _, infob = solve.gesv_batched(ctx(), bad_A, bad_B)
+    assert list(infob) == [-1, -1, 0]
+    _, itb, infob2 = solve.dsgesv_batched(ctx(), bad_A, bad_B)
+    assert list(infob2) == [-1, -1, 0] and itb[0] == 0 and itb[1] == 0
