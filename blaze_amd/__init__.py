@@ -55,6 +55,16 @@ SIGNATURES = {
     "bzg_dgemm_fp64acc_i8": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]),
     "bzg_fp64acc_slices": (i32, [i64, i32]),
     "bzg_fp64acc_operand_bits": (i32, [i64, i32, i32]),
+    "bzg_server_create": (i32, [vp, C.POINTER(vp)]),
+    "bzg_server_destroy": (i32, [vp]),
+    "bzg_server_add_csr": (i32, [vp, i64, i64, vp, vp, vp, C.POINTER(i32)]),
+    "bzg_submit_gemm": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, C.POINTER(i64)]),
+    "bzg_submit_spmv": (i32, [vp, i32, vp, C.POINTER(i64)]),
+    "bzg_submit_solve": (i32, [vp, i32, i32, vp, vp, C.POINTER(i64)]),
+    "bzg_poll": (i32, [vp, i64, C.POINTER(i32)]),
+    "bzg_wait": (i32, [vp, i64, vp]),
+    "bzg_release": (i32, [vp, i64]),
+    "bzg_server_get_stats": (i32, [vp, vp]),
 }
 
 

@@ -1,6 +1,8 @@
 // Context, stream and memory entry points of the C ABI.
 #include "common.hpp"
 
+#include <algorithm>
+
 #include <cstring>
 
 namespace bzg {
@@ -10,6 +12,9 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out) {
   Workspace& w = ctx->ws[slot];
   if (w.bytes < bytes) {
+    // small buffers grow geometrically so that slowly growing requests (batch metadata)
+    // do not reallocate on every call; large ones exactly
+    if (w.bytes && bytes < (64u << 20)) bytes = std::max(bytes, 2 * w.bytes);
     if (w.ptr) {
       BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
       BZG_HIP_CHECK(hipFree(w.ptr));

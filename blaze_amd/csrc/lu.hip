@@ -808,11 +808,14 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, 
   BZG_LAUNCH_CHECK();
   // the few systems of order > NSMALL first (their kernel holds 2 workgroups per CU), then
   // the rest at full occupancy
-  hipLaunchKernelGGL((bzg::gesv_batched_kernel<double, bzg::NMAX>), dim3(ctx->num_cus * 2), dim3(bzg::NT), 0,
+  // persistent grids never larger than the batch (a few systems: a few workgroups)
+  const int64_t cap = std::max<int64_t>(count, 1);
+  hipLaunchKernelGGL((bzg::gesv_batched_kernel<double, bzg::NMAX>),
+                     dim3((unsigned)std::min<int64_t>(ctx->num_cus * 2, cap)), dim3(bzg::NT), 0,
                      ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
                      (const int32_t*)big, (const int*)(counters + 1), counters + 2);
   BZG_LAUNCH_CHECK();
-  const int grid = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
+  const int grid = (int)std::min<int64_t>(ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64, cap);
   hipLaunchKernelGGL((bzg::gesv_batched_kernel<double, bzg::NSMALL>), dim3(grid), dim3(bzg::NT), 0,
                      ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
                      (const int32_t*)big, (const int*)(counters + 1), counters + 0);
@@ -830,8 +833,13 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
   BZG_REQUIRE(n && nrhs && a_off && b_off && A && B && X && iter && info, BZG_ERR_INVALID_ARGUMENT,
               "dsgesv_batched: null array");
   BZG_REQUIRE(max_iter >= 0 && max_iter <= 1000, BZG_ERR_INVALID_ARGUMENT, "dsgesv_batched: bad max_iter");
-  const int grid32 = ctx->num_cus * bzg::g_tuning.wg_per_cu_mixed;
-  const int grid_big = ctx->num_cus * 2;
+  // persistent grids never larger than the batch; the workspace is sized for the full grid
+  // so that it is not regrown as batch sizes vary
+  const int64_t cap = std::max<int64_t>(count, 1);
+  const int grid32_full = ctx->num_cus * bzg::g_tuning.wg_per_cu_mixed;
+  const int grid_big_full = ctx->num_cus * 2;
+  const int grid32 = (int)std::min<int64_t>(grid32_full, cap);
+  const int grid_big = (int)std::min<int64_t>(grid_big_full, cap);
   // scratch: slot 0 counters ([0] small work, [1] fallback count, [2] fallback work,
   // [3] big count, [4] big work); slot 1 FP32 matrices of the small kernel's workgroups;
   // slot 5 FP32 matrices of the big kernel's; slot 2 FP64 matrices of the fallback kernel;
@@ -839,9 +847,9 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
   void *cnt = nullptr, *w32 = nullptr, *w32b = nullptr, *w64 = nullptr, *lst = nullptr;
   const size_t nn_small = (size_t)bzg::NSMALL * bzg::NSMALL, nn_big = (size_t)bzg::NMAX * bzg::NMAX;
   bzg_status st = bzg::workspace(ctx, 0, 256, &cnt);
-  if (!st) st = bzg::workspace(ctx, 1, (size_t)grid32 * nn_small * sizeof(float), &w32);
-  if (!st) st = bzg::workspace(ctx, 5, (size_t)grid_big * nn_big * sizeof(float), &w32b);
-  if (!st) st = bzg::workspace(ctx, 2, (size_t)grid_big * nn_big * sizeof(double), &w64);
+  if (!st) st = bzg::workspace(ctx, 1, (size_t)grid32_full * nn_small * sizeof(float), &w32);
+  if (!st) st = bzg::workspace(ctx, 5, (size_t)grid_big_full * nn_big * sizeof(float), &w32b);
+  if (!st) st = bzg::workspace(ctx, 2, (size_t)grid_big_full * nn_big * sizeof(double), &w64);
   if (!st) st = bzg::workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
   if (st) return st;
   int* counters = static_cast<int*>(cnt);

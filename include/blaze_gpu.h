@@ -21,8 +21,11 @@
  *   bzg_dgesv_batched                  <- solve() (blaze/math/dense/LSE.h:6343-6373) ->
  *        gesv (blaze/math/lapack/gesv.h:136-175) -> dgesv_ (blaze/math/lapack/clapack/gesv.h:207)
  *   bzg_dsgesv_batched                 <- same call site; LAPACK dsgesv semantics (mixed precision)
- *   bzg_dgemm_fp64acc                  <- no reference function: FP64-accurate GEMM from BF16
- *        residue slices; drop-in for bzg_dgemm at the same call site (DMatDMatMultExpr.h:516)
+ *   bzg_dgemm_fp64acc[_i8]             <- no reference function: FP64-accurate GEMM from BF16
+ *        (INT8) residue slices; drop-in for bzg_dgemm at the same call site
+ *        (DMatDMatMultExpr.h:516)
+ *   bzg_server_* / bzg_submit_*        <- no reference function: a request server over the same
+ *        operations for concurrent clients
  */
 #ifndef BLAZE_GPU_H
 #define BLAZE_GPU_H
@@ -109,10 +112,11 @@ bzg_status bzg_igemv(bzg_context* ctx, int64_t m, int64_t n, int32_t alpha, cons
  * (the reference order, SVecDVecInnerExpr.h:127-135) or as a fixed-shape tree; results are
  * deterministic run to run.
  *
- * A plan analyses the row-length histogram once (host side, O(n_rows)) and picks the kernel:
- * short rows -> several rows per wavefront (vector-CSR with 2..64 lanes per row), long rows
- * -> one wavefront per row; rows are grouped in nnz-balanced blocks.  The plan owns device
- * metadata; the matrix arrays stay owned by the caller. */
+ * A plan analyses the row-length histogram once (host side, O(n_rows)) and picks the kernels:
+ * rows of up to 32 nonzeros are summed sequentially, either in nnz-balanced row blocks staged
+ * through LDS (2048 or 4096 nonzeros per block) or by lane groups (2..16 lanes per row);
+ * rows of 33..256 nonzeros get a wavefront each, longer rows a workgroup (fixed-shape tree
+ * sums).  The plan owns device metadata; the matrix arrays stay owned by the caller. */
 typedef struct bzg_csr_plan bzg_csr_plan;
 bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
                                const int32_t* rowptr_host, bzg_csr_plan** plan);
@@ -185,6 +189,65 @@ int32_t bzg_fp64acc_slices(int64_t k, int32_t moduli);
 /* Bits P kept per scaled operand (|A'|,|B'| <= 2^P) for this k / moduli choice; the
  * elementwise error is bounded by k * 2^-P * max|A_i,:| * max|B_:,j|.  -1 if invalid. */
 int32_t bzg_fp64acc_operand_bits(int64_t k, int32_t moduli, int32_t int8_slices);
+
+
+/* ---- continuous-batching server -----------------------------------------------------------
+ * No reference counterpart (the reference evaluates one expression per call); it serves the
+ * same three operations to many concurrent clients.  Operands are DEVICE pointers that stay
+ * valid until the request completes; CSR matrices are uploaded once (bzg_server_add_csr) and
+ * stay resident.  Results (C, y or X, row-major, ld = n / 1 / nrhs) are written to server-owned
+ * HBM and stay valid until bzg_release.  A request is admitted when fewer than max_inflight
+ * launches are running and its result (plus staging) fits in hbm_budget_bytes together with
+ * everything not yet released and the resident matrices; admission is FIFO, and all admissible
+ * solve requests waiting behind the head join its batch (one bzg_dsgesv_batched launch, FP32 LU +
+ * FP64 refinement to LAPACK dsgesv's test, FP64 fallback).  Work is spread over `streams` HIP
+ * streams so that requests overlap on the device.  Each result is bit-identical to the same
+ * request run alone.  All functions are thread-safe. */
+typedef struct bzg_server bzg_server;
+typedef struct {
+  size_t hbm_budget_bytes; /* admission budget: unreleased results + staging + resident CSR */
+  int32_t max_inflight;    /* launches on the device at once (>= 1) */
+  int32_t streams;         /* HIP streams / library contexts used by the server (1..16) */
+  int32_t solve_batch_max; /* largest number of solve requests fused into one launch */
+} bzg_server_config;
+typedef struct {
+  bzg_status status;      /* launch status (0 ok) */
+  int32_t kind;           /* 0 GEMM, 1 SpMV, 2 solve */
+  int32_t info;           /* solve: LAPACK info (0 or first zero pivot; -1 out of range) */
+  int32_t iterations;     /* solve: refinement steps (>= 0) or < 0 when the FP64 LU was used */
+  double* result;         /* device pointer of the result */
+  int64_t result_elems;
+  double t_submit_us, t_admit_us, t_done_us; /* steady-clock microseconds */
+} bzg_request_result;
+typedef struct {
+  int64_t submitted, admitted, completed, released;
+  int64_t launches, solve_batches, max_solve_batch, inflight;
+  int64_t bytes_in_use, bytes_peak, budget;
+} bzg_server_stats;
+
+bzg_status bzg_server_create(const bzg_server_config* cfg, bzg_server** server);
+/* Waits for every admitted request, then frees all server memory (results included). */
+bzg_status bzg_server_destroy(bzg_server* server);
+/* Uploads a double CSR matrix (host arrays, same format as bzg_dcsrmv) and builds its plan;
+ * it counts against the budget for the server's lifetime. */
+bzg_status bzg_server_add_csr(bzg_server* server, int64_t n_rows, int64_t n_cols,
+                              const int32_t* rowptr, const int32_t* colidx, const double* val,
+                              int32_t* handle);
+/* C = A * B (A m x k, B k x n, device, row-major with leading dimensions). */
+bzg_status bzg_submit_gemm(bzg_server* server, int64_t m, int64_t n, int64_t k, const double* A,
+                           int64_t lda, const double* B, int64_t ldb, int64_t* id);
+/* y = S * x for a resident matrix (x device, n_cols entries). */
+bzg_status bzg_submit_spmv(bzg_server* server, int32_t csr, const double* x, int64_t* id);
+/* X = A^-1 B (A n x n, B n x nrhs, device, row-major, ld n / nrhs), 1 <= n <= 512,
+ * 1 <= nrhs <= 8. */
+bzg_status bzg_submit_solve(bzg_server* server, int32_t n, int32_t nrhs, const double* A,
+                            const double* B, int64_t* id);
+bzg_status bzg_poll(bzg_server* server, int64_t id, int32_t* done);
+/* Blocks until the request has completed and describes it. */
+bzg_status bzg_wait(bzg_server* server, int64_t id, bzg_request_result* out);
+/* Frees the request's result (its bytes return to the budget); BZG_ERR_BUSY while running. */
+bzg_status bzg_release(bzg_server* server, int64_t id);
+bzg_status bzg_server_get_stats(bzg_server* server, bzg_server_stats* out);
 
 #ifdef __cplusplus
 }
