@@ -36,6 +36,9 @@ SIGNATURES = {
     "bzg_memcpy": (i32, [vp, vp, vp, C.c_size_t, C.c_int]),
     "bzg_memset_zero": (i32, [vp, vp, C.c_size_t]),
     "bzg_dgemm": (i32, [vp, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64]),
+    "bzg_dgemm_grouped": (i32, [vp, i64, vp, vp]),
+    "bzg_gemm_autotune": (i32, [vp, i32, i64, i64, i64, C.POINTER(i32), C.POINTER(f32)]),
+    "bzg_gemm_tuning_clear": (i32, []),
     "bzg_sgemm": (i32, [vp, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64]),
     "bzg_igemm": (i32, [vp, i64, i64, i64, i32, vp, i64, vp, i64, i32, vp, i64]),
     "bzg_dgemv": (i32, [vp, i64, i64, f64, vp, i64, vp, f64, vp]),
@@ -45,12 +48,14 @@ SIGNATURES = {
     "bzg_csr_plan_destroy": (i32, [vp]),
     "bzg_csr_plan_tune": (i32, [vp, i32, i32, i32]),
     "bzg_csr_plan_describe": (i32, [vp, C.c_char_p, C.c_size_t]),
+    "bzg_csr_plan_autotune": (i32, [vp, vp, i32, vp, vp, vp, vp, vp, C.POINTER(f32)]),
     "bzg_dcsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_scsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_icsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dsgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, i32, vp]),
     "bzg_batched_order": (i32, [vp, i64, vp, vp]),
+    "bzg_solver_autotune": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, i64, vp, i64, vp, C.POINTER(i32), C.POINTER(f32)]),
     "bzg_dgemm_fp64acc": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]),
     "bzg_dgemm_fp64acc_i8": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]),
     "bzg_fp64acc_slices": (i32, [i64, i32]),

@@ -15,6 +15,10 @@
 // micro-tiles in wrap-around uint32 arithmetic (bit-exact against any summation order).
 #include "common.hpp"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace bzg {
 namespace {
 
@@ -75,12 +79,20 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int& tm, i
   tn = in_group / gsize;
 }
 
+template <typename T, int BM, int BN>
+struct GemmShm {
+  T As[2][BM * (kBK + kPadA)];
+  T Bs[2][kBK * (BN + kPadB)];
+};
+
+// One BM x BN tile (tm, tn) of C = alpha A B + beta C by the 4 wavefronts of the workgroup.
+// Every element of C is summed in the same k order (slabs of kBK, MFMA k-steps of 4) whatever
+// the tile shape, so all tile shapes give bit-identical results.
 template <typename T, int BM, int BN, int WM, int WN, bool VEC>
-__global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, int K, T alpha,
-                                                             const T* __restrict__ A, long lda,
-                                                             const T* __restrict__ B, long ldb,
-                                                             T beta, T* __restrict__ C,
-                                                             long ldc) {
+__device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T* __restrict__ A,
+                                          long lda, const T* __restrict__ B, long ldb, T beta,
+                                          T* __restrict__ C, long ldc, int tm, int tn,
+                                          GemmShm<T, BM, BN>& sh) {
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
   using V2 = typename Vec2<T>::type;
@@ -98,8 +110,8 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, in
   static_assert(NA >= 1 && NB >= 1, "tile too small for 256 threads");
   static_assert((BM / WM) * (BN / WN) == 4, "four wavefronts per workgroup");
 
-  __shared__ T As[2][BM * LDA_S];
-  __shared__ T Bs[2][kBK * LDB_S];
+  auto& As = sh.As;
+  auto& Bs = sh.Bs;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -107,10 +119,6 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, in
   const int wm0 = (wave / WAVES_N) * WM;
   const int wn0 = (wave % WAVES_N) * WN;
 
-  const int tiles_m = (M + BM - 1) / BM;
-  const int tiles_n = (N + BN - 1) / BN;
-  int tm, tn;
-  tile_coords(tiles_m, tiles_n, tm, tn);
   const int m0 = tm * BM;
   const int n0 = tn * BN;
 
@@ -213,6 +221,102 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, in
       }
 }
 
+template <typename T, int BM, int BN, int WM, int WN, bool VEC>
+__global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, int K, T alpha,
+                                                             const T* __restrict__ A, long lda,
+                                                             const T* __restrict__ B, long ldb,
+                                                             T beta, T* __restrict__ C,
+                                                             long ldc) {
+  __shared__ GemmShm<T, BM, BN> sh;
+  const int tiles_m = (M + BM - 1) / BM;
+  const int tiles_n = (N + BN - 1) / BN;
+  int tm, tn;
+  tile_coords(tiles_m, tiles_n, tm, tn);
+  gemm_tile<T, BM, BN, WM, WN, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, sh);
+}
+
+// ---- grouped GEMM: many independent products of uneven sizes in one persistent launch ----
+constexpr int kGT = 64;  // tile of the grouped kernel (64 x 64, waves of 32 x 32)
+
+// Tiles of one problem; a problem that fails the bzg_dgemm argument rules gets none (and
+// status -1 when a status array is given), an empty one none (status 0).
+__device__ __forceinline__ int64_t grouped_tiles(const bzg_dgemm_problem& q, int32_t* status) {
+  const bool ok = q.m >= 0 && q.n >= 0 && q.k >= 0 && q.m < (1LL << 31) && q.n < (1LL << 31) &&
+                  q.k < (1LL << 31) && q.lda >= (q.k > 0 ? q.k : 1) && q.ldb >= (q.n > 0 ? q.n : 1) &&
+                  q.ldc >= (q.n > 0 ? q.n : 1) && (q.m == 0 || q.n == 0 || q.C != nullptr) &&
+                  (q.m == 0 || q.n == 0 || q.k == 0 || (q.A != nullptr && q.B != nullptr));
+  if (status) *status = ok ? 0 : -1;
+  if (!ok || q.m == 0 || q.n == 0) return 0;
+  return ((q.m + kGT - 1) / kGT) * ((q.n + kGT - 1) / kGT);
+}
+
+// prefix[p] = number of 64 x 64 tiles of problems 0..p-1 (p = 0..count); one workgroup.
+__global__ __launch_bounds__(1024) void grouped_prefix_kernel(int count, const bzg_dgemm_problem* __restrict__ pr,
+                                                              int64_t* __restrict__ prefix,
+                                                              int32_t* __restrict__ status) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (count + 1023) / 1024;
+  const int b = t * per, e = min(count, b + per);
+  int64_t s = 0;
+  for (int p = b; p < e; ++p) s += grouped_tiles(pr[p], nullptr);
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int64_t acc = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const int64_t v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int64_t acc = part[t];
+  for (int p = b; p < e; ++p) {
+    prefix[p] = acc;
+    acc += grouped_tiles(pr[p], status ? status + p : nullptr);
+  }
+  if (t == 1023) prefix[count] = acc;  // the last range ends at count: acc is the total
+}
+
+// Persistent: workgroup w takes tiles w, w + grid, ... of the concatenated tile list (a fixed
+// assignment, no atomics); the problem of a tile is found by binary search in the prefix.
+// Problems with k == 0 are scaled (C = beta C) tile by tile.
+__global__ __launch_bounds__(kThreads, 2) void gemm_grouped_kernel(int count, const bzg_dgemm_problem* __restrict__ pr,
+                                                                   const int64_t* __restrict__ prefix) {
+  __shared__ GemmShm<double, kGT, kGT> sh;
+  const int64_t total = prefix[count];
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = count - 1;  // last p with prefix[p] <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const bzg_dgemm_problem q = pr[lo];
+    const int tiles_n = (int)((q.n + kGT - 1) / kGT);
+    const int local = (int)(t - prefix[lo]);
+    const int tm = local / tiles_n, tn = local % tiles_n;
+    const bool vec = ((uintptr_t)q.A % 16 == 0) && ((uintptr_t)q.B % 16 == 0) && (q.lda % 2 == 0) &&
+                     (q.ldb % 2 == 0);
+    if (q.k == 0) {
+      for (int e = threadIdx.x; e < kGT * kGT; e += kThreads) {
+        const long r = (long)tm * kGT + e / kGT, c = (long)tn * kGT + e % kGT;
+        if (r < q.m && c < q.n) {
+          double* p = q.C + r * q.ldc + c;
+          *p = (q.beta == 0.0) ? 0.0 : q.beta * *p;
+        }
+      }
+    } else if (vec) {
+      gemm_tile<double, kGT, kGT, 32, 32, true>((int)q.m, (int)q.n, (int)q.k, q.alpha, q.A, q.lda, q.B, q.ldb,
+                                                q.beta, q.C, q.ldc, tm, tn, sh);
+    } else {
+      gemm_tile<double, kGT, kGT, 32, 32, false>((int)q.m, (int)q.n, (int)q.k, q.alpha, q.A, q.lda, q.B, q.ldb,
+                                                 q.beta, q.C, q.ldc, tm, tn, sh);
+    }
+    __syncthreads();  // LDS of this tile is free before the next one is staged
+  }
+}
+
 // C = beta*C for k == 0 (and the empty product).
 template <typename T>
 __global__ void scale_matrix_kernel(int M, int N, T beta, T* C, long ldc) {
@@ -307,9 +411,51 @@ bzg_status launch_tile(bzg_context* ctx, int m, int n, int k, T alpha, const T* 
   return BZG_OK;
 }
 
-// Tile choice by output size only (never by load), so a product's result never depends on
-// what else is running.  Large tiles cut L2->LDS traffic; small tiles keep >= ~256
-// workgroups in flight for small products.
+// ---- tile-shape autotuner ------------------------------------------------------------
+// Tile shapes: 0 = 32 x 32, 1 = 64 x 64, 2 = 128 x 128.  bzg_gemm_autotune times the three
+// shapes on a product of the given size and records the fastest for that size class
+// (ceil(log2) of m, n, k and the element type); later products of the class use it.  All
+// shapes give bit-identical results, so tuning changes speed only.
+struct TuneKey {
+  int type, lm, ln, lk;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(type, lm, ln, lk) < std::tie(o.type, o.lm, o.ln, o.lk);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+
+int ceil_log2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) ++l;
+  return l;
+}
+TuneKey tune_key(int type, int64_t m, int64_t n, int64_t k) {
+  return TuneKey{type, ceil_log2(m), ceil_log2(n), ceil_log2(k)};
+}
+
+template <typename T>
+int heuristic_shape(int64_t m, int64_t n) {
+  const long t128 = ((m + 127) / 128) * ((n + 127) / 128);
+  const long t64 = ((m + 63) / 64) * ((n + 63) / 64);
+  if (t128 >= 240) return 2;
+  if (t64 >= 200) return 1;
+  return 0;
+}
+
+template <typename T>
+bzg_status launch_shape(bzg_context* ctx, int shape, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
+                        int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
+  if (shape == 2)
+    return launch_tile<T, 128, 128, 64, 64>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (shape == 1)
+    return launch_tile<T, 64, 64, 32, 32>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  return launch_tile<T, 32, 32, 16, 16>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+// Tile choice by size only (never by load), so a product's result never depends on what
+// else is running: the tuned shape of the size class if there is one, else a heuristic
+// (large tiles cut L2->LDS traffic; small tiles keep >= ~256 workgroups in flight).
 template <typename T>
 bzg_status gemm_dispatch(bzg_context* ctx, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
                          int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
@@ -323,13 +469,63 @@ bzg_status gemm_dispatch(bzg_context* ctx, int64_t m, int64_t n, int64_t k, T al
     BZG_LAUNCH_CHECK();
     return BZG_OK;
   }
-  const long t128 = ((m + 127) / 128) * ((n + 127) / 128);
-  const long t64 = ((m + 63) / 64) * ((n + 63) / 64);
-  if (t128 >= 240)
-    return launch_tile<T, 128, 128, 64, 64>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-  if (t64 >= 200)
-    return launch_tile<T, 64, 64, 32, 32>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-  return launch_tile<T, 32, 32, 16, 16>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  int shape = heuristic_shape<T>(m, n);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(tune_key(sizeof(T) == 8 ? 0 : 1, m, n, k));
+    if (it != g_tune.end()) shape = it->second;
+  }
+  return launch_shape<T>(ctx, shape, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+template <typename T>
+bzg_status gemm_autotune(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t* chosen, float* ms_out) {
+  BZG_REQUIRE(ctx && m > 0 && n > 0 && k > 0, BZG_ERR_INVALID_ARGUMENT, "gemm_autotune: bad argument");
+  BZG_REQUIRE(m < (1LL << 31) && n < (1LL << 31) && k < (1LL << 31), BZG_ERR_UNSUPPORTED,
+              "gemm_autotune: dimension >= 2^31");
+  const size_t bytes = sizeof(T) * ((size_t)m * k + (size_t)k * n + (size_t)m * n);
+  void* ws = nullptr;
+  bzg_status st = workspace(ctx, 7, bytes, &ws);
+  if (st) return st;
+  T* A = static_cast<T*>(ws);
+  T* B = A + (size_t)m * k;
+  T* C = B + (size_t)k * n;
+  // operand values do not change the timing of these kernels: zero-filled scratch
+  BZG_HIP_CHECK(hipMemsetAsync(ws, 0, bytes, ctx->stream));
+  hipEvent_t e0, e1;
+  BZG_HIP_CHECK(hipEventCreate(&e0));
+  BZG_HIP_CHECK(hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 0.f;
+  const bool prof = ctx->profile;
+  ctx->profile = false;
+  for (int shape = 0; shape < 3 && !st; ++shape) {
+    st = launch_shape<T>(ctx, shape, m, n, k, T(1), A, k, B, n, T(0), C, n);  // warm-up
+    if (st) break;
+    const int reps = 3;
+    hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps && !st; ++r) st = launch_shape<T>(ctx, shape, m, n, k, T(1), A, k, B, n, T(0), C, n);
+    hipEventRecord(e1, ctx->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) st = BZG_ERR_HIP;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    if (!st && (best < 0 || ms < best_ms)) {
+      best = shape;
+      best_ms = ms;
+    }
+  }
+  ctx->profile = prof;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st) return st;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[tune_key(sizeof(T) == 8 ? 0 : 1, m, n, k)] = best;
+  }
+  if (chosen) *chosen = best;
+  if (ms_out) *ms_out = best_ms;
+  return BZG_OK;
 }
 
 }  // namespace
@@ -341,6 +537,38 @@ bzg_status bzg_dgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, double a
                      const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                      double* C, int64_t ldc) {
   return bzg::gemm_dispatch<double>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+bzg_status bzg_dgemm_grouped(bzg_context* ctx, int64_t count, const bzg_dgemm_problem* problems,
+                             int32_t* status) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dgemm_grouped: null context");
+  BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "dgemm_grouped: bad count");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(problems != nullptr, BZG_ERR_INVALID_ARGUMENT, "dgemm_grouped: null problem array");
+  void* ws = nullptr;
+  bzg_status st = bzg::workspace(ctx, 6, (size_t)(count + 1) * sizeof(int64_t), &ws);
+  if (st) return st;
+  int64_t* prefix = static_cast<int64_t*>(ws);
+  bzg::ProfScope prof(ctx, "dgemm_grouped");
+  hipLaunchKernelGGL(bzg::grouped_prefix_kernel, dim3(1), dim3(1024), 0, ctx->stream, (int)count, problems, prefix,
+                     status);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bzg::gemm_grouped_kernel, dim3((unsigned)(ctx->num_cus * 4)), dim3(bzg::kThreads), 0,
+                     ctx->stream, (int)count, problems, prefix);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+bzg_status bzg_gemm_autotune(bzg_context* ctx, int32_t fp32, int64_t m, int64_t n, int64_t k,
+                             int32_t* shape, float* ms) {
+  return fp32 ? bzg::gemm_autotune<float>(ctx, m, n, k, shape, ms)
+              : bzg::gemm_autotune<double>(ctx, m, n, k, shape, ms);
+}
+
+bzg_status bzg_gemm_tuning_clear(void) {
+  std::lock_guard<std::mutex> lk(bzg::g_tune_mu);
+  bzg::g_tune.clear();
+  return BZG_OK;
 }
 
 bzg_status bzg_sgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, float alpha,

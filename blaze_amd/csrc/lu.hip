@@ -744,7 +744,7 @@ __global__ __launch_bounds__(256) void classify_kernel(int count, const int32_t*
 
 }  // namespace
 
-// Occupancy of the persistent grids (workgroups per CU); chosen by the autotuner.
+// Occupancy of the persistent grids (workgroups per CU); bzg_solver_autotune picks them.
 struct SolverTuning {
   int wg_per_cu_fp64 = 8;
   int wg_per_cu_mixed = 8;
@@ -875,6 +875,66 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
                      nrhs, a_off, b_off, A, B, X, info, static_cast<const int32_t*>(fb_list), counters + 1,
                      counters + 2, static_cast<double*>(w64));
   BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+bzg_status bzg_solver_autotune(bzg_context* ctx, int32_t mixed, int64_t count, const int32_t* n,
+                               const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
+                               const double* A, int64_t a_elems, const double* B, int64_t b_elems,
+                               const int32_t* order, int32_t* wg_per_cu, float* ms_out) {
+  BZG_REQUIRE(ctx && count > 0 && n && nrhs && a_off && b_off && A && B && a_elems > 0 && b_elems > 0,
+              BZG_ERR_INVALID_ARGUMENT, "solver_autotune: bad argument");
+  // scratch: copies of A and B (the FP64 solver works in place) or X, plus iter / info
+  const size_t bytes = (size_t)(mixed ? 0 : a_elems) * 8 + (size_t)b_elems * 8 + (size_t)count * 8 + 256;
+  void* ws = nullptr;
+  bzg_status st = bzg::workspace(ctx, 7, bytes, &ws);
+  if (st) return st;
+  double* Aw = static_cast<double*>(ws);
+  double* Bw = Aw + (mixed ? 0 : a_elems);
+  int32_t* it = reinterpret_cast<int32_t*>(Bw + b_elems);
+  int32_t* inf = it + count;
+  hipEvent_t e0, e1;
+  BZG_HIP_CHECK(hipEventCreate(&e0));
+  BZG_HIP_CHECK(hipEventCreate(&e1));
+  const bool prof = ctx->profile;
+  ctx->profile = false;
+  int& knob = mixed ? bzg::g_tuning.wg_per_cu_mixed : bzg::g_tuning.wg_per_cu_fp64;
+  const int saved = knob;
+  int best = -1;
+  float best_ms = 0.f;
+  auto run = [&]() -> bzg_status {
+    if (mixed)
+      return bzg_dsgesv_batched(ctx, count, n, nrhs, a_off, b_off, A, B, Bw, it, inf, 0.0, 30, order);
+    if (hipMemcpyAsync(Aw, A, (size_t)a_elems * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(Bw, B, (size_t)b_elems * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+      return BZG_ERR_HIP;
+    return bzg_dgesv_batched(ctx, count, n, nrhs, a_off, b_off, nullptr, Aw, Bw, nullptr, inf, order);
+  };
+  for (int cand : {1, 2, 3, 4, 6, 8}) {
+    knob = cand;
+    st = run();  // warm-up (and workspace growth)
+    if (st) break;
+    hipEventRecord(e0, ctx->stream);
+    st = run();
+    hipEventRecord(e1, ctx->stream);
+    if (st || hipEventSynchronize(e1) != hipSuccess) {
+      if (!st) st = BZG_ERR_HIP;
+      break;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (best < 0 || ms < best_ms) {
+      best = cand;
+      best_ms = ms;
+    }
+  }
+  ctx->profile = prof;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  knob = st ? saved : best;
+  if (st) return st;
+  if (wg_per_cu) *wg_per_cu = best;
+  if (ms_out) *ms_out = best_ms;
   return BZG_OK;
 }
 

@@ -566,4 +566,52 @@ bzg_status bzg_icsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t*
   return bzg::csrmv<int32_t>(ctx, plan, rowptr, colidx, val, x, y);
 }
 
+bzg_status bzg_csr_plan_autotune(bzg_context* ctx, bzg_csr_plan* plan, int32_t fp32, const int32_t* rowptr,
+                                 const int32_t* colidx, const void* val, const void* x, void* y, float* ms_out) {
+  BZG_REQUIRE(ctx && plan && rowptr && x && y, BZG_ERR_INVALID_ARGUMENT, "csr_plan_autotune: null argument");
+  // candidates: row blocks (2048 / 4096 nonzeros, cached / non-temporal loads) and lane groups
+  struct Cand { int lanes, rpg, cap; };
+  std::vector<Cand> cands = {{0, 1, 0}, {0, 2, 0}, {0, 1, 1}, {0, 2, 1}};
+  for (int lanes : {2, 4, 8, 16})
+    for (int rpg : {1, 2})
+      for (int cap : {4, 8}) cands.push_back({lanes, rpg, cap});
+  hipEvent_t e0, e1;
+  BZG_HIP_CHECK(hipEventCreate(&e0));
+  BZG_HIP_CHECK(hipEventCreate(&e1));
+  const bool prof = ctx->profile;
+  ctx->profile = false;
+  auto run = [&]() -> bzg_status {
+    return fp32 ? bzg::csrmv<float>(ctx, plan, rowptr, colidx, static_cast<const float*>(val),
+                                    static_cast<const float*>(x), static_cast<float*>(y))
+                : bzg::csrmv<double>(ctx, plan, rowptr, colidx, static_cast<const double*>(val),
+                                     static_cast<const double*>(x), static_cast<double*>(y));
+  };
+  int best = -1;
+  float best_ms = 0.f;
+  bzg_status st = BZG_OK;
+  for (int i = 0; i < (int)cands.size() && !st; ++i) {
+    st = bzg_csr_plan_tune(plan, cands[i].lanes, cands[i].rpg, cands[i].cap);
+    if (st) break;
+    st = run();  // warm-up
+    const int reps = 5;
+    hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps && !st; ++r) st = run();
+    hipEventRecord(e1, ctx->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) st = BZG_ERR_HIP;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    if (!st && (best < 0 || ms < best_ms)) {
+      best = i;
+      best_ms = ms;
+    }
+  }
+  ctx->profile = prof;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st) return st;
+  if (ms_out) *ms_out = best_ms;
+  return bzg_csr_plan_tune(plan, cands[best].lanes, cands[best].rpg, cands[best].cap);
+}
+
 }  // extern "C"

@@ -98,6 +98,38 @@ bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t 
                      const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
                      int32_t beta, int32_t* C, int64_t ldc);
 
+/* ---- grouped GEMM: many independent products of uneven sizes in one launch ------------
+ * No single reference function: a batch of DMatDMatMultExpr assignments
+ * (DMatDMatMultExpr.h:516-569), each C_p = alpha_p A_p B_p + beta_p C_p with bzg_dgemm's
+ * rules.  `problems` is a DEVICE array of `count` descriptors (so that the call can be captured
+ * in a HIP graph and replayed with new descriptor contents); `status` (device, may be NULL)
+ * receives 0 or -1 (descriptor rejected, product skipped) per problem.  One persistent launch
+ * walks the 64 x 64 tiles of all products; each C element is summed in the same k order as
+ * bzg_dgemm, so every product is bit-identical to its own bzg_dgemm call. */
+typedef struct {
+  int64_t m, n, k;
+  const double* A;
+  int64_t lda;
+  const double* B;
+  int64_t ldb;
+  double* C;
+  int64_t ldc;
+  double alpha, beta;
+} bzg_dgemm_problem;
+bzg_status bzg_dgemm_grouped(bzg_context* ctx, int64_t count, const bzg_dgemm_problem* problems,
+                             int32_t* status);
+
+/* Tile-shape autotuner (no reference counterpart; the reference's blocking is fixed at
+ * compile time, blaze/math/dense/MMM.h + Blocking.h:58-61).  Times the three MFMA tile shapes
+ * (32 x 32, 64 x 64, 128 x 128) on an m x n x k product (fp32 != 0: float, else double) and
+ * records the fastest for the size class (ceil(log2) of m, n, k); later bzg_dgemm / bzg_sgemm
+ * calls of that class use it.  All shapes give bit-identical results.  Writes the chosen
+ * shape (0, 1, 2) and its time in ms (either may be NULL).  Synchronises the stream. */
+bzg_status bzg_gemm_autotune(bzg_context* ctx, int32_t fp32, int64_t m, int64_t n, int64_t k,
+                             int32_t* shape, float* ms);
+/* Forgets every tuned size class (back to the built-in heuristic). */
+bzg_status bzg_gemm_tuning_clear(void);
+
 /* ---- dense GEMV:  y = alpha * A * x + beta * y  (row-major A, device pointers) ---------- */
 bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,
                      int64_t lda, const double* x, double beta, double* y);
@@ -126,6 +158,14 @@ bzg_status bzg_csr_plan_destroy(bzg_csr_plan* plan);
  * autotuner.  Results stay bit-identical (every row is still summed in storage order). */
 bzg_status bzg_csr_plan_tune(bzg_csr_plan* plan, int32_t lanes, int32_t rows_per_group,
                              int32_t cap);
+/* Autotuner: times the short-row kernel configurations (row blocks of 2048 / 4096 nonzeros
+ * with cached or non-temporal loads; lane groups of 2..16 lanes, 1-2 rows, 4-8 loads in
+ * flight) on this matrix (fp32 != 0: float values / x / y, else double) and keeps the fastest
+ * in the plan; y is overwritten.  Results are identical for every configuration.  Writes the
+ * best time in ms (may be NULL).  Synchronises the stream. */
+bzg_status bzg_csr_plan_autotune(bzg_context* ctx, bzg_csr_plan* plan, int32_t fp32,
+                                 const int32_t* rowptr, const int32_t* colidx, const void* val,
+                                 const void* x, void* y, float* ms);
 /* Describes the plan: writes a short text (kernel choice, lanes per row, blocks) to buf. */
 bzg_status bzg_csr_plan_describe(const bzg_csr_plan* plan, char* buf, size_t len);
 bzg_status bzg_dcsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,
@@ -166,6 +206,16 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
                               const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
                               const double* A, const double* B, double* X, int32_t* iter,
                               int32_t* info, double tol, int32_t max_iter, const int32_t* order);
+/* Autotuner of the batched solvers' persistent grids (no reference counterpart): times
+ * 1, 2, 3, 4, 6 and 8 workgroups per CU on the given batch (mixed != 0: bzg_dsgesv_batched,
+ * else bzg_dgesv_batched) and keeps the fastest for later calls.  A and B are not modified
+ * (the FP64 solver runs on copies; a_elems / b_elems are the pool sizes in doubles).  Results
+ * do not depend on the grid.  Writes the choice and its time in ms (either may be NULL).
+ * Synchronises the stream. */
+bzg_status bzg_solver_autotune(bzg_context* ctx, int32_t mixed, int64_t count, const int32_t* n,
+                               const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
+                               const double* A, int64_t a_elems, const double* B, int64_t b_elems,
+                               const int32_t* order, int32_t* wg_per_cu, float* ms);
 /* Longest-first work order for the batched solvers: writes to order_dev (device, count
  * entries) the system indices sorted by descending n (stable), computed from n_host. */
 bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_host,
