@@ -124,6 +124,7 @@ struct Group {
   size_t hcls = 0;
   size_t budget_bytes = 0;    // bytes charged to the budget for the pool (released with it)
   int live = 0;               // requests of the batch not yet released
+  bool gemm_batch = false;    // a grouped GEMM launch (descriptor pool freed at completion)
   double est_us = 0;          // estimated device time (stream choice)
   int32_t* h_iter = nullptr;  // pinned copies of iter / info, valid once the event completed
   int32_t* h_info = nullptr;
@@ -236,6 +237,34 @@ bzg_status launch_single(bzg_server* s, Request* r, Group* g) {
     st = bzg_dcsrmv(c, M.plan, M.rowptr, M.colidx, M.val, r->A, r->out);
   }
   return st;
+}
+
+// GEMMs up to this many flops are fused with the other waiting ones into one grouped launch
+// (64 x 64 tiles); larger ones run alone with the tile shape chosen for their size.  Either
+// way every element is summed in the same order, so the result does not depend on the choice.
+constexpr double kGroupFlops = 2.0 * 1024 * 1024 * 1024;
+bool groupable(const Request* r) { return r->kind == kGemm && 2.0 * r->m * r->n * r->k <= kGroupFlops; }
+
+// Launches several GEMM requests as one bzg_dgemm_grouped call (descriptors staged through
+// pinned memory into the group's device pool).
+bzg_status launch_gemms(bzg_server* s, Group* g) {
+  bzg_context* c = s->ctx[g->stream];
+  const int cnt = (int)g->reqs.size();
+  const size_t bytes = (size_t)cnt * sizeof(bzg_dgemm_problem);
+  bzg_status st = s->dev.get(bytes, &g->dpool, &g->dcls);
+  if (!st) st = s->host.get(bytes, &g->hpool, &g->hcls);
+  if (st) return st;
+  auto* h = static_cast<bzg_dgemm_problem*>(g->hpool);
+  for (int i = 0; i < cnt; ++i) {
+    Request* r = g->reqs[i];
+    void* p = nullptr;
+    st = s->dev.get((size_t)r->out_elems * sizeof(double), &p, &r->out_cls);
+    if (st) return st;
+    r->out = static_cast<double*>(p);
+    h[i] = bzg_dgemm_problem{r->m, r->n, r->k, r->A, r->lda, r->B, r->ldb, r->out, r->n, 1.0, 0.0};
+  }
+  BZG_HIP_CHECK(hipMemcpyAsync(g->dpool, g->hpool, bytes, hipMemcpyHostToDevice, c->stream));
+  return bzg_dgemm_grouped(c, cnt, static_cast<const bzg_dgemm_problem*>(g->dpool), nullptr);
 }
 
 // Launches a batch of solve requests as one bzg_dsgesv_batched call.
@@ -354,7 +383,9 @@ void dispatcher(bzg_server* s) {
         s->load[g->stream] -= g->est_us;
         s->events.push_back(g->ev);
         if (g->reqs[0]->kind != kSolve) {
-          g->reqs[0]->group = nullptr;
+          for (auto* r : g->reqs) r->group = nullptr;
+          s->dev.put(g->dpool, g->dcls);
+          s->host.put(g->hpool, g->hcls);
           delete g;
         }
       }
@@ -383,7 +414,24 @@ void dispatcher(bzg_server* s) {
       }
       const double t = now_us();
       bzg_status st = BZG_OK;
-      if (head->kind != kSolve) {
+      if (groupable(head)) {
+        // the head's GEMM batch: every admissible waiting small GEMM, in arrival order
+        for (auto it = s->pending.begin(); it != s->pending.end();) {
+          Request* r = *it;
+          if (!groupable(r)) { ++it; continue; }
+          const size_t b = request_bytes(s, r);
+          if ((int)g->reqs.size() >= s->cfg.solve_batch_max || s->used + b > s->cfg.hbm_budget_bytes) break;
+          s->used += b;
+          g->reqs.push_back(r);
+          r->group = g;
+          r->t_admit = t;
+          it = s->pending.erase(it);
+        }
+        g->gemm_batch = true;
+        st = launch_gemms(s, g);
+        s->st.gemm_batches++;
+        s->st.max_gemm_batch = std::max<int64_t>(s->st.max_gemm_batch, (int64_t)g->reqs.size());
+      } else if (head->kind != kSolve) {
         s->pending.pop_front();
         g->reqs.push_back(head);
         head->group = g;
@@ -431,7 +479,9 @@ void dispatcher(bzg_server* s) {
         if (g->reqs[0]->kind == kSolve) {
           g->live = (int)g->reqs.size();
         } else {
-          g->reqs[0]->group = nullptr;
+          for (auto* r : g->reqs) r->group = nullptr;
+          s->dev.put(g->dpool, g->dcls);
+          s->host.put(g->hpool, g->hcls);
           delete g;
         }
         lk.unlock();

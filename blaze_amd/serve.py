@@ -22,8 +22,8 @@ class RequestResult(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("submitted", "admitted", "completed", "released", "launches",
-                                         "solve_batches", "max_solve_batch", "inflight", "bytes_in_use",
-                                         "bytes_peak", "budget")]
+                                         "solve_batches", "max_solve_batch", "gemm_batches", "max_gemm_batch",
+                                         "inflight", "bytes_in_use", "bytes_peak", "budget")]
 
 
 KINDS = ("gemm", "spmv", "solve")

@@ -250,7 +250,8 @@ int32_t bzg_fp64acc_operand_bits(int64_t k, int32_t moduli, int32_t int8_slices)
  * launches are running and its result (plus staging) fits in hbm_budget_bytes together with
  * everything not yet released and the resident matrices; admission is FIFO, and all admissible
  * solve requests waiting behind the head join its batch (one bzg_dsgesv_batched launch, FP32 LU +
- * FP64 refinement to LAPACK dsgesv's test, FP64 fallback).  Work is spread over `streams` HIP
+ * FP64 refinement to LAPACK dsgesv's test, FP64 fallback); likewise GEMMs of up to 2^31 flops
+ * (one bzg_dgemm_grouped launch).  Work is spread over `streams` HIP
  * streams so that requests overlap on the device.  Each result is bit-identical to the same
  * request run alone.  All functions are thread-safe. */
 typedef struct bzg_server bzg_server;
@@ -258,7 +259,7 @@ typedef struct {
   size_t hbm_budget_bytes; /* admission budget: unreleased results + staging + resident CSR */
   int32_t max_inflight;    /* launches on the device at once (>= 1) */
   int32_t streams;         /* HIP streams / library contexts used by the server (1..16) */
-  int32_t solve_batch_max; /* largest number of solve requests fused into one launch */
+  int32_t solve_batch_max; /* largest number of solve (or GEMM) requests fused into one launch */
 } bzg_server_config;
 typedef struct {
   bzg_status status;      /* launch status (0 ok) */
@@ -271,7 +272,7 @@ typedef struct {
 } bzg_request_result;
 typedef struct {
   int64_t submitted, admitted, completed, released;
-  int64_t launches, solve_batches, max_solve_batch, inflight;
+  int64_t launches, solve_batches, max_solve_batch, gemm_batches, max_gemm_batch, inflight;
   int64_t bytes_in_use, bytes_peak, budget;
 } bzg_server_stats;
 
