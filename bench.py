@@ -119,9 +119,42 @@ class Bench:
             t = self.timed(lambda: self.ctx.call(fn, m, n, k, vp(dA), k, vp(dB), n, vp(dC), n, 0),
                            steps, warmup)
             Ch = dC.cpu().numpy()
+            nm, bits, est = C.c_int32(), C.c_int32(), C.c_double()
+            self.ctx.call("bzg_fp64acc_last_plan", C.byref(nm), C.byref(bits), C.byref(est))
             out[name] = {"tflops": fl / t / 1e12, "ms": t * 1e3,
                          "rel_frob_err": self.rel_frob(Ch[rows, cols], ref),
-                         "vs_native": (fl / t / 1e12) / out["native_fp64"]["tflops"]}
+                         "vs_native": (fl / t / 1e12) / out["native_fp64"]["tflops"],
+                         "moduli": nm.value, "operand_bits": bits.value, "estimated_err": est.value}
+        return out
+
+    @staticmethod
+    def ref_full(A, B):
+        """Extended-precision (x87 long double, 64-bit significand) A @ B, rows split over threads."""
+        from concurrent.futures import ThreadPoolExecutor
+        Bl = B.astype(np.longdouble)
+        parts = np.array_split(np.arange(A.shape[0]), 16)
+        with ThreadPoolExecutor(16) as ex:
+            outs = list(ex.map(lambda r: A[r].astype(np.longdouble) @ Bl, parts))
+        return np.concatenate(outs)
+
+    def sigma_case(self, A, B, variants):
+        """Full relative Frobenius error of native FP64 and each split variant (automatic count)."""
+        torch = self.torch
+        m, k = A.shape
+        n = B.shape[1]
+        ref = self.ref_full(A, B)
+        den = np.sqrt(np.sum(ref ** 2))
+        dA, dB = self.dev(A), self.dev(B)
+        dC = torch.empty((m, n), dtype=torch.float64, device="cuda")
+        self.ctx.call("bzg_dgemm", m, n, k, 1.0, vp(dA), k, vp(dB), n, 0.0, vp(dC), n)
+        out = {"native_fp64": float(np.sqrt(np.sum((dC.cpu().numpy() - ref) ** 2)) / den)}
+        for name, fn in variants:
+            self.ctx.call(fn, m, n, k, vp(dA), k, vp(dB), n, vp(dC), n, 0)
+            got = dC.cpu().numpy()
+            nm, bits, est = C.c_int32(), C.c_int32(), C.c_double()
+            self.ctx.call("bzg_fp64acc_last_plan", C.byref(nm), C.byref(bits), C.byref(est))
+            out[name] = {"rel_frob_err": float(np.sqrt(np.sum((got - ref) ** 2)) / den), "moduli": nm.value,
+                         "operand_bits": bits.value, "estimated_err": est.value}
         return out
 
     def roofline_fp64acc(self, m, n, k, A, B, fn, slices, reps=5):
@@ -355,9 +388,7 @@ def gemm_leg(b, rec, args):
     else:
         head = "fp64acc_" + args.variant
     fn = dict(variants)[head]
-    slices = bz.lib().bzg_fp64acc_slices(k, 0) if head.endswith("bf16") else None
-    if slices is None:
-        slices = _i8_slices(k)
+    slices = res[head]["moduli"]  # chosen by the library from the operands (automatic count)
     rec["value"] = res[head]["tflops"]
     rec["ms_per_step"] = res[head]["ms"]
     rec["dtype"] = "int8" if head.endswith("int8") else "bf16"
@@ -373,13 +404,14 @@ def gemm_leg(b, rec, args):
     rec["fp64acc_8192"] = {v: res[v] for v, _ in variants}
     rec["roofline"] = b.roofline_fp64acc(m, n, k, A, B, fn, slices)
     # error per exponent spread (smaller size keeps the run short; K is what matters)
+    # error per exponent spread: the full relative Frobenius error of a 512 x 512 x k product
+    # (for wide spreads a few entries carry most of ||C||: a sampled ratio would mislead)
     errs = {}
+    ms_ = 512
     for sigma in (0, 4, 10):
-        ms_ = 2048
         Ag = b.graded((ms_, k), 30 + sigma, sigma); Bg = b.graded((k, ms_), 40 + sigma, sigma)
-        r_ = b.gemm_case(ms_, ms_, k, Ag, Bg, 1, 1, variants)
-        errs[f"sigma{sigma}"] = {v: r_[v]["rel_frob_err"] for v in ("native_fp64",) + tuple(x for x, _ in variants)}
-    rec["rel_frob_err_by_sigma_k8192"] = errs
+        errs[f"sigma{sigma}"] = b.sigma_case(Ag, Bg, variants)
+    rec[f"rel_frob_err_by_sigma_{ms_}x{ms_}x{k}"] = errs
     if m == 8192:
         mr, nr_, kr = 16384, 1024, 16384
         Ar = b.uniform((mr, kr), 3); Br = b.uniform((kr, nr_), 4)
@@ -444,15 +476,6 @@ def main():
         b.sync()
         rec["copy_bytes"] = 1 << 30
     print(json.dumps(rec), flush=True)
-
-
-def _i8_slices(k):
-    # same selection rule as the library (P >= 50 bits per operand) for moduli <= 255
-    import blaze_amd as bz
-    for nm in range(4, 25):
-        if bz.lib().bzg_fp64acc_operand_bits(k, nm, 1) >= 50:
-            return nm
-    return None
 
 
 if __name__ == "__main__":

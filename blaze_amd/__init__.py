@@ -58,6 +58,7 @@ SIGNATURES = {
     "bzg_solver_autotune": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, i64, vp, i64, vp, C.POINTER(i32), C.POINTER(f32)]),
     "bzg_dgemm_fp64acc": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]),
     "bzg_dgemm_fp64acc_i8": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]),
+    "bzg_fp64acc_last_plan": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(f64)]),
     "bzg_fp64acc_slices": (i32, [i64, i32]),
     "bzg_fp64acc_operand_bits": (i32, [i64, i32, i32]),
     "bzg_server_create": (i32, [vp, C.POINTER(vp)]),

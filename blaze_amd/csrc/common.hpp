@@ -47,8 +47,11 @@ struct bzg_context {
   hipStream_t stream = nullptr;
   int num_cus = 256;
   // Cached scratch for kernels that need it (split GEMM slices, solver staging).
-  bzg::Workspace ws[8];
+  bzg::Workspace ws[12];
   // Optional HIP-event timing of named kernels (bzg_profile_enable / bzg_profile_read).
+  // plan of the last FP64-accurate GEMM on this context (bzg_fp64acc_last_plan)
+  int fp64acc_moduli = 0, fp64acc_bits = 0;
+  double fp64acc_est = 0.0;
   bool profile = false;
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
 };

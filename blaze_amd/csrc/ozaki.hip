@@ -15,7 +15,7 @@
 //      FP32 (|r r'| <= 65025) and every 256-long K chunk sums to at most 256*65025 + 767 < 2^24,
 //      so the FP32 accumulator is an exact integer; it is reduced mod m_l after each chunk;
 //   4. CRT: C' = sum_l C'_l W_l mod M, W_l = (M/m_l) * ((M/m_l)^-1 mod m_l), M = prod m_l.  W_l
-//      and M are split into 36-bit chunks so that the three partial sums are exact in FP64;
+//      and M are split completely into 40-bit (39 for BF16) chunks so every partial sum is exact in FP64;
 //      |C'| <= k 2^(2P) <= M/4 makes the symmetric reduction unique;
 //   5. C = C' * 2^-(s_i + t_j), rounded once to FP64.
 // The INT8 variant uses moduli <= 255 (|r| <= 127) on v_mfma_i32_32x32x32_i8 with exact INT32
@@ -36,13 +36,16 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMaxModuli = 24;
 
+constexpr int kMaxChunks = 6;  // chunks of W_l and M: 24 moduli x 9 bits <= 216 bits, >= 39 bits each
+
 struct CrtConsts {
   double m[kMaxModuli];
   double minv[kMaxModuli];
   double c30[kMaxModuli];  // 2^30 mod m
-  double w1[kMaxModuli], w2[kMaxModuli], w3[kMaxModuli];
-  double M1, M2, M3, Minv;
-  int n;
+  double w[kMaxChunks][kMaxModuli];  // W_l = sum_c w[c][l] exactly (chunk c: cb bits at E-cb(c+1))
+  double M[kMaxChunks];              // M = sum_c M[c] exactly
+  double Minv;
+  int n, nch;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -81,10 +84,10 @@ struct Big {
     }
     return b;
   }
-  // floor(this / 2^sh) mod 2^36 as a double
-  double chunk36(int sh) const {
+  // floor(this / 2^sh) mod 2^width as a double (width <= 52)
+  double chunk(int sh, int width) const {
     uint64_t v = 0;
-    for (int b = 35; b >= 0; --b) {
+    for (int b = width - 1; b >= 0; --b) {
       const int pos = sh + b;
       uint64_t bit = 0;
       if (pos >= 0) {
@@ -138,12 +141,16 @@ static int operand_bits(const std::vector<uint32_t>& ms, int64_t k) {
   return (int)std::floor((lm - 2.0 - lk) / 2.0);
 }
 
-static void build_crt(const std::vector<uint32_t>& ms, CrtConsts& c) {
+// Chunk width cb: a partial sum S_c = sum_l r_l w[c][l] with |r_l| <= rmax and up to 24 terms,
+// and S_c - q M[c] with |q| <= 24 rmax, must stay below 2^53 units of the chunk: cb = 40 for
+// INT8 residues (|r| <= 127), 39 for BF16 ones (|r| <= 255).
+static void build_crt(const std::vector<uint32_t>& ms, CrtConsts& c, int cb) {
   const int n = (int)ms.size();
   Big M(1);
   for (auto m : ms) M.mul_small(m);
   const int E = M.bits();
   c.n = n;
+  c.nch = (E + cb - 1) / cb;  // every bit of W_l < M: the partial sums below are exact
   for (int l = 0; l < n; ++l) {
     Big Ml(1);
     for (int q = 0; q < n; ++q)
@@ -155,51 +162,130 @@ static void build_crt(const std::vector<uint32_t>& ms, CrtConsts& c) {
     c.m[l] = (double)ms[l];
     c.minv[l] = 1.0 / (double)ms[l];
     c.c30[l] = (double)((1ull << 30) % ms[l]);
-    c.w1[l] = std::ldexp(W.chunk36(E - 36), E - 36);
-    c.w2[l] = std::ldexp(W.chunk36(E - 72), E - 72);
-    c.w3[l] = std::ldexp(W.chunk36(E - 108), E - 108);
+    for (int ch = 0; ch < c.nch; ++ch) c.w[ch][l] = std::ldexp(W.chunk(E - cb * (ch + 1), cb), E - cb * (ch + 1));
   }
-  c.M1 = std::ldexp(M.chunk36(E - 36), E - 36);
-  c.M2 = std::ldexp(M.chunk36(E - 72), E - 72);
-  c.M3 = std::ldexp(M.chunk36(E - 108), E - 108);
-  c.Minv = 1.0 / (c.M1 + c.M2 + c.M3);
+  for (int ch = 0; ch < c.nch; ++ch) c.M[ch] = std::ldexp(M.chunk(E - cb * (ch + 1), cb), E - cb * (ch + 1));
+  c.Minv = 1.0 / (c.M[0] + c.M[1]);
 }
 
 // ------------------------------------------------------------------------------------------
 // device: scaling exponents
 // ------------------------------------------------------------------------------------------
-// Row exponents of A (m x k, lda): s_i = P - 1 - ilogb(max_j |a_ij|)  (0 for zero rows).
-__global__ __launch_bounds__(256) void row_scale_kernel(int m, int k, const double* __restrict__ A,
-                                                        long lda, int P, int* __restrict__ s) {
+// Row statistics of A (m x k, lda), one wavefront per row: rmax_i = max_j |a_ij| and
+// rn2_i = sum_j a_ij^2 (fixed summation order: lane-strided partials, then a butterfly).
+__global__ __launch_bounds__(256) void row_stats_kernel(int m, int k, const double* __restrict__ A,
+                                                        long lda, double* __restrict__ rmax,
+                                                        double* __restrict__ rn2) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x % 64;
   if (wave >= m) return;
   const double* row = A + (long)wave * lda;
-  double mx = 0.0;
-  for (int j = lane; j < k; j += 64) mx = fmax(mx, fabs(row[j]));
+  double mx = 0.0, sq = 0.0;
+  for (int j = lane; j < k; j += 64) {
+    const double v = row[j];
+    mx = fmax(mx, fabs(v));
+    sq = fma(v, v, sq);
+  }
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if (lane == 0) s[wave] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
+  for (int off = 32; off >= 1; off >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, off, 64));
+    sq += __shfl_xor(sq, off, 64);
+  }
+  if (lane == 0) {
+    rmax[wave] = mx;
+    rn2[wave] = sq;
+  }
+}
+
+// s_i = P - 1 - ilogb(rmax_i)  (0 for zero rows): |rint(a_ij 2^s_i)| <= 2^P.
+__global__ __launch_bounds__(256) void row_scale_kernel(int m, const double* __restrict__ rmax, int P,
+                                                        int* __restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double mx = rmax[i];
+  s[i] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
 }
 
 // Column maxima of |B| (k x n, ldb) as FP64 bit patterns (non-negative doubles order like
 // their bit patterns): a (n / 64) x (k / 256) grid of 64-column x 256-row blocks (y strides over K), combined with
 // one atomicMax per column and block.  cmax must be zero on entry.
+// Also writes the block's partial sum of squares of each column to cn2[blockIdx.y * n + c]
+// (fixed order within the block), for the accuracy estimate of the automatic moduli count.
 __global__ __launch_bounds__(256) void col_max_kernel(int k, int n, const double* __restrict__ B,
-                                                      long ldb, unsigned long long* __restrict__ cmax) {
-  __shared__ double part[4][64];
+                                                      long ldb, unsigned long long* __restrict__ cmax,
+                                                      double* __restrict__ cn2) {
+  __shared__ double part[4][64], psq[4][64];
   const int c = blockIdx.x * 64 + threadIdx.x % 64, ph = threadIdx.x / 64;
-  double mx = 0.0;
+  double mx = 0.0, sq = 0.0;
   if (c < n)
     for (int r0 = blockIdx.y * 256; r0 < k; r0 += gridDim.y * 256) {
       const int r1 = min(k, r0 + 256);
-      for (int i = r0 + ph; i < r1; i += 4) mx = fmax(mx, fabs(B[(long)i * ldb + c]));
+      for (int i = r0 + ph; i < r1; i += 4) {
+        const double v = B[(long)i * ldb + c];
+        mx = fmax(mx, fabs(v));
+        sq = fma(v, v, sq);
+      }
     }
   part[ph][threadIdx.x % 64] = mx;
+  psq[ph][threadIdx.x % 64] = sq;
   __syncthreads();
   if (ph == 0 && c < n) {
     mx = fmax(fmax(part[0][threadIdx.x], part[1][threadIdx.x]), fmax(part[2][threadIdx.x], part[3][threadIdx.x]));
-    atomicMax(&cmax[c], (unsigned long long)__double_as_longlong(mx));
+    if (cmax) atomicMax(&cmax[c], (unsigned long long)__double_as_longlong(mx));
+    cn2[(long)blockIdx.y * n + c] = (psq[0][threadIdx.x] + psq[1][threadIdx.x]) + (psq[2][threadIdx.x] + psq[3][threadIdx.x]);
   }
+}
+
+// out[j] = sum_y part[y * n + j] for j < n (fixed order): column sums of squares from the
+// per-block partials of col_max_kernel.
+__global__ __launch_bounds__(256) void sum_partials_kernel(int ny, int n, const double* __restrict__ part,
+                                                           double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double a = 0.0;
+  for (int y = 0; y < ny; ++y) a += part[(long)y * n + j];
+  out[j] = a;
+}
+
+// Accuracy statistics (one workgroup, fixed summation order): out[0] = sum_i D_i^2,
+// out[1] = ||A||_F^2, out[2] = sum_j E_j^2, out[3] = ||B||_F^2, out[4] = sum_q ||A_:q||^2 ||B_q:||^2,
+// where D_i = 2^(ilogb(rmax_i)+1) and E_j likewise for the column maxima of B; bcn2 holds the
+// column sums of squares of B, acn2 those of A (k columns), brn2 the row sums of squares of B.
+__global__ __launch_bounds__(1024) void fp64acc_stats_kernel(int m, int n, int k,
+                                                            const double* __restrict__ rmax,
+                                                            const double* __restrict__ rn2,
+                                                            const unsigned long long* __restrict__ cmax,
+                                                            const double* __restrict__ bcn2,
+                                                            const double* __restrict__ acn2,
+                                                            const double* __restrict__ brn2, double* __restrict__ out) {
+  __shared__ double red[5][1024];
+  const int t = threadIdx.x;
+  double sa = 0, fa = 0, sb = 0, fb = 0, cc = 0;
+  for (int q = t; q < k; q += 1024) cc = fma(acn2[q], brn2[q], cc);
+  for (int i = t; i < m; i += 1024) {
+    const double mx = rmax[i];
+    if (mx > 0.0) {
+      const double d = ldexp(1.0, ilogb(mx) + 1);
+      sa = fma(d, d, sa);
+    }
+    fa += rn2[i];
+  }
+  for (int j = t; j < n; j += 1024) {
+    const double mx = __longlong_as_double((long long)cmax[j]);
+    if (mx > 0.0) {
+      const double e = ldexp(1.0, ilogb(mx) + 1);
+      sb = fma(e, e, sb);
+    }
+    fb += bcn2[j];
+  }
+  red[0][t] = sa; red[1][t] = fa; red[2][t] = sb; red[3][t] = fb; red[4][t] = cc;
+  __syncthreads();
+  for (int w = 512; w >= 1; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) red[q][t] += red[q][t + w];
+    __syncthreads();
+  }
+  if (t < 5) out[t] = red[t][0];
 }
 
 __global__ __launch_bounds__(256) void zero_u64_kernel(int n, unsigned long long* __restrict__ p) {
@@ -566,36 +652,50 @@ __global__ __launch_bounds__(256) void crt_kernel(int m, int n, int mp, int np, 
   const int j0 = (blockIdx.x * 256 + threadIdx.x) * 4;
   if (j0 >= n) return;
   const long plane = (long)mp * np;
+  const int nch = cc->nch;
   int tj[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) tj[u] = (j0 + u < n) ? t[j0 + u] : 0;
   for (int i = blockIdx.y; i < m; i += gridDim.y) {
     const R* src = Cres + (long)i * np + j0;
-    double S1[4] = {0, 0, 0, 0}, S2[4] = {0, 0, 0, 0}, S3[4] = {0, 0, 0, 0};
+    // S[c] = sum_l r_l w[c][l]: every term a multiple of chunk c's unit, below 2^53 units: exact
+    double S[kMaxChunks][4];
+#pragma unroll
+    for (int ch = 0; ch < kMaxChunks; ++ch)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) S[ch][u] = 0.0;
     for (int l = 0; l < nmod; ++l) {
       const auto q4 = *reinterpret_cast<const typename Quad<R>::V*>(src + l * plane);
       const double r[4] = {(double)q4.x, (double)q4.y, (double)q4.z, (double)q4.w};
-      const double w1 = cc->w1[l], w2 = cc->w2[l], w3 = cc->w3[l];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        S1[u] = fma(r[u], w1, S1[u]);  // exact: multiples of a common ulp, below 2^53 ulps
-        S2[u] = fma(r[u], w2, S2[u]);
-        S3[u] = fma(r[u], w3, S3[u]);
+      for (int ch = 0; ch < kMaxChunks; ++ch) {
+        if (ch < nch) {
+          const double wl = cc->w[ch][l];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) S[ch][u] = fma(r[u], wl, S[ch][u]);
+        }
       }
     }
     const int si = s[i];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (j0 + u >= n) break;
-      const double q = rint(S1[u] * cc->Minv);
-      const double T1 = fma(-q, cc->M1, S1[u]);  // exact
-      const double T2 = fma(-q, cc->M2, S2[u]);  // exact
-      const double T3 = fma(-q, cc->M3, S3[u]);
-      // T1 + T2 needs up to ~70 bits: add it error-free (TwoSum) so that C' is rounded only once.
-      const double sum = T1 + T2;
-      const double bb = sum - T1;
-      const double err = (T1 - (sum - bb)) + (T2 - bb);
-      const double Cp = sum + (err + T3);
+      // X = sum_c S[c] = C' + q M with |C'| <= M/4: q from the leading chunk, then the chunks of
+      // X - q M (each exact) summed with error-free transformations, rounded once
+      const double q = rint(S[0][u] * cc->Minv);
+      double hi = fma(-q, cc->M[0], S[0][u]);  // exact
+      double lo = 0.0;
+#pragma unroll
+      for (int ch = 1; ch < kMaxChunks; ++ch) {
+        if (ch < nch) {
+          const double t = fma(-q, cc->M[ch], S[ch][u]);  // exact
+          const double sum = hi + t;                      // TwoSum(hi, t)
+          const double bb = sum - hi;
+          lo += (hi - (sum - bb)) + (t - bb);
+          hi = sum;
+        }
+      }
+      const double Cp = hi + lo;
       C[(long)i * ldc + j0 + u] = ldexp(Cp, -(si + tj[u]));
     }
   }
@@ -605,14 +705,41 @@ struct Fp64AccPlan {
   int mode;  // 0 bf16, 1 int8
   int nmod;
   int P;
+  double est;  // estimated relative Frobenius error (automatic count), else 0
   CrtConsts host;
 };
 
-bzg_status plan_fp64acc(int64_t k, int32_t moduli, int mode, Fp64AccPlan& pl) {
+// Automatic moduli count: the smallest count whose operand precision P brings the estimated
+// relative Frobenius error below kAutoTarget.  Rounding A_i to P bits below 2^(ilogb(max)+1)
+// leaves independent errors of variance (2^-P D_i)^2 / 12 per entry, so
+//   ||dC||_F^2 ~ 2^-2P / 12 (sum_i D_i^2 ||B||_F^2 + sum_j E_j^2 ||A||_F^2),
+// and for operands with independent signs ||C||_F^2 ~ sum_q ||A_:q||^2 ||B_q:||^2 = CC, giving
+//   est(P) = 2^-P sqrt((sum D_i^2 ||B||_F^2 + sum E_j^2 ||A||_F^2) / (12 CC)).
+// (Uniform [-1, 1) entries: est = 0.71 2^-P; for wide exponent spreads CC is far below
+// ||A||^2 ||B||^2 / k, which is what makes them need more moduli.)  Sign-coherent operands make
+// ||C|| larger and the estimate conservative; a product that cancels to far below sqrt(CC) is
+// not covered by any a-priori choice.
+constexpr double kAutoTarget = 4e-15;
+
+double auto_estimate(int P, int64_t, const double* st) {
+  if (!(st[4] > 0.0)) return 0.0;  // a zero operand or no overlapping support: C = 0 exactly
+  const double q = (st[0] * st[3] + st[2] * st[1]) / (12.0 * st[4]);
+  if (!(q < 1e300)) return 1.0;  // overflowing statistics (|entries| ~ 1e150): most moduli
+  return std::ldexp(std::sqrt(q), -P);
+}
+
+bzg_status plan_fp64acc(int64_t k, int32_t moduli, int mode, Fp64AccPlan& pl, const double* stats = nullptr) {
   const uint32_t top = mode == 0 ? 511u : 255u;
   int nm = moduli;
-  if (nm <= 0) {
-    // smallest count giving P >= 50 bits per operand (rel. error ~2^-50 for row/col-scaled data)
+  pl.est = 0.0;
+  if (nm <= 0 && stats) {
+    for (nm = 4; nm < kMaxModuli; ++nm) {
+      const int P = std::min(operand_bits(pick_moduli(nm, top), k), 104);
+      if (P >= 1 && auto_estimate(P, k, stats) <= kAutoTarget) break;
+    }
+    pl.est = auto_estimate(std::min(operand_bits(pick_moduli(nm, top), k), 104), k, stats);
+  } else if (nm <= 0) {
+    // without statistics: smallest count giving P >= 50 bits per operand
     for (nm = 4; nm <= kMaxModuli; ++nm)
       if (operand_bits(pick_moduli(nm, top), k) >= 50) break;
   }
@@ -625,7 +752,7 @@ bzg_status plan_fp64acc(int64_t k, int32_t moduli, int mode, Fp64AccPlan& pl) {
   BZG_REQUIRE(pl.P >= 1, BZG_ERR_UNSUPPORTED, "fp64acc: too few moduli for this k");
   pl.P = std::min(pl.P, 104);  // |A'| <= 2^104 keeps the residue split exact
   BZG_REQUIRE(mode == 0 || k <= (1LL << 17), BZG_ERR_UNSUPPORTED, "fp64acc int8: k > 2^17");
-  build_crt(ms, pl.host);
+  build_crt(ms, pl.host, mode == 1 ? 40 : 39);
   return BZG_OK;
 }
 
@@ -643,10 +770,65 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   if (m == 0 || n == 0) return BZG_OK;
   if (k == 0) return bzg_dgemm(ctx, m, n, 0, 1.0, A, lda, B, ldb, 0.0, C, ldc);
   BZG_REQUIRE(A && B && C, BZG_ERR_INVALID_ARGUMENT, "fp64acc: null pointer");
-  Fp64AccPlan pl;
-  bzg_status st = plan_fp64acc(k, moduli, mode, pl);
-  if (st) return st;
+  BZG_REQUIRE(moduli >= 0, BZG_ERR_INVALID_ARGUMENT, "fp64acc: moduli must be 0 (automatic) or 2..24");
   ProfScope prof_total(ctx, "fp64acc_total");
+  // ---- operand statistics (row / column maxima, sums of squares) ----
+  const long ny = std::min<long>((k + 255) / 256, 16384);
+  const long nya = std::min<long>((m + 255) / 256, 16384);
+  const size_t stat_bytes = 64 + (size_t)m * 16 + (size_t)n * 8 + (size_t)ny * n * 8 + (size_t)nya * k * 8 +
+                            (size_t)k * 16 + (size_t)(n + k) * 8;
+  void* sws = nullptr;
+  bzg_status st = workspace(ctx, 8, stat_bytes, &sws);
+  if (st) return st;
+  double* sout = static_cast<double*>(sws);
+  double* rmax = sout + 8;
+  double* rn2 = rmax + m;
+  unsigned long long* cmaxB = reinterpret_cast<unsigned long long*>(rn2 + m);
+  double* cn2 = reinterpret_cast<double*>(cmaxB + n);
+  double* acn2 = cn2 + ny * n;     // partial column sums of squares of A
+  double* bmax = acn2 + nya * k;   // row maxima of B (unused) and row sums of squares
+  double* brn2 = bmax + k;
+  double* bcn2 = brn2 + k;         // column sums of squares of B and of A
+  double* acol = bcn2 + n;
+  hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)((m * 64 + 255) / 256)), dim3(256), 0, ctx->stream, (int)m,
+                     (int)k, A, (long)lda, rmax, rn2);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
+                     cmaxB);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((n + 63) / 64), (unsigned)ny), dim3(256), 0, ctx->stream,
+                     (int)k, (int)n, B, (long)ldb, cmaxB, cn2);
+  BZG_LAUNCH_CHECK();
+  Fp64AccPlan pl;
+  if (moduli == 0) {
+    // automatic count: column norms of A and row norms of B complete the statistics, which
+    // come back to the host (one stream synchronisation)
+    hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((k + 63) / 64), (unsigned)nya), dim3(256), 0, ctx->stream,
+                       (int)m, (int)k, A, (long)lda, (unsigned long long*)nullptr, acn2);
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)((k * 64 + 255) / 256)), dim3(256), 0, ctx->stream, (int)k,
+                       (int)n, B, (long)ldb, bmax, brn2);
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)ny,
+                       (int)n, cn2, bcn2);
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, ctx->stream, (int)nya,
+                       (int)k, acn2, acol);
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(fp64acc_stats_kernel, dim3(1), dim3(1024), 0, ctx->stream, (int)m, (int)n, (int)k, rmax, rn2,
+                       cmaxB, bcn2, acol, brn2, sout);
+    BZG_LAUNCH_CHECK();
+    double hs[5];
+    BZG_HIP_CHECK(hipMemcpyAsync(hs, sout, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+    BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    st = plan_fp64acc(k, 0, mode, pl, hs);
+  } else {
+    st = plan_fp64acc(k, moduli, mode, pl);
+  }
+  if (st) return st;
+  ctx->fp64acc_moduli = pl.nmod;
+  ctx->fp64acc_bits = pl.P;
+  ctx->fp64acc_est = pl.est;
   const int esz = mode == 0 ? 2 : 1;
   const int kq = RBYTES / esz;  // K elements per stage
   const long mp = (m + RT - 1) / RT * RT, np = (n + RT - 1) / RT * RT;
@@ -654,7 +836,7 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   const size_t a_bytes = (size_t)pl.nmod * mp * kp * esz;
   const size_t b_bytes = (size_t)pl.nmod * np * kp * esz;
   const size_t c_bytes = (size_t)pl.nmod * mp * np * (mode == 0 ? sizeof(int16_t) : sizeof(int8_t));
-  const size_t misc_bytes = 4096 + (size_t)(mp + np) * sizeof(int) + (size_t)np * 8;
+  const size_t misc_bytes = 4096 + (size_t)(mp + np) * sizeof(int);
   const size_t total = a_bytes + b_bytes + c_bytes + misc_bytes;
   void* ws = nullptr;
   st = workspace(ctx, 4, total, &ws);
@@ -663,21 +845,14 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   CrtConsts* dcc = reinterpret_cast<CrtConsts*>(base);
   int* sA = reinterpret_cast<int*>(base + 4096);
   int* tB = sA + mp;
-  unsigned long long* cmaxB = reinterpret_cast<unsigned long long*>(base + 4096 + (size_t)(mp + np) * sizeof(int));
   char* Ares = base + misc_bytes;
   char* Bres = Ares + a_bytes;
   char* Cres = Bres + b_bytes;
   static_assert(sizeof(CrtConsts) <= 4096, "CRT constants block");
   BZG_HIP_CHECK(hipMemcpyAsync(dcc, &pl.host, sizeof(CrtConsts), hipMemcpyHostToDevice, ctx->stream));
 
-  hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)((m * 64 + 255) / 256)), dim3(256), 0, ctx->stream,
-                     (int)m, (int)k, A, (long)lda, pl.P, sA);
-  BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
-                     cmaxB);
-  BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((n + 63) / 64), (unsigned)std::min<long>((k + 255) / 256, 16384)), dim3(256),
-                     0, ctx->stream, (int)k, (int)n, B, (long)ldb, cmaxB);
+  hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, (int)m, rmax,
+                     pl.P, sA);
   BZG_LAUNCH_CHECK();
   hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
                      cmaxB, pl.P, tB);
@@ -748,6 +923,14 @@ bzg_status bzg_dgemm_fp64acc_i8(bzg_context* ctx, int64_t m, int64_t n, int64_t 
                                 int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
                                 int32_t moduli) {
   return bzg::dgemm_fp64acc_impl(ctx, m, n, k, A, lda, B, ldb, C, ldc, moduli, 1);
+}
+
+bzg_status bzg_fp64acc_last_plan(bzg_context* ctx, int32_t* moduli, int32_t* bits, double* est_rel_err) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "fp64acc_last_plan: null context");
+  if (moduli) *moduli = ctx->fp64acc_moduli;
+  if (bits) *bits = ctx->fp64acc_bits;
+  if (est_rel_err) *est_rel_err = ctx->fp64acc_est;
+  return BZG_OK;
 }
 
 int32_t bzg_fp64acc_slices(int64_t k, int32_t moduli) {

@@ -223,9 +223,13 @@ bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_h
 
 /* ---- FP64-accurate GEMM from BF16 residue slices (Ozaki scheme II on MFMA) --------------
  * Same contract as bzg_dgemm with alpha = 1, beta = 0.  `moduli` selects the number of
- * coprime moduli (BF16 slice GEMMs); 0 picks it from k (>= 1e-14 relative Frobenius
- * accuracy for inputs without extreme exponent spread).  Workspace is allocated from the
- * context on first use and cached. */
+ * coprime moduli (slice GEMMs), 2..24.  0 = automatic: row / column maxima and sums of squares
+ * of A and B give an estimate of the relative Frobenius error for each count,
+ * 2^-P sqrt(k/12 (sum_i D_i^2 / ||A||^2 + sum_j E_j^2 / ||B||^2)) with D_i (E_j) the power of
+ * two above row i's (column j's) largest |entry|, and the smallest count whose estimate is
+ * <= 4e-15 is used (24 if none is); this reads 4 numbers back to the host (one stream
+ * synchronisation), so capture a graph with an explicit count.  Workspace is allocated from
+ * the context on first use and cached. */
 bzg_status bzg_dgemm_fp64acc(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                              const double* A, int64_t lda, const double* B, int64_t ldb,
                              double* C, int64_t ldc, int32_t moduli);
@@ -234,7 +238,11 @@ bzg_status bzg_dgemm_fp64acc(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
 bzg_status bzg_dgemm_fp64acc_i8(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                                 const double* A, int64_t lda, const double* B, int64_t ldb,
                                 double* C, int64_t ldc, int32_t moduli);
-/* Number of BF16 slice GEMMs that bzg_dgemm_fp64acc will run for this k / moduli choice. */
+/* Plan of the last FP64-accurate GEMM on this context: moduli count, operand bits P and the
+ * error estimate used by the automatic choice (0 for an explicit count); any may be NULL. */
+bzg_status bzg_fp64acc_last_plan(bzg_context* ctx, int32_t* moduli, int32_t* bits, double* est_rel_err);
+/* Number of BF16 slice GEMMs that bzg_dgemm_fp64acc runs for this k and an explicit moduli
+ * count (0: the count giving P >= 50 bits, the choice without operand statistics). */
 int32_t bzg_fp64acc_slices(int64_t k, int32_t moduli);
 /* Bits P kept per scaled operand (|A'|,|B'| <= 2^P) for this k / moduli choice; the
  * elementwise error is bounded by k * 2^-P * max|A_i,:| * max|B_:,j|.  -1 if invalid. */
