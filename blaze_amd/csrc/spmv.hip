@@ -159,9 +159,11 @@ __global__ __launch_bounds__(256) void csr_vector_kernel(int64_t n_rows, const i
 // fully coalesced loads.  The products go to LDS and thread t then adds the products of rows
 // t and t+256 in storage order: the reference's sequential sum, bit for bit.  A block
 // descriptor holds (first row, end row, first nonzero, end nonzero).
-// Persistent and software-pipelined: each workgroup walks a contiguous range of blocks (ranges
-// laid out XCD by XCD) and issues the value/column loads of block i+1 before it gathers x for
-// block i, so the gather latency overlaps the stream.
+// One workgroup per block, in plain dispatch order: the workgroups in flight then cover one
+// narrow window of the nonzero stream, which keeps few DRAM pages open at a time (measured on
+// the 7-point Laplacians: 5.2 TB/s at 256^3 against 4.0-4.4 TB/s for a persistent grid whose
+// workgroups each walk their own range), and the x entries that neighbouring blocks gather are
+// shared through the last-level cache.  One LDS buffer of BNNZ products per workgroup.
 constexpr int kBlockRowsMax = 512;
 constexpr int kShortRow = 32;  // rows up to this length are summed sequentially
 
@@ -171,69 +173,54 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n_blocks, const 
                                                         const int32_t* __restrict__ colidx,
                                                         const T* __restrict__ val,
                                                         const T* __restrict__ x, T* __restrict__ y) {
-  __shared__ T prod[2][BNNZ];
+  __shared__ T prod[BNNZ];
   constexpr int PER = BNNZ / 256;
   const int tid = threadIdx.x;
-  const int64_t G = gridDim.x;
-  const int64_t per = (n_blocks + G - 1) / G;
-  int64_t b = xcd_block() * per;
-  const int64_t bend = min(n_blocks, b + per);
-  if (b >= bend) return;  // uniform over the workgroup
-  T v[PER];
-  int32_t c[PER];
-  auto load = [&](const int4& d) {
+  const int64_t b = blockIdx.x;
+  if (b >= n_blocks) return;  // uniform over the workgroup
+  const int4 cur = blocks[b];
+  const int cnt = cur.w - cur.z;
+  T p[PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + 256 * i;
-      const bool ok = e < d.w - d.z;
-      if (NT) {
-        v[i] = ok ? __builtin_nontemporal_load(val + d.z + e) : T(0);
-        c[i] = ok ? __builtin_nontemporal_load(colidx + d.z + e) : 0;
-      } else {
-        v[i] = ok ? val[d.z + e] : T(0);
-        c[i] = ok ? colidx[d.z + e] : 0;
-      }
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + 256 * i;
+    const bool ok = e < cnt;
+    T v;
+    int32_t c;
+    if (NT) {
+      v = ok ? __builtin_nontemporal_load(val + cur.z + e) : T(0);
+      c = ok ? __builtin_nontemporal_load(colidx + cur.z + e) : 0;
+    } else {
+      v = ok ? val[cur.z + e] : T(0);
+      c = ok ? colidx[cur.z + e] : 0;
     }
-  };
-  int4 d = blocks[b];
-  load(d);
-  int buf = 0;
-  for (; b < bend; ++b) {
-    const int4 cur = d;
-    T p[PER];
+    p[i] = mul_rn(v, x[c]);  // padding lanes gather x[0] (the host never launches this with nnz = 0)
+  }
+  const int nrow = cur.y - cur.x;
+  int32_t rs[2], re[2];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) p[i] = mul_rn(v[i], x[c[i]]);
-    const int nrow = cur.y - cur.x;
-    int32_t rs[2], re[2];
+  for (int h = 0; h < 2; ++h) {
+    const int t = tid + 256 * h;
+    rs[h] = re[h] = 0;
+    if (t < nrow) {
+      rs[h] = rowptr[cur.x + t] - cur.z;
+      re[h] = rowptr[cur.x + t + 1] - cur.z;
+    }
+  }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int t = tid + 256 * h;
-      rs[h] = re[h] = 0;
-      if (t < nrow) {
-        rs[h] = rowptr[cur.x + t] - cur.z;
-        re[h] = rowptr[cur.x + t + 1] - cur.z;
-      }
-    }
-    if (b + 1 < bend) {
-      d = blocks[b + 1];
-      load(d);
-    }
+  for (int i = 0; i < PER; ++i) {
+    const int e = tid + 256 * i;
+    if (e < cnt) prod[e] = p[i];
+  }
+  __syncthreads();
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + 256 * i;
-      if (e < cur.w - cur.z) prod[buf][e] = p[i];
+  for (int h = 0; h < 2; ++h) {
+    const int t = tid + 256 * h;
+    if (t < nrow) {
+      T s = neg_zero<T>();
+      for (int32_t k = rs[h]; k < re[h]; ++k) s = add_rn(s, prod[k]);
+      y[cur.x + t] = (re[h] == rs[h]) ? T(0) : s;
     }
-    __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int t = tid + 256 * h;
-      if (t < nrow) {
-        T s = neg_zero<T>();
-        for (int32_t k = rs[h]; k < re[h]; ++k) s = add_rn(s, prod[buf][k]);
-        y[cur.x + t] = (re[h] == rs[h]) ? T(0) : s;
-      }
-    }
-    buf ^= 1;  // the other buffer is rewritten only after the next barrier
   }
 }
 
@@ -286,6 +273,13 @@ __global__ __launch_bounds__(256) void csr_rows_kernel(int64_t n_rows, const int
   }
 }
 
+// y = 0 for a matrix without nonzeros (x may then be null, so no SpMV kernel runs).
+template <typename T>
+__global__ __launch_bounds__(256) void zero_rows_kernel(int64_t n, T* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = T(0);
+}
+
 template <typename T>
 bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,
                  const int32_t* colidx, const T* val, const T* x, T* y) {
@@ -294,9 +288,14 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
   BZG_REQUIRE(rowptr && y, BZG_ERR_INVALID_ARGUMENT, "csrmv: null rowptr or y");
   if (plan->nnz > 0) BZG_REQUIRE(colidx && val && x, BZG_ERR_INVALID_ARGUMENT, "csrmv: null CSR arrays or x");
   ProfScope prof(ctx, "csrmv");
+  if (plan->nnz == 0) {
+    hipLaunchKernelGGL(zero_rows_kernel<T>, dim3((unsigned)((plan->n_rows + 255) / 256)), dim3(256), 0,
+                       ctx->stream, plan->n_rows, y);
+    BZG_LAUNCH_CHECK();
+    return BZG_OK;
+  }
   if (plan->n_short > 0 && plan->mode == 0) {
-    // persistent grid: 8 workgroups per CU (multiple of the 8 XCDs), at most one per block
-    const int64_t blocks = std::min<int64_t>((int64_t)ctx->num_cus * 8, (plan->n_blocks + 7) / 8 * 8);
+    const int64_t blocks = plan->n_blocks;  // one workgroup per row block
     BZG_REQUIRE(blocks < (1LL << 31), BZG_ERR_UNSUPPORTED, "csrmv: too many row blocks");
     if (plan->block_nnz == 4096) {
       if (plan->block_nt)

@@ -218,3 +218,14 @@ def test_spmv_empty_rows_and_int32():
     va = np.array([2, -3, 5], np.int32); x = np.array([1, 2, 3, 4], np.int32)
     got, _ = gpu_csrmv(rp, ci, va, x, 4)
     assert got.tolist() == [0, 2 * 2 - 3 * 4, 0, 5]
+
+
+def test_spmv_no_nonzeros_null_arrays():
+    # rows without any nonzero: y = 0, and x / colidx / val may be null (nothing is gathered)
+    for dt, fn in ((np.float64, "bzg_dcsrmv"), (np.float32, "bzg_scsrmv")):
+        rp = np.zeros(1001, np.int32)
+        plan = Plan(rp, 1000)
+        drp = dev(rp)
+        dy = full((1000,), dt, -5)
+        ctx().call(fn, plan.h, p(drp), C.c_void_p(0), C.c_void_p(0), C.c_void_p(0), p(dy))
+        assert not host(dy).any()
