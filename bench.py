@@ -211,6 +211,15 @@ class Bench:
         flops = float(np.sum((2.0 / 3.0) * n.astype(np.float64) ** 3 - 0.5 * n.astype(np.float64) ** 2
                              + (5.0 / 6.0) * n + r * (2.0 * n.astype(np.float64) ** 2 - n)))
         out = {"systems": count, "flops_getrf_getrs": flops}
+        # project autotuner: persistent-grid size (workgroups per CU) of each solver on this batch
+        tuned = {}
+        for mixed in (0, 1):
+            wg, tms = C.c_int32(0), C.c_float(0)
+            self.ctx.call("bzg_solver_autotune", mixed, count, vp(dn), vp(dr), vp(da), vp(db), vp(dA0), na,
+                          vp(dB0), nb, vp(order), C.byref(wg), C.byref(tms))
+            tuned["mixed" if mixed else "fp64"] = {"workgroups_per_cu": wg.value, "ms": tms.value}
+        out["autotuned_grid"] = tuned
+        log(f"[solve] autotuned grid: {tuned}")
 
         # plain FP64 (A, B overwritten: restore from the pristine copies before each run)
         dA, dB = torch.empty_like(dA0), torch.empty_like(dB0)
@@ -231,27 +240,30 @@ class Bench:
         out["fp64_lu"]["info_nonzero"] = int((dinfo != 0).sum().item())
         del dA
 
+        # mixed precision, two stopping tests: the 1e-12 target (tol = 1e-12 / sqrt(256), so
+        # ||r|| <= 1e-12 ||A|| ||x|| for every n <= 256) and LAPACK dsgesv's own (eps = 2^-53)
         dX = torch.empty_like(dB0)
         dit = torch.empty(count, dtype=torch.int32, device="cuda")
-        lat = []
-        for it in range(warmup + steps):
-            ev[0].record()
-            self.ctx.call("bzg_dsgesv_batched", count, vp(dn), vp(dr), vp(da), vp(db), vp(dA0), vp(dB0),
-                          vp(dX), vp(dit), vp(dinfo), 0.0, 30, vp(order))
-            ev[1].record()
-            self.sync()
-            if it >= warmup:
-                lat.append(ev[0].elapsed_time(ev[1]) / 1e3)
-        out["mixed_fp32_lu_fp64_ir"] = self._lat_stats(lat, count, flops)
-        its = dit.cpu().numpy()
-        out["mixed_fp32_lu_fp64_ir"].update({
-            "fallback_fp64": int((its < 0).sum()), "mean_iterations": float(its[its >= 0].mean()),
-            "max_iterations": int(its[its >= 0].max()) if (its >= 0).any() else None})
-        out["speedup_mixed_vs_fp64"] = (out["mixed_fp32_lu_fp64_ir"]["systems_per_s"] /
-                                        out["fp64_lu"]["systems_per_s"])
-        # accuracy: normwise backward error of every system, on the GPU in FP64 (grouped by n)
-        out["mixed_fp32_lu_fp64_ir"]["max_backward_error"] = self._backward_errors(
-            n, r, a_off, b_off, dA0, dB0, dX)
+        for key, tol in (("mixed_fp32_lu_fp64_ir", 1e-12 / 16), ("mixed_lapack_eps", 0.0)):
+            lat = []
+            for it in range(warmup + steps):
+                ev[0].record()
+                self.ctx.call("bzg_dsgesv_batched", count, vp(dn), vp(dr), vp(da), vp(db), vp(dA0), vp(dB0),
+                              vp(dX), vp(dit), vp(dinfo), tol, 30, vp(order))
+                ev[1].record()
+                self.sync()
+                if it >= warmup:
+                    lat.append(ev[0].elapsed_time(ev[1]) / 1e3)
+            rec = self._lat_stats(lat, count, flops)
+            its = dit.cpu().numpy()
+            rec.update({
+                "tol": tol, "fallback_fp64": int((its < 0).sum()), "mean_iterations": float(its[its >= 0].mean()),
+                "max_iterations": int(its[its >= 0].max()) if (its >= 0).any() else None,
+                "speedup_vs_fp64": rec["systems_per_s"] / out["fp64_lu"]["systems_per_s"]})
+            # accuracy: normwise backward error on a sample, on the GPU in FP64 (grouped by n)
+            rec["max_backward_error"] = self._backward_errors(n, r, a_off, b_off, dA0, dB0, dX)
+            out[key] = rec
+        out["speedup_mixed_vs_fp64"] = out["mixed_fp32_lu_fp64_ir"]["speedup_vs_fp64"]
         out["fp64_lu"]["max_backward_error"] = self._backward_errors(n, r, a_off, b_off, dA0, dB0, X64)
         return out
 

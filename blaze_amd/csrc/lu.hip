@@ -62,6 +62,7 @@ struct Misc {
   int swsrc[2 * NB];          // interchange sources: rows j0.. (first NB), pivot rows (last NB)
   int swdst[2 * NB];          // ... and their destinations (-1: none)
   int perm[NM];               // row i of the factored system = original row perm[i]
+  int pinv[NM];               // original row r is row pinv[r] of the factored system
   int sys;
   unsigned long long cmax[RMAX * 2];  // column maxima (bit patterns of non-negative doubles)
   int flag;
@@ -143,24 +144,51 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
           // Candidates: logical rows jj..m-1.  A NaN entry ranks below every number (as in
           // idamax, where NaN never compares greater) but above the non-candidates, so the
           // pivot is always a row of the matrix.  Ties: the lower logical row.
-          double v = -1.0;
-          int r = 1 << 30, wq = 0;
+          int p;
+          if constexpr (sizeof(T) == 4) {
+            // one 64-bit key per candidate: (|a| bits + 2, or 1 for NaN, 0 for a non-candidate)
+            // above (2^32 - 1 - logical row); the wavefront maximum is the pivot
+            unsigned long long key = 0ull;
 #pragma unroll
-          for (int q = 0; q < Q; ++q) {
-            double a = (lpos[q] >= jj && lpos[q] < m) ? (double)tabs(P[q][jj]) : -1.0;
-            if (a != a) a = -0.5;
-            if (a > v || (a == v && lpos[q] < r)) { v = a; r = lpos[q]; wq = q; }
-          }
-          int wl = lane;
+            for (int q = 0; q < Q; ++q) {
+              const bool cand = lpos[q] >= jj && lpos[q] < m;
+              const float a = (float)P[q][jj];
+              const unsigned hi = !cand ? 0u : (a != a ? 1u : (__float_as_uint(a) & 0x7FFFFFFFu) + 2u);
+              const unsigned long long k = ((unsigned long long)hi << 32) | (0xFFFFFFFFu - (unsigned)lpos[q]);
+              key = k > key ? k : key;
+            }
 #pragma unroll
-          for (int off = 32; off >= 1; off >>= 1) {
-            const double ov = __shfl_xor(v, off, 64);
-            const int orr = __shfl_xor(r, off, 64);
-            const int owl = __shfl_xor(wl, off, 64), owq = __shfl_xor(wq, off, 64);
-            if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; wl = owl; wq = owq; }
+            for (int off = 32; off >= 1; off >>= 1) {
+              const unsigned long long o = __shfl_xor(key, off, 64);
+              key = o > key ? o : key;
+            }
+            p = (int)(0xFFFFFFFFu - (unsigned)__builtin_amdgcn_readfirstlane((int)(key & 0xFFFFFFFFull)));
+          } else {
+            double v = -1.0;
+            int r = 1 << 30;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+              double a = (lpos[q] >= jj && lpos[q] < m) ? (double)tabs(P[q][jj]) : -1.0;
+              if (a != a) a = -0.5;
+              if (a > v || (a == v && lpos[q] < r)) { v = a; r = lpos[q]; }
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+              const double ov = __shfl_xor(v, off, 64);
+              const int orr = __shfl_xor(r, off, 64);
+              if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; }
+            }
+            p = __builtin_amdgcn_readfirstlane(r);
           }
-          const int p = __builtin_amdgcn_readfirstlane(r);
-          const int pl = __builtin_amdgcn_readfirstlane(wl), pq = __builtin_amdgcn_readfirstlane(wq);
+          // physical place of logical row p: the lane holding it (ballot) and its register row
+          bool mine = false;
+          int myq = 0;
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+            if (lpos[q] == p) { mine = true; myq = q; }
+          const unsigned long long hold = __ballot(mine);
+          const int pl = __builtin_amdgcn_readfirstlane(__ffsll((long long)hold) - 1);
+          const int pq = __builtin_amdgcn_readlane(myq, pl);
           // pivot row values (physical row (pl, pq)), kept in scalar registers
           T pr[NB];
 #pragma unroll
@@ -287,7 +315,7 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
     // Each wavefront takes its tiles in batches of TB: all loads of a batch are issued before
     // the MFMAs, so a batch costs one HBM/L2 round trip instead of one per tile.
     if (rem > 0) {
-      constexpr int TB = sizeof(T) == 8 ? 4 : 8;
+      constexpr int TB = (sizeof(T) == 8 ? 16 : 32) / (NM > 256 ? 2 : 1);  // register budget
       const int tiles = (rem + 15) / 16;
       const int r0 = j0 + NB;
       const int total = tiles * tiles;
@@ -336,9 +364,10 @@ __device__ __forceinline__ int lu_factor(T* __restrict__ A, int n, int lda, int3
 // Dot products of up to 16 consecutive rows of M (row0 .. row0+nrows-1) with the columns of
 // the LDS matrix X (ld RMAX) over k in [kb, ke):  out(r, c) = sum_k M(row0+r, k) * X(k, c).
 // Thread t handles rows t/16 + 4j (j < 4) and k = kb + t%16 (mod 16): the loads of all 16
-// rows are in flight together.  The 16 partials of a row are combined by a fixed butterfly;
-// every lane of a 16-lane group ends with the row's sums.
-template <typename TM, typename TX, typename TA>
+// rows over U k-steps (16 x 16U values) are in flight together, so a row block costs
+// ceil((ke - kb) / 16U) memory round trips.  The 16 partials of a row are combined by a fixed
+// butterfly; every lane of a 16-lane group ends with the row's sums.
+template <int U, typename TM, typename TX, typename TA>
 __device__ __forceinline__ void rows_dot(const TM* __restrict__ M, int lda, int row0, int nrows,
                                          int kb, int ke, const TX* __restrict__ X, int nrhs,
                                          TA (&acc)[4][RMAX]) {
@@ -348,18 +377,17 @@ __device__ __forceinline__ void rows_dot(const TM* __restrict__ M, int lda, int 
   for (int j = 0; j < 4; ++j)
 #pragma unroll
     for (int c = 0; c < RMAX; ++c) acc[j][c] = TA(0);
-  // four k-steps (16 rows x 64 values) of loads in flight before the multiply-adds
-  for (int k0 = kb + ks; k0 < ke; k0 += 64) {
-    TA a[4][4];
+  for (int k0 = kb + ks; k0 < ke; k0 += 16 * U) {
+    TA a[U][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int r = r0 + 4 * j, k = k0 + 16 * u;
         a[u][j] = (r < nrows && k < ke) ? (TA)M[(long)(row0 + r) * lda + k] : TA(0);
       }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int k = k0 + 16 * u;
       if (k < ke) {
 #pragma unroll
@@ -399,7 +427,7 @@ __device__ __forceinline__ void lu_solve(const T* __restrict__ LU, int n, int ld
       dv[j] = (r < nr && ks < nr) ? LU[(long)(b0 + r) * lda + b0 + ks] : T(0);
     }
     T acc[4][RMAX];
-    rows_dot<T, T, T>(LU, lda, b0, nr, kb, ke, X, nrhs, acc);
+    rows_dot<8, T, T, T>(LU, lda, b0, nr, kb, ke, X, nrhs, acc);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int r = r0 + 4 * j;
@@ -519,9 +547,8 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32
 // ---- mixed precision: FP32 LU + FP64 refinement (dsgesv) --------------------------------
 template <int NM>
 struct IRShm {
-  double x[NM * RMAX];
-  double r[NM * RMAX];
-  float d[NM * RMAX];
+  double x[NM * RMAX];        // current solution (FP64)
+  float d[NM * RMAX];         // P r in FP32, solved in place into the correction
   float diag[16 * 17];
 };
 
@@ -557,35 +584,41 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
     if (tid < 2 * RMAX) misc.cmax[tid] = 0ull;
     __syncthreads();
     {
-      // 16 lanes per row, 4 rows per step, 16 loads per lane in flight per 256 columns; the row
-      // sums of |a| use a fixed order (lane-strided partial sums, then a butterfly) so
-      // ||A||_inf and with it the stopping test are reproducible run to run
+      // 16 lanes per row, 16 rows per step (rows r0 + 4j), 64 loads per lane in flight per 256
+      // columns; the row sums of |a| use a fixed order (lane-strided partial sums, then a
+      // butterfly) so ||A||_inf and with it the stopping test are reproducible run to run
       const int r0 = tid / 16, ks = tid % 16;
       double mx = 0.0;
-      for (int rb = 0; rb < n; rb += NT / 16) {
-        const int r = rb + r0;
-        double sum = 0.0;
-        if (r < n) {
-          const double* ar = a + (long)r * n;
-          float* wr = W + (long)r * n;
-          for (int k0 = 0; k0 < n; k0 += 256) {
-            double v[16];
+      for (int rb = 0; rb < n; rb += 16) {
+        double sum[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < n; k0 += 256) {
+          double v[4][16];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int r = rb + r0 + 4 * j;
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
               const int k = k0 + ks + 16 * u;
-              v[u] = (k < n) ? ar[k] : 0.0;
+              v[j][u] = (r < n && k < n) ? a[(long)r * n + k] : 0.0;
             }
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int r = rb + r0 + 4 * j;
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
               const int k = k0 + ks + 16 * u;
-              if (k < n) wr[k] = (float)v[u];
-              sum += fabs(v[u]);
+              if (r < n && k < n) W[(long)r * n + k] = (float)v[j][u];
+              sum[j] += fabs(v[j][u]);
             }
           }
         }
 #pragma unroll
-        for (int off = 8; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
-        mx = fmax(mx, sum);
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int off = 8; off >= 1; off >>= 1) sum[j] += __shfl_xor(sum[j], off, 64);
+          mx = fmax(mx, sum[j]);
+        }
       }
 #pragma unroll
       for (int off = 32; off >= 16; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
@@ -607,6 +640,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
       __syncthreads();
       lu_solve<float>(W, n, n, sh.ir.d, nrhs, sh.ir.diag);
       for (int e = tid; e < n * RMAX; e += NT) sh.ir.x[e] = (double)sh.ir.d[e];
+      for (int i = tid; i < n; i += NT) misc.pinv[misc.perm[i]] = i;
       __syncthreads();
       double prev = 1e308;
       int stalls = 0;
@@ -618,7 +652,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
         for (int rb = 0; rb < n; rb += 16) {
           const int nr = min(16, n - rb);
           double acc[4][RMAX];
-          rows_dot<double, double, double>(a, n, rb, nr, 0, n, sh.ir.x, nrhs, acc);
+          rows_dot<16, double, double, double>(a, n, rb, nr, 0, n, sh.ir.x, nrhs, acc);
           const int r0 = tid / 16, ks = tid % 16;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
@@ -630,7 +664,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
               for (int c = 1; c < RMAX; ++c)
                 if (c == ks) v = acc[j][c];
               const double rv2 = b[(long)(rb + r) * nrhs + ks] - v;
-              sh.ir.r[(rb + r) * RMAX + ks] = rv2;
+              sh.ir.d[misc.pinv[rb + r] * RMAX + ks] = (float)rv2;  // P r for the correction
               atomicMax(&misc.cmax[ks], (unsigned long long)__double_as_longlong(fabs(rv2)));
               atomicMax(&misc.cmax[RMAX + ks],
                         (unsigned long long)__double_as_longlong(fabs(sh.ir.x[(rb + r) * RMAX + ks])));
@@ -655,12 +689,7 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
           stalls = 0;
         }
         prev = worst;
-        // d = FP32 solve of P r; x += d
-        for (int e = tid; e < n * RMAX; e += NT) {
-          const int i = e / RMAX, c = e % RMAX;
-          sh.ir.d[e] = (c < nrhs) ? (float)sh.ir.r[misc.perm[i] * RMAX + c] : 0.0f;
-        }
-        __syncthreads();
+        // d = FP32 solve of P r (written by the residual pass); x += d
         lu_solve<float>(W, n, n, sh.ir.d, nrhs, sh.ir.diag);
         for (int e = tid; e < n * RMAX; e += NT) sh.ir.x[e] += (double)sh.ir.d[e];
         __syncthreads();
@@ -902,17 +931,24 @@ bzg_status bzg_solver_autotune(bzg_context* ctx, int32_t mixed, int64_t count, c
   const int saved = knob;
   int best = -1;
   float best_ms = 0.f;
-  auto run = [&]() -> bzg_status {
-    if (mixed)
-      return bzg_dsgesv_batched(ctx, count, n, nrhs, a_off, b_off, A, B, Bw, it, inf, 0.0, 30, order);
+  // the FP64 solver overwrites its operands: fresh copies before every run, outside the timing
+  auto restore = [&]() -> bzg_status {
+    if (mixed) return BZG_OK;
     if (hipMemcpyAsync(Aw, A, (size_t)a_elems * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
         hipMemcpyAsync(Bw, B, (size_t)b_elems * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
       return BZG_ERR_HIP;
+    return BZG_OK;
+  };
+  auto run = [&]() -> bzg_status {
+    if (mixed)
+      return bzg_dsgesv_batched(ctx, count, n, nrhs, a_off, b_off, A, B, Bw, it, inf, 0.0, 30, order);
     return bzg_dgesv_batched(ctx, count, n, nrhs, a_off, b_off, nullptr, Aw, Bw, nullptr, inf, order);
   };
   for (int cand : {1, 2, 3, 4, 6, 8}) {
     knob = cand;
-    st = run();  // warm-up (and workspace growth)
+    st = restore();
+    if (!st) st = run();  // warm-up (and workspace growth)
+    if (!st) st = restore();
     if (st) break;
     hipEventRecord(e0, ctx->stream);
     st = run();
