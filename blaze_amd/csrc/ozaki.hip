@@ -36,7 +36,12 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMaxModuli = 24;
 
-constexpr int kMaxChunks = 6;  // chunks of W_l and M: 24 moduli x 9 bits <= 216 bits, >= 39 bits each
+constexpr int kMaxChunks = 6;
+// Residue planes (one per modulus) are spaced by their size plus 19 x 256 bytes: planes of a
+// power-of-two size would put element (i, j) of every plane in the same DRAM channel and bank,
+// and the kernels touch all planes at once.
+constexpr long kPlanePadBytes = 19 * 256;
+__host__ __device__ constexpr long plane_elems(long elems, int esz) { return elems + kPlanePadBytes / esz; }  // chunks of W_l and M: 24 moduli x 9 bits <= 216 bits, >= 39 bits each
 
 struct CrtConsts {
   double m[kMaxModuli];
@@ -171,29 +176,17 @@ static void build_crt(const std::vector<uint32_t>& ms, CrtConsts& c, int cb) {
 // ------------------------------------------------------------------------------------------
 // device: scaling exponents
 // ------------------------------------------------------------------------------------------
-// Row statistics of A (m x k, lda), one wavefront per row: rmax_i = max_j |a_ij| and
-// rn2_i = sum_j a_ij^2 (fixed summation order: lane-strided partials, then a butterfly).
-__global__ __launch_bounds__(256) void row_stats_kernel(int m, int k, const double* __restrict__ A,
-                                                        long lda, double* __restrict__ rmax,
-                                                        double* __restrict__ rn2) {
+// Row maxima of |A| (m x k, lda), one wavefront per row.
+__global__ __launch_bounds__(256) void row_max_kernel(int m, int k, const double* __restrict__ A, long lda,
+                                                      double* __restrict__ rmax) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, lane = threadIdx.x % 64;
   if (wave >= m) return;
   const double* row = A + (long)wave * lda;
-  double mx = 0.0, sq = 0.0;
-  for (int j = lane; j < k; j += 64) {
-    const double v = row[j];
-    mx = fmax(mx, fabs(v));
-    sq = fma(v, v, sq);
-  }
+  double mx = 0.0;
+  for (int j = lane; j < k; j += 64) mx = fmax(mx, fabs(row[j]));
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    mx = fmax(mx, __shfl_xor(mx, off, 64));
-    sq += __shfl_xor(sq, off, 64);
-  }
-  if (lane == 0) {
-    rmax[wave] = mx;
-    rn2[wave] = sq;
-  }
+  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) rmax[wave] = mx;
 }
 
 // s_i = P - 1 - ilogb(rmax_i)  (0 for zero rows): |rint(a_ij 2^s_i)| <= 2^P.
@@ -206,68 +199,126 @@ __global__ __launch_bounds__(256) void row_scale_kernel(int m, const double* __r
 }
 
 // Column maxima of |B| (k x n, ldb) as FP64 bit patterns (non-negative doubles order like
-// their bit patterns): a (n / 64) x (k / 256) grid of 64-column x 256-row blocks (y strides over K), combined with
-// one atomicMax per column and block.  cmax must be zero on entry.
-// Also writes the block's partial sum of squares of each column to cn2[blockIdx.y * n + c]
-// (fixed order within the block), for the accuracy estimate of the automatic moduli count.
+// their bit patterns): a (n / 64) x (k / 256) grid of 64-column x 256-row blocks (y strides over
+// K), combined with one atomicMax per column and block.  cmax must be zero on entry.
 __global__ __launch_bounds__(256) void col_max_kernel(int k, int n, const double* __restrict__ B,
-                                                      long ldb, unsigned long long* __restrict__ cmax,
-                                                      double* __restrict__ cn2) {
-  __shared__ double part[4][64], psq[4][64];
+                                                      long ldb, unsigned long long* __restrict__ cmax) {
+  __shared__ double part[4][64];
   const int c = blockIdx.x * 64 + threadIdx.x % 64, ph = threadIdx.x / 64;
-  double mx = 0.0, sq = 0.0;
+  double mx = 0.0;
   if (c < n)
     for (int r0 = blockIdx.y * 256; r0 < k; r0 += gridDim.y * 256) {
       const int r1 = min(k, r0 + 256);
-      for (int i = r0 + ph; i < r1; i += 4) {
-        const double v = B[(long)i * ldb + c];
-        mx = fmax(mx, fabs(v));
-        sq = fma(v, v, sq);
-      }
+      for (int i = r0 + ph; i < r1; i += 4) mx = fmax(mx, fabs(B[(long)i * ldb + c]));
     }
   part[ph][threadIdx.x % 64] = mx;
-  psq[ph][threadIdx.x % 64] = sq;
   __syncthreads();
   if (ph == 0 && c < n) {
     mx = fmax(fmax(part[0][threadIdx.x], part[1][threadIdx.x]), fmax(part[2][threadIdx.x], part[3][threadIdx.x]));
-    if (cmax) atomicMax(&cmax[c], (unsigned long long)__double_as_longlong(mx));
-    cn2[(long)blockIdx.y * n + c] = (psq[0][threadIdx.x] + psq[1][threadIdx.x]) + (psq[2][threadIdx.x] + psq[3][threadIdx.x]);
+    atomicMax(&cmax[c], (unsigned long long)__double_as_longlong(mx));
   }
 }
 
-// out[j] = sum_y part[y * n + j] for j < n (fixed order): column sums of squares from the
-// per-block partials of col_max_kernel.
-__global__ __launch_bounds__(256) void sum_partials_kernel(int ny, int n, const double* __restrict__ part,
-                                                           double* __restrict__ out) {
+// One pass over a row-major r x c matrix X (ld ldx) for the automatic moduli count: a
+// workgroup covers 64 rows x 1024 columns (each wavefront 16 rows, lane l columns l + 64u).
+// ROWMAX (A):  rowpart[x][i] = max |x_ij| over column range x, colpart[y][j] = sum of x_ij^2 over
+//              the row blocks of workgroup row y;
+// !ROWMAX (B): rowpart[x][i] = sum of x_ij^2 over column range x, colpart[y][j] = max |x_ij|.
+// Every sum has a fixed order (rows in sequence per lane, a fixed butterfly per row, the four
+// wavefronts in order); reduce_partials_kernel combines the partials.
+template <bool ROWMAX>
+__global__ __launch_bounds__(256) void stats_pass_kernel(int r, int c, const double* __restrict__ X, long ldx,
+                                                         double* __restrict__ rowpart,
+                                                         double* __restrict__ colpart) {
+  __shared__ double comb[4][1024];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * 1024;
+  double acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+  for (int rb = blockIdx.y * 64 + wave * 16; rb < r; rb += gridDim.y * 64) {
+    const int re = min(r, rb + 16);
+    for (int i = rb; i < re; ++i) {
+      const double* row = X + (long)i * ldx;
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int j = c0 + lane + 64 * u;
+        v[u] = (j < c) ? row[j] : 0.0;
+      }
+      double red = 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        if (ROWMAX) {
+          acc[u] = fma(v[u], v[u], acc[u]);
+          red = fmax(red, fabs(v[u]));
+        } else {
+          acc[u] = fmax(acc[u], fabs(v[u]));
+          red = fma(v[u], v[u], red);
+        }
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(red, off, 64);
+        red = ROWMAX ? fmax(red, o) : red + o;
+      }
+      if (lane == 0) rowpart[(long)blockIdx.x * r + i] = red;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) comb[wave][lane + 64 * u] = acc[u];
+  __syncthreads();
+  for (int jj = threadIdx.x; jj < 1024; jj += 256) {
+    const int j = c0 + jj;
+    if (j < c) {
+      const double a = comb[0][jj], b = comb[1][jj], d = comb[2][jj], e = comb[3][jj];
+      colpart[(long)blockIdx.y * c + j] = ROWMAX ? (a + b) + (d + e) : fmax(fmax(a, b), fmax(d, e));
+    }
+  }
+}
+
+// out[j] = sum (MAX: max) over y < ny of part[y * n + j], in order.
+template <bool MAX>
+__global__ __launch_bounds__(256) void reduce_partials_kernel(int ny, int n, const double* __restrict__ part,
+                                                              double* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double a = 0.0;
-  for (int y = 0; y < ny; ++y) a += part[(long)y * n + j];
+  int y = 0;
+  for (; y + 8 <= ny; y += 8) {  // eight loads in flight, combined in order
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[(long)(y + u) * n + j];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a = MAX ? fmax(a, v[u]) : a + v[u];
+  }
+  for (; y < ny; ++y) a = MAX ? fmax(a, part[(long)y * n + j]) : a + part[(long)y * n + j];
   out[j] = a;
 }
 
 // Accuracy statistics (one workgroup, fixed summation order): out[0] = sum_i D_i^2,
 // out[1] = ||A||_F^2, out[2] = sum_j E_j^2, out[3] = ||B||_F^2, out[4] = sum_q ||A_:q||^2 ||B_q:||^2,
-// where D_i = 2^(ilogb(rmax_i)+1) and E_j likewise for the column maxima of B; bcn2 holds the
-// column sums of squares of B, acn2 those of A (k columns), brn2 the row sums of squares of B.
+// where D_i = 2^(ilogb(rmax_i)+1) and E_j likewise for the column maxima of B; acn2 holds the
+// column sums of squares of A, brn2 the row sums of squares of B (both of length k).
 __global__ __launch_bounds__(1024) void fp64acc_stats_kernel(int m, int n, int k,
                                                             const double* __restrict__ rmax,
-                                                            const double* __restrict__ rn2,
                                                             const unsigned long long* __restrict__ cmax,
-                                                            const double* __restrict__ bcn2,
                                                             const double* __restrict__ acn2,
                                                             const double* __restrict__ brn2, double* __restrict__ out) {
   __shared__ double red[5][1024];
   const int t = threadIdx.x;
   double sa = 0, fa = 0, sb = 0, fb = 0, cc = 0;
-  for (int q = t; q < k; q += 1024) cc = fma(acn2[q], brn2[q], cc);
+  for (int q = t; q < k; q += 1024) {
+    cc = fma(acn2[q], brn2[q], cc);
+    fa += acn2[q];
+    fb += brn2[q];
+  }
   for (int i = t; i < m; i += 1024) {
     const double mx = rmax[i];
     if (mx > 0.0) {
       const double d = ldexp(1.0, ilogb(mx) + 1);
       sa = fma(d, d, sa);
     }
-    fa += rn2[i];
   }
   for (int j = t; j < n; j += 1024) {
     const double mx = __longlong_as_double((long long)cmax[j]);
@@ -275,7 +326,6 @@ __global__ __launch_bounds__(1024) void fp64acc_stats_kernel(int m, int n, int k
       const double e = ldexp(1.0, ilogb(mx) + 1);
       sb = fma(e, e, sb);
     }
-    fb += bcn2[j];
   }
   red[0][t] = sa; red[1][t] = fa; red[2][t] = sb; red[3][t] = fb; red[4][t] = cc;
   __syncthreads();
@@ -343,75 +393,109 @@ struct SliceOf<1> {  // INT8: 4 slices packed in 4 bytes
   }
 };
 
-__device__ __forceinline__ double residue(double v, int wide, const CrtConsts* __restrict__ cc, int l) {
-  return wide ? sym_mod_wide(v, cc->m[l], cc->minv[l], cc->c30[l]) : sym_mod(v, cc->m[l], cc->minv[l]);
+// RM = 0: |v| <= 2^50.  Then fl(v * fl(1/m)) is within |v/m| 2^-52 < 1/(2m) of v/m, and v/m (m
+// odd) is at least 1/(2m) away from every half-integer, so rint returns the nearest quotient
+// and v - m q already lies in [-(m-1)/2, (m-1)/2].  RM = 1: |v| <= 2^56, quotient within one
+// (sym_mod's corrections).  RM = 2: wider (sym_mod_wide).
+template <int RM>
+__device__ __forceinline__ double residue(double v, const CrtConsts* __restrict__ cc, int l) {
+  if constexpr (RM == 0) return fma(-cc->m[l], rint(v * cc->minv[l]), v);
+  else if constexpr (RM == 1) return sym_mod(v, cc->m[l], cc->minv[l]);
+  else return sym_mod_wide(v, cc->m[l], cc->minv[l], cc->c30[l]);
 }
 
-// A residues: Ares[l][i][kk] (rows padded to mp, k padded to kp (a multiple of 4), zeros in the
-// padding).  Grid: x covers kp in 1024-element chunks (4 per thread), y strides over rows.
+// 16 residues (one modulus) packed for a 16-byte (INT8) or two 16-byte (BF16) stores.
 template <int MODE>
+__device__ __forceinline__ void store16(typename SliceOf<MODE>::T* __restrict__ o, const double* r) {
+  using S = SliceOf<MODE>;
+  if constexpr (MODE == 1) {
+    *reinterpret_cast<uint4*>(o) = make_uint4(S::pack(r), S::pack(r + 4), S::pack(r + 8), S::pack(r + 12));
+  } else {
+    const uint2 a = S::pack(r), b = S::pack(r + 4), c = S::pack(r + 8), d = S::pack(r + 12);
+    reinterpret_cast<uint4*>(o)[0] = make_uint4(a.x, a.y, b.x, b.y);
+    reinterpret_cast<uint4*>(o)[1] = make_uint4(c.x, c.y, d.x, d.y);
+  }
+}
+
+// A residues: Ares[l][i][kk] (rows padded to mp, k padded to kp (a multiple of 64), zeros in
+// the padding).  16 consecutive kk per thread: 16-byte loads of A where the row is aligned,
+// one 16-byte (INT8) or two (BF16) stores per modulus, so a wavefront writes 1 KB (2 KB)
+// contiguous per plane.  Grid: x covers kp in 4096-element chunks, y strides over rows.
+template <int MODE, int RM>
 __global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const double* __restrict__ A,
                                                          long lda, const int* __restrict__ s, int mp,
-                                                         int kp, int nmod, int wide,
+                                                         int kp, int nmod,
                                                          const CrtConsts* __restrict__ cc,
                                                          typename SliceOf<MODE>::T* __restrict__ out) {
-  using S = SliceOf<MODE>;
-  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 16;
   if (k0 >= kp) return;
-  const long plane = (long)mp * kp;
+  const long plane = plane_elems((long)mp * kp, sizeof(typename SliceOf<MODE>::T));
   for (int i = blockIdx.y; i < mp; i += gridDim.y) {
-    double v[4];
+    double v[16];
+    const double* src = A + (long)i * lda + k0;
+    if (i < m && k0 + 16 <= k && ((uintptr_t)src & 15) == 0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double2 t2 = reinterpret_cast<const double2*>(src)[u];
+        v[2 * u] = t2.x;
+        v[2 * u + 1] = t2.y;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = (i < m && k0 + u < k) ? src[u] : 0.0;
+    }
     const int si = i < m ? s[i] : 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int kk = k0 + u;
-      v[u] = (i < m && kk < k) ? rint(ldexp(A[(long)i * lda + kk], si)) : 0.0;
-    }
-    typename S::T* o = out + (long)i * kp + k0;
+    for (int u = 0; u < 16; ++u) v[u] = rint(ldexp(v[u], si));
+    typename SliceOf<MODE>::T* o = out + (long)i * kp + k0;
     for (int l = 0; l < nmod; ++l) {
-      double r[4];
+      double r[16];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) r[u] = residue(v[u], wide, cc, l);
-      *reinterpret_cast<typename S::Pack*>(o + l * plane) = S::pack(r);
+      for (int u = 0; u < 16; ++u) r[u] = residue<RM>(v[u], cc, l);
+      store16<MODE>(o + l * plane, r);
     }
   }
 }
 
-// B residues, transposed: Bres[l][j][kk] (n rows padded to np, k padded to kp).  64 x 64 tiles:
-// coalesced reads along j into LDS, then 4 consecutive kk per thread, packed stores along kk.
-template <int MODE>
+// B residues, transposed: Bres[l][j][kk] (n rows padded to np, k padded to kp).  A workgroup
+// transposes a 128 (kk) x 32 (j) tile through LDS: reads along j (256 bytes per row), then 16
+// consecutive kk per thread and one 16-byte (INT8) or two (BF16) stores per modulus (128 bytes
+// contiguous per j row).  LDS column c of row r is stored at c ^ (((r >> 4) & 3) << 3), which
+// spreads both access patterns over the banks.  Grid x runs over the j blocks, so neighbouring
+// workgroups read neighbouring pieces of the same rows of B (measured 1.25x faster than kk
+// blocks first at 8192^2).
+template <int MODE, int RM>
 __global__ __launch_bounds__(256) void residues_b_kernel(int k, int n, const double* __restrict__ B,
                                                          long ldb, const int* __restrict__ t, int np,
-                                                         int kp, int nmod, int wide,
+                                                         int kp, int nmod,
                                                          const CrtConsts* __restrict__ cc,
                                                          typename SliceOf<MODE>::T* __restrict__ out) {
-  using S = SliceOf<MODE>;
-  __shared__ double tile[64][65];
-  const int kb = blockIdx.x * 64, jb = blockIdx.y * 64;
-  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-  const int tj = (jb + tx < n) ? t[jb + tx] : 0;
-  for (int r = ty; r < 64; r += 4) {  // read B[kb+r][jb+tx]
-    const int kk = kb + r, j = jb + tx;
-    double v = 0.0;
-    if (kk < k && j < n) v = rint(ldexp(B[(long)kk * ldb + j], tj));
-    tile[r][tx] = v;
+  __shared__ double tile[128][32];
+  const int kb = blockIdx.y * 128, jb = blockIdx.x * 32;
+  const int tid = threadIdx.x;
+  {
+    const int jr = tid & 31, j = jb + jr;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int r = (tid >> 5) + 8 * i, kk = kb + r;
+      tile[r][jr ^ (((r >> 4) & 3) << 3)] = (kk < k && j < n) ? B[(long)kk * ldb + j] : 0.0;
+    }
   }
   __syncthreads();
-  const long plane = (long)np * kp;
-  const int jj = threadIdx.x / 16, g = threadIdx.x % 16;  // 16 rows of j per pass, 4 kk each
-  for (int jr = jj; jr < 64; jr += 16) {
-    const int j = jb + jr, kk = kb + 4 * g;
-    if (j >= np || kk >= kp) continue;
-    double v[4];
+  const int jr = tid >> 3, g = tid & 7;
+  const int j = jb + jr, kk0 = kb + 16 * g;
+  if (j >= np || kk0 >= kp) return;
+  const int tj = (j < n) ? t[j] : 0;
+  double v[16];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = tile[4 * g + u][jr];
-    typename S::T* o = out + (long)j * kp + kk;
-    for (int l = 0; l < nmod; ++l) {
-      double r[4];
+  for (int u = 0; u < 16; ++u) v[u] = rint(ldexp(tile[16 * g + u][jr ^ ((g & 3) << 3)], tj));
+  const long plane = plane_elems((long)np * kp, sizeof(typename SliceOf<MODE>::T));
+  typename SliceOf<MODE>::T* o = out + (long)j * kp + kk0;
+  for (int l = 0; l < nmod; ++l) {
+    double r[16];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) r[u] = residue(v[u], wide, cc, l);
-      *reinterpret_cast<typename S::Pack*>(o + l * plane) = S::pack(r);
-    }
+    for (int u = 0; u < 16; ++u) r[u] = residue<RM>(v[u], cc, l);
+    store16<MODE>(o + l * plane, r);
   }
 }
 
@@ -458,7 +542,7 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
   char* Bs = smem + 2 * RSTAGE;      // [2][BN rows][RBYTES]
 
   const int l = blockIdx.z;
-  const long a_plane = (long)mp * kp * ESZ, b_plane = (long)np * kp * ESZ;
+  const long a_plane = plane_elems((long)mp * kp * ESZ, 1), b_plane = plane_elems((long)np * kp * ESZ, 1);
   const char* Ag = Ares + l * a_plane;
   const char* Bg = Bres + l * b_plane;
   const float mf = (float)cc->m[l];
@@ -598,7 +682,7 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
   }
 
   // epilogue: symmetric residue mod m
-  const long cplane = (long)mp * np;
+  const long cplane = plane_elems((long)mp * np, sizeof(ResOut<MODE>));
   ResOut<MODE>* Cg = Cres + l * cplane;
   const int mi = (int)cc->m[l];
   const int half = (mi - 1) / 2;
@@ -630,73 +714,102 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
 // ------------------------------------------------------------------------------------------
 // device: CRT recombination and scaling
 // ------------------------------------------------------------------------------------------
-template <typename R>
-struct Quad;
-template <>
-struct Quad<int8_t> {
-  using V = char4;
-};
-template <>
-struct Quad<int16_t> {
-  using V = short4;
-};
-
-// 4 consecutive columns per thread (np is a multiple of 256); grid x covers np / 1024, y strides
+// 16 consecutive columns per thread (np is a multiple of 256): one 16-byte (int8) or two
+// (int16) loads per residue plane; NCH = number of CRT chunks.  A workgroup covers 4096 columns
+// of a row; the results go through LDS (one padding double per 16) so the stores to C are
+// 16 bytes per lane and contiguous across the wavefront.  Grid x covers np / 4096, y strides
 // over rows.
-template <typename R>
+__device__ __forceinline__ int crt_pos(int g) { return g + (g >> 4); }
+
+template <typename R, int NCH>
 __global__ __launch_bounds__(256) void crt_kernel(int m, int n, int mp, int np, int nmod,
                                                   const R* __restrict__ Cres,
                                                   const int* __restrict__ s, const int* __restrict__ t,
                                                   const CrtConsts* __restrict__ cc, double* __restrict__ C,
                                                   long ldc) {
-  const int j0 = (blockIdx.x * 256 + threadIdx.x) * 4;
-  if (j0 >= n) return;
-  const long plane = (long)mp * np;
-  const int nch = cc->nch;
-  int tj[4];
+  __shared__ double stage[4096 + 256];
+  const int jb = blockIdx.x * 4096;
+  const int j0 = jb + threadIdx.x * 16;
+  const bool active = j0 < n;
+  const long plane = plane_elems((long)mp * np, sizeof(R));
+  int tj[16];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) tj[u] = (j0 + u < n) ? t[j0 + u] : 0;
+  for (int u = 0; u < 16; ++u) tj[u] = (j0 + u < n) ? t[j0 + u] : 0;
   for (int i = blockIdx.y; i < m; i += gridDim.y) {
-    const R* src = Cres + (long)i * np + j0;
-    // S[c] = sum_l r_l w[c][l]: every term a multiple of chunk c's unit, below 2^53 units: exact
-    double S[kMaxChunks][4];
+    double out[16];
+    if (active) {
+      const R* src = Cres + (long)i * np + j0;
+      // S[c] = sum_l r_l w[c][l]: every term a multiple of chunk c's unit, below 2^53 units: exact
+      double S[NCH][16];
 #pragma unroll
-    for (int ch = 0; ch < kMaxChunks; ++ch)
+      for (int ch = 0; ch < NCH; ++ch)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) S[ch][u] = 0.0;
-    for (int l = 0; l < nmod; ++l) {
-      const auto q4 = *reinterpret_cast<const typename Quad<R>::V*>(src + l * plane);
-      const double r[4] = {(double)q4.x, (double)q4.y, (double)q4.z, (double)q4.w};
+        for (int u = 0; u < 16; ++u) S[ch][u] = 0.0;
+      // planes in groups of 8: a group's loads are issued before its first multiply-add
+      constexpr int WPL = 4 * (int)sizeof(R);  // 32-bit words per plane and thread
+      constexpr int GRP = 8;
+      for (int l0 = 0; l0 < nmod; l0 += GRP) {
+        int raw[GRP][WPL];
 #pragma unroll
-      for (int ch = 0; ch < kMaxChunks; ++ch) {
-        if (ch < nch) {
-          const double wl = cc->w[ch][l];
+        for (int g = 0; g < GRP; ++g)
+          if (l0 + g < nmod)
 #pragma unroll
-          for (int u = 0; u < 4; ++u) S[ch][u] = fma(r[u], wl, S[ch][u]);
+            for (int h = 0; h < WPL / 4; ++h) {
+              const int4 q = reinterpret_cast<const int4*>(src + (l0 + g) * plane)[h];
+              raw[g][4 * h] = q.x; raw[g][4 * h + 1] = q.y; raw[g][4 * h + 2] = q.z; raw[g][4 * h + 3] = q.w;
+            }
+#pragma unroll
+        for (int g = 0; g < GRP; ++g) {
+          if (l0 + g < nmod) {
+            double r[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+              r[u] = sizeof(R) == 1 ? (double)(int8_t)(raw[g][u / 4] >> (8 * (u % 4)))
+                                    : (double)(int16_t)(raw[g][u / 2] >> (16 * (u % 2)));
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+              const double wl = cc->w[ch][l0 + g];
+#pragma unroll
+              for (int u = 0; u < 16; ++u) S[ch][u] = fma(r[u], wl, S[ch][u]);
+            }
+          }
         }
       }
-    }
-    const int si = s[i];
+      const int si = s[i];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (j0 + u >= n) break;
-      // X = sum_c S[c] = C' + q M with |C'| <= M/4: q from the leading chunk, then the chunks of
-      // X - q M (each exact) summed with error-free transformations, rounded once
-      const double q = rint(S[0][u] * cc->Minv);
-      double hi = fma(-q, cc->M[0], S[0][u]);  // exact
-      double lo = 0.0;
+      for (int u = 0; u < 16; ++u) {
+        // X = sum_c S[c] = C' + q M with |C'| <= M/4: q from the leading chunk, then the chunks
+        // of X - q M (each exact) summed with error-free transformations, rounded once
+        const double q = rint(S[0][u] * cc->Minv);
+        double hi = fma(-q, cc->M[0], S[0][u]);  // exact
+        double lo = 0.0;
 #pragma unroll
-      for (int ch = 1; ch < kMaxChunks; ++ch) {
-        if (ch < nch) {
-          const double t = fma(-q, cc->M[ch], S[ch][u]);  // exact
-          const double sum = hi + t;                      // TwoSum(hi, t)
+        for (int ch = 1; ch < NCH; ++ch) {
+          const double tt = fma(-q, cc->M[ch], S[ch][u]);  // exact
+          const double sum = hi + tt;                      // TwoSum(hi, tt)
           const double bb = sum - hi;
-          lo += (hi - (sum - bb)) + (t - bb);
+          lo += (hi - (sum - bb)) + (tt - bb);
           hi = sum;
         }
+        out[u] = ldexp(hi + lo, -(si + tj[u]));
       }
-      const double Cp = hi + lo;
-      C[(long)i * ldc + j0 + u] = ldexp(Cp, -(si + tj[u]));
+    }
+    __syncthreads();  // the previous row's stores have read the stage
+    if (active)
+#pragma unroll
+      for (int u = 0; u < 16; ++u) stage[crt_pos(threadIdx.x * 16 + u)] = out[u];
+    __syncthreads();
+    double* dst = C + (long)i * ldc + jb;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int g = 2 * threadIdx.x + 512 * v;
+      const double a = stage[crt_pos(g)], b = stage[crt_pos(g + 1)];
+      if (jb + g + 1 < n && ((uintptr_t)(dst + g) & 15) == 0) {
+        *reinterpret_cast<double2*>(dst + g) = make_double2(a, b);
+      } else {
+        if (jb + g < n) dst[g] = a;
+        if (jb + g + 1 < n) dst[g + 1] = b;
+      }
     }
   }
 }
@@ -719,7 +832,7 @@ struct Fp64AccPlan {
 // ||A||^2 ||B||^2 / k, which is what makes them need more moduli.)  Sign-coherent operands make
 // ||C|| larger and the estimate conservative; a product that cancels to far below sqrt(CC) is
 // not covered by any a-priori choice.
-constexpr double kAutoTarget = 4e-15;
+constexpr double kAutoTarget = 8e-15;
 
 double auto_estimate(int P, int64_t, const double* st) {
   if (!(st[4] > 0.0)) return 0.0;  // a zero operand or no overlapping support: C = 0 exactly
@@ -772,57 +885,62 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   BZG_REQUIRE(A && B && C, BZG_ERR_INVALID_ARGUMENT, "fp64acc: null pointer");
   BZG_REQUIRE(moduli >= 0, BZG_ERR_INVALID_ARGUMENT, "fp64acc: moduli must be 0 (automatic) or 2..24");
   ProfScope prof_total(ctx, "fp64acc_total");
-  // ---- operand statistics (row / column maxima, sums of squares) ----
+  // ---- operand statistics: row maxima of A and column maxima of B (the scaling exponents);
+  // for the automatic count also the column sums of squares of A and the row sums of squares of
+  // B, in one pass over each operand ----
   const long ny = std::min<long>((k + 255) / 256, 16384);
-  const long nya = std::min<long>((m + 255) / 256, 16384);
-  const size_t stat_bytes = 64 + (size_t)m * 16 + (size_t)n * 8 + (size_t)ny * n * 8 + (size_t)nya * k * 8 +
-                            (size_t)k * 16 + (size_t)(n + k) * 8;
+  const long gya = std::min<long>((m + 63) / 64, 65535), gyb = std::min<long>((k + 63) / 64, 65535);
+  const long nxa = (k + 1023) / 1024, nxb = (n + 1023) / 1024;
+  const size_t stat_bytes = 64 + (size_t)m * 8 + (size_t)n * 8 + (size_t)k * 16 +
+                            (size_t)(nxa * m + gya * k + nxb * k + gyb * n) * 8;
   void* sws = nullptr;
   bzg_status st = workspace(ctx, 8, stat_bytes, &sws);
   if (st) return st;
   double* sout = static_cast<double*>(sws);
   double* rmax = sout + 8;
-  double* rn2 = rmax + m;
-  unsigned long long* cmaxB = reinterpret_cast<unsigned long long*>(rn2 + m);
-  double* cn2 = reinterpret_cast<double*>(cmaxB + n);
-  double* acn2 = cn2 + ny * n;     // partial column sums of squares of A
-  double* bmax = acn2 + nya * k;   // row maxima of B (unused) and row sums of squares
-  double* brn2 = bmax + k;
-  double* bcn2 = brn2 + k;         // column sums of squares of B and of A
-  double* acol = bcn2 + n;
-  hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)((m * 64 + 255) / 256)), dim3(256), 0, ctx->stream, (int)m,
-                     (int)k, A, (long)lda, rmax, rn2);
-  BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
-                     cmaxB);
-  BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((n + 63) / 64), (unsigned)ny), dim3(256), 0, ctx->stream,
-                     (int)k, (int)n, B, (long)ldb, cmaxB, cn2);
-  BZG_LAUNCH_CHECK();
+  unsigned long long* cmaxB = reinterpret_cast<unsigned long long*>(rmax + m);
+  double* acol = reinterpret_cast<double*>(cmaxB + n);   // column sums of squares of A (k)
+  double* brn2 = acol + k;       // row sums of squares of B (k)
+  double* pa_row = brn2 + k;     // partials of the fused passes
+  double* pa_col = pa_row + nxa * m;
+  double* pb_row = pa_col + gya * k;
+  double* pb_col = pb_row + nxb * k;
   Fp64AccPlan pl;
   if (moduli == 0) {
-    // automatic count: column norms of A and row norms of B complete the statistics, which
-    // come back to the host (one stream synchronisation)
-    hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((k + 63) / 64), (unsigned)nya), dim3(256), 0, ctx->stream,
-                       (int)m, (int)k, A, (long)lda, (unsigned long long*)nullptr, acn2);
+    // automatic count: the statistics come back to the host (one stream synchronisation)
+    hipLaunchKernelGGL(stats_pass_kernel<true>, dim3((unsigned)nxa, (unsigned)gya), dim3(256), 0, ctx->stream,
+                       (int)m, (int)k, A, (long)lda, pa_row, pa_col);
     BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)((k * 64 + 255) / 256)), dim3(256), 0, ctx->stream, (int)k,
-                       (int)n, B, (long)ldb, bmax, brn2);
+    hipLaunchKernelGGL(stats_pass_kernel<false>, dim3((unsigned)nxb, (unsigned)gyb), dim3(256), 0, ctx->stream,
+                       (int)k, (int)n, B, (long)ldb, pb_row, pb_col);
     BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)ny,
-                       (int)n, cn2, bcn2);
+    hipLaunchKernelGGL(reduce_partials_kernel<true>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)nxa, (int)m, pa_row, rmax);
+    hipLaunchKernelGGL(reduce_partials_kernel<false>, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)gya, (int)k, pa_col, acol);
+    hipLaunchKernelGGL(reduce_partials_kernel<false>, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)nxb, (int)k, pb_row, brn2);
+    // column maxima of B as FP64 bit patterns (non-negative doubles order like their bits)
+    hipLaunchKernelGGL(reduce_partials_kernel<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)gyb, (int)n, pb_col, reinterpret_cast<double*>(cmaxB));
     BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, ctx->stream, (int)nya,
-                       (int)k, acn2, acol);
-    BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(fp64acc_stats_kernel, dim3(1), dim3(1024), 0, ctx->stream, (int)m, (int)n, (int)k, rmax, rn2,
-                       cmaxB, bcn2, acol, brn2, sout);
+    hipLaunchKernelGGL(fp64acc_stats_kernel, dim3(1), dim3(1024), 0, ctx->stream, (int)m, (int)n, (int)k, rmax,
+                       cmaxB, acol, brn2, sout);
     BZG_LAUNCH_CHECK();
     double hs[5];
     BZG_HIP_CHECK(hipMemcpyAsync(hs, sout, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
     BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     st = plan_fp64acc(k, 0, mode, pl, hs);
   } else {
+    hipLaunchKernelGGL(row_max_kernel, dim3((unsigned)((m * 64 + 255) / 256)), dim3(256), 0, ctx->stream, (int)m,
+                       (int)k, A, (long)lda, rmax);
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
+                       cmaxB);
+    BZG_LAUNCH_CHECK();
+    hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((n + 63) / 64), (unsigned)ny), dim3(256), 0, ctx->stream,
+                       (int)k, (int)n, B, (long)ldb, cmaxB);
+    BZG_LAUNCH_CHECK();
     st = plan_fp64acc(k, moduli, mode, pl);
   }
   if (st) return st;
@@ -833,9 +951,9 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   const int kq = RBYTES / esz;  // K elements per stage
   const long mp = (m + RT - 1) / RT * RT, np = (n + RT - 1) / RT * RT;
   const long kp = (k + kq - 1) / kq * kq;
-  const size_t a_bytes = (size_t)pl.nmod * mp * kp * esz;
-  const size_t b_bytes = (size_t)pl.nmod * np * kp * esz;
-  const size_t c_bytes = (size_t)pl.nmod * mp * np * (mode == 0 ? sizeof(int16_t) : sizeof(int8_t));
+  const size_t a_bytes = (size_t)pl.nmod * plane_elems(mp * kp * esz, 1);
+  const size_t b_bytes = (size_t)pl.nmod * plane_elems(np * kp * esz, 1);
+  const size_t c_bytes = (size_t)pl.nmod * plane_elems(mp * np * (mode == 0 ? 2 : 1), 1);
   const size_t misc_bytes = 4096 + (size_t)(mp + np) * sizeof(int);
   const size_t total = a_bytes + b_bytes + c_bytes + misc_bytes;
   void* ws = nullptr;
@@ -857,24 +975,21 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
                      cmaxB, pl.P, tB);
   BZG_LAUNCH_CHECK();
-  const int wide = pl.P > 56 ? 1 : 0;
-  const dim3 ga((unsigned)((kp + 1023) / 1024), (unsigned)std::min<long>(mp, 16384));
-  const dim3 gb((unsigned)(kp / 64), (unsigned)(np / 64));
+  const int rm = pl.P > 56 ? 2 : (pl.P > 50 ? 1 : 0);
+  const dim3 ga((unsigned)((kp + 4095) / 4096), (unsigned)std::min<long>(mp, 16384));
+  const dim3 gb((unsigned)(np / 32), (unsigned)((kp + 127) / 128));
+#define BZG_RES(MD, RMV, T)                                                                          \
+  hipLaunchKernelGGL((residues_a_kernel<MD, RMV>), ga, dim3(256), 0, ctx->stream, (int)m, (int)k, A, (long)lda, \
+                     sA, (int)mp, (int)kp, pl.nmod, dcc, reinterpret_cast<T*>(Ares));                  \
+  hipLaunchKernelGGL((residues_b_kernel<MD, RMV>), gb, dim3(256), 0, ctx->stream, (int)k, (int)n, B, (long)ldb, \
+                     tB, (int)np, (int)kp, pl.nmod, dcc, reinterpret_cast<T*>(Bres))
   if (mode == 0) {
-    hipLaunchKernelGGL(residues_a_kernel<0>, ga, dim3(256), 0, ctx->stream, (int)m, (int)k, A, (long)lda, sA,
-                       (int)mp, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<__bf16*>(Ares));
-    BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(residues_b_kernel<0>, gb, dim3(256), 0, ctx->stream, (int)k, (int)n, B, (long)ldb, tB,
-                       (int)np, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<__bf16*>(Bres));
-    BZG_LAUNCH_CHECK();
+    if (rm == 0) { BZG_RES(0, 0, __bf16); } else if (rm == 1) { BZG_RES(0, 1, __bf16); } else { BZG_RES(0, 2, __bf16); }
   } else {
-    hipLaunchKernelGGL(residues_a_kernel<1>, ga, dim3(256), 0, ctx->stream, (int)m, (int)k, A, (long)lda, sA,
-                       (int)mp, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<int8_t*>(Ares));
-    BZG_LAUNCH_CHECK();
-    hipLaunchKernelGGL(residues_b_kernel<1>, gb, dim3(256), 0, ctx->stream, (int)k, (int)n, B, (long)ldb, tB,
-                       (int)np, (int)kp, pl.nmod, wide, dcc, reinterpret_cast<int8_t*>(Bres));
-    BZG_LAUNCH_CHECK();
+    if (rm == 0) { BZG_RES(1, 0, int8_t); } else if (rm == 1) { BZG_RES(1, 1, int8_t); } else { BZG_RES(1, 2, int8_t); }
   }
+#undef BZG_RES
+  BZG_LAUNCH_CHECK();
   const size_t lds = 4 * RSTAGE;
   static bool attr_set[2] = {false, false};
   if (!attr_set[mode]) {
@@ -898,13 +1013,27 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                        reinterpret_cast<int8_t*>(Cres), dcc);
   BZG_LAUNCH_CHECK();
   }
-  const dim3 gc((unsigned)((n + 1023) / 1024), (unsigned)std::min<long>(m, 16384));
-  if (mode == 0)
-    hipLaunchKernelGGL(crt_kernel<int16_t>, gc, dim3(256), 0, ctx->stream, (int)m, (int)n, (int)mp, (int)np,
-                       pl.nmod, reinterpret_cast<const int16_t*>(Cres), sA, tB, dcc, C, (long)ldc);
-  else
-    hipLaunchKernelGGL(crt_kernel<int8_t>, gc, dim3(256), 0, ctx->stream, (int)m, (int)n, (int)mp, (int)np,
-                       pl.nmod, reinterpret_cast<const int8_t*>(Cres), sA, tB, dcc, C, (long)ldc);
+  const dim3 gc((unsigned)((n + 4095) / 4096), (unsigned)std::min<long>(m, 16384));
+  const int nch = pl.host.nch;
+  BZG_REQUIRE(nch >= 2 && nch <= kMaxChunks, BZG_ERR_UNSUPPORTED, "fp64acc: CRT chunk count out of range");
+#define BZG_CRT(R, NC)                                                                             \
+  hipLaunchKernelGGL((crt_kernel<R, NC>), gc, dim3(256), 0, ctx->stream, (int)m, (int)n, (int)mp, (int)np, \
+                     pl.nmod, reinterpret_cast<const R*>(Cres), sA, tB, dcc, C, (long)ldc)
+#define BZG_CRT_ALL(R)                     \
+  switch (nch) {                           \
+    case 2: BZG_CRT(R, 2); break;          \
+    case 3: BZG_CRT(R, 3); break;          \
+    case 4: BZG_CRT(R, 4); break;          \
+    case 5: BZG_CRT(R, 5); break;          \
+    default: BZG_CRT(R, 6); break;         \
+  }
+  if (mode == 0) {
+    BZG_CRT_ALL(int16_t)
+  } else {
+    BZG_CRT_ALL(int8_t)
+  }
+#undef BZG_CRT_ALL
+#undef BZG_CRT
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }

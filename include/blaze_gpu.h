@@ -225,11 +225,12 @@ bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_h
  * Same contract as bzg_dgemm with alpha = 1, beta = 0.  `moduli` selects the number of
  * coprime moduli (slice GEMMs), 2..24.  0 = automatic: row / column maxima and sums of squares
  * of A and B give an estimate of the relative Frobenius error for each count,
- * 2^-P sqrt(k/12 (sum_i D_i^2 / ||A||^2 + sum_j E_j^2 / ||B||^2)) with D_i (E_j) the power of
- * two above row i's (column j's) largest |entry|, and the smallest count whose estimate is
- * <= 4e-15 is used (24 if none is); this reads 4 numbers back to the host (one stream
- * synchronisation), so capture a graph with an explicit count.  Workspace is allocated from
- * the context on first use and cached. */
+ * 2^-P sqrt((sum_i D_i^2 ||B||_F^2 + sum_j E_j^2 ||A||_F^2) / (12 sum_q ||A_:q||^2 ||B_q:||^2))
+ * with P the bits per operand the count allows and D_i (E_j) the power of two above row i's
+ * (column j's) largest |entry|; the smallest count whose estimate is <= 8e-15 is used (24 if
+ * none is).  This reads 5 numbers back to the host (one stream synchronisation), so capture a
+ * graph with an explicit count.  Workspace is allocated from the context on first use and
+ * cached. */
 bzg_status bzg_dgemm_fp64acc(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                              const double* A, int64_t lda, const double* B, int64_t ldb,
                              double* C, int64_t ldc, int32_t moduli);

@@ -104,7 +104,7 @@ def test_more_moduli_tighten_error_for_wide_exponents():
 @pytest.mark.parametrize("sigma", [0, 4, 10])
 def test_automatic_moduli_meet_1e14_per_sigma(fn, sigma):
     # entries u 2^e, e = round(N(0, sigma)): the automatic count (operand statistics -> error
-    # estimate <= 4e-15) keeps the relative Frobenius error below 1e-14 for every sigma, and
+    # estimate <= 8e-15) keeps the relative Frobenius error below 1e-14 for every sigma, and
     # the estimate is an upper bound of the measured error within a factor of a few
     rng = np.random.default_rng(60 + sigma)
     m, n, k = 512, 384, 4096
@@ -117,7 +117,7 @@ def test_automatic_moduli_meet_1e14_per_sigma(fn, sigma):
     hi, lo = oracle.gemm_dd_entries(A, B, rows, cols)
     err = np.sqrt(np.sum((Cg.ravel() - hi - lo) ** 2) / np.sum(hi ** 2))
     assert err <= 1e-14, (fn, sigma, nm, P, est, err)
-    assert 0 < est <= 4e-15 and 2 <= nm <= 24
+    assert 0 < est <= 8e-15 and 2 <= nm <= 24
 
 
 def last_plan():
