@@ -37,6 +37,7 @@ void set_error(const std::string& msg);
 struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
+  unsigned generation = 0;  // bumped on every (re)allocation: contents cached in it are gone
 };
 
 }  // namespace bzg
@@ -52,6 +53,9 @@ struct bzg_context {
   // plan of the last FP64-accurate GEMM on this context (bzg_fp64acc_last_plan)
   int fp64acc_moduli = 0, fp64acc_bits = 0;
   double fp64acc_est = 0.0;
+  double* host_stats = nullptr;       // pinned readback of the split GEMM's operand statistics
+  unsigned fp64acc_cc_gen = 0;       // workspace generation holding the split GEMM's CRT constants
+  int fp64acc_cc_key = -1;            // ... and for which (slice type, moduli count); -1: none
   bool profile = false;
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
 };

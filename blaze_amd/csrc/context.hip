@@ -28,6 +28,7 @@ bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out) {
       return BZG_ERR_OUT_OF_MEMORY;
     }
     w.bytes = bytes;
+    ++w.generation;
   }
   *out = w.ptr;
   return BZG_OK;
@@ -70,6 +71,7 @@ bzg_status bzg_destroy(bzg_context* ctx) {
     }
   for (auto& w : ctx->ws)
     if (w.ptr) hipFree(w.ptr);
+  if (ctx->host_stats) hipHostFree(ctx->host_stats);
   if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return BZG_OK;

@@ -23,6 +23,8 @@
 // Error: |C - AB| <= 2^-P * (row max |A_i|)(col max |B_j|) * k  +  one FP64 rounding.
 #include "common.hpp"
 
+#include <mutex>
+
 #include <cmath>
 #include <type_traits>
 #include <cstring>
@@ -189,13 +191,11 @@ __global__ __launch_bounds__(256) void row_max_kernel(int m, int k, const double
   if (lane == 0) rmax[wave] = mx;
 }
 
-// s_i = P - 1 - ilogb(rmax_i)  (0 for zero rows): |rint(a_ij 2^s_i)| <= 2^P.
-__global__ __launch_bounds__(256) void row_scale_kernel(int m, const double* __restrict__ rmax, int P,
-                                                        int* __restrict__ s) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const double mx = rmax[i];
-  s[i] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
+// Scaling exponents: s_i = P - 1 - ilogb(max_j |a_ij|) (0 for a zero row), so that
+// |rint(a_ij 2^s_i)| <= 2^P; t_j likewise from the column maxima of B (FP64 bit patterns).
+__device__ __forceinline__ int row_exp(double mx, int P) { return (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0; }
+__device__ __forceinline__ int col_exp(unsigned long long bits, int P) {
+  return row_exp(__longlong_as_double((long long)bits), P);
 }
 
 // Column maxima of |B| (k x n, ldb) as FP64 bit patterns (non-negative doubles order like
@@ -343,15 +343,6 @@ __global__ __launch_bounds__(256) void zero_u64_kernel(int n, unsigned long long
   if (i < n) p[i] = 0ull;
 }
 
-// t_j = P - 1 - ilogb(max_i |b_ij|)  (0 for zero columns).
-__global__ __launch_bounds__(256) void col_scale_kernel(int n, const unsigned long long* __restrict__ cmax,
-                                                        int P, int* __restrict__ t) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const double mx = __longlong_as_double((long long)cmax[j]);
-  t[j] = (mx > 0.0) ? (P - 1 - ilogb(mx)) : 0;
-}
-
 // Symmetric residue of the integer-valued double v modulo m (|result| <= (m-1)/2).
 __device__ __forceinline__ double sym_mod(double v, double m, double minv) {
   double r = fma(-m, rint(v * minv), v);  // exact: the true value is a small integer
@@ -423,8 +414,8 @@ __device__ __forceinline__ void store16(typename SliceOf<MODE>::T* __restrict__ 
 // contiguous per plane.  Grid: x covers kp in 4096-element chunks, y strides over rows.
 template <int MODE, int RM>
 __global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const double* __restrict__ A,
-                                                         long lda, const int* __restrict__ s, int mp,
-                                                         int kp, int nmod,
+                                                         long lda, const double* __restrict__ rmax, int P,
+                                                         int* __restrict__ s_out, int mp, int kp, int nmod,
                                                          const CrtConsts* __restrict__ cc,
                                                          typename SliceOf<MODE>::T* __restrict__ out) {
   const int k0 = (blockIdx.x * 256 + threadIdx.x) * 16;
@@ -444,7 +435,8 @@ __global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const dou
 #pragma unroll
       for (int u = 0; u < 16; ++u) v[u] = (i < m && k0 + u < k) ? src[u] : 0.0;
     }
-    const int si = i < m ? s[i] : 0;
+    const int si = i < m ? row_exp(rmax[i], P) : 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && i < m) s_out[i] = si;  // for the CRT
 #pragma unroll
     for (int u = 0; u < 16; ++u) v[u] = rint(ldexp(v[u], si));
     typename SliceOf<MODE>::T* o = out + (long)i * kp + k0;
@@ -458,7 +450,7 @@ __global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const dou
 }
 
 // B residues, transposed: Bres[l][j][kk] (n rows padded to np, k padded to kp).  A workgroup
-// transposes a 128 (kk) x 32 (j) tile through LDS: reads along j (256 bytes per row), then 16
+// transposes a 128 (kk) x 32 (j) tile through LDS: 16-byte reads along j (256 bytes per row), then 16
 // consecutive kk per thread and one 16-byte (INT8) or two (BF16) stores per modulus (128 bytes
 // contiguous per j row).  LDS column c of row r is stored at c ^ (((r >> 4) & 3) << 3), which
 // spreads both access patterns over the banks.  Grid x runs over the j blocks, so neighbouring
@@ -466,26 +458,39 @@ __global__ __launch_bounds__(256) void residues_a_kernel(int m, int k, const dou
 // blocks first at 8192^2).
 template <int MODE, int RM>
 __global__ __launch_bounds__(256) void residues_b_kernel(int k, int n, const double* __restrict__ B,
-                                                         long ldb, const int* __restrict__ t, int np,
-                                                         int kp, int nmod,
+                                                         long ldb, const unsigned long long* __restrict__ cmax,
+                                                         int P, int* __restrict__ t_out, int np, int kp, int nmod,
                                                          const CrtConsts* __restrict__ cc,
                                                          typename SliceOf<MODE>::T* __restrict__ out) {
-  __shared__ double tile[128][32];
+  __shared__ __attribute__((aligned(16))) double tile[128][32];
   const int kb = blockIdx.y * 128, jb = blockIdx.x * 32;
   const int tid = threadIdx.x;
   {
-    const int jr = tid & 31, j = jb + jr;
+    // 16-byte loads: lane pairs of columns (2 jp, 2 jp + 1), 16 rows per pass
+    const int jp = tid & 15, j = jb + 2 * jp;
+    const bool vec = ((((uintptr_t)B) | ((uintptr_t)ldb * 8)) & 15) == 0 && j + 1 < n;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int r = (tid >> 5) + 8 * i, kk = kb + r;
-      tile[r][jr ^ (((r >> 4) & 3) << 3)] = (kk < k && j < n) ? B[(long)kk * ldb + j] : 0.0;
+    for (int i = 0; i < 8; ++i) {
+      const int r = (tid >> 4) + 16 * i, kk = kb + r;
+      double2 v = make_double2(0.0, 0.0);
+      if (kk < k) {
+        const double* src = B + (long)kk * ldb + j;
+        if (vec) {
+          v = *reinterpret_cast<const double2*>(src);
+        } else {
+          if (j < n) v.x = src[0];
+          if (j + 1 < n) v.y = src[1];
+        }
+      }
+      *reinterpret_cast<double2*>(&tile[r][(2 * jp) ^ (((r >> 4) & 3) << 3)]) = v;
     }
   }
   __syncthreads();
   const int jr = tid >> 3, g = tid & 7;
   const int j = jb + jr, kk0 = kb + 16 * g;
   if (j >= np || kk0 >= kp) return;
-  const int tj = (j < n) ? t[j] : 0;
+  const int tj = (j < n) ? col_exp(cmax[j], P) : 0;
+  if (blockIdx.y == 0 && g == 0 && j < n) t_out[j] = tj;  // for the CRT
   double v[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) v[u] = rint(ldexp(tile[16 * g + u][jr ^ ((g & 3) << 3)], tj));
@@ -714,11 +719,14 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
 // ------------------------------------------------------------------------------------------
 // device: CRT recombination and scaling
 // ------------------------------------------------------------------------------------------
-// 16 consecutive columns per thread (np is a multiple of 256): one 16-byte (int8) or two
-// (int16) loads per residue plane; NCH = number of CRT chunks.  A workgroup covers 4096 columns
-// of a row; the results go through LDS (one padding double per 16) so the stores to C are
-// 16 bytes per lane and contiguous across the wavefront.  Grid x covers np / 4096, y strides
-// over rows.
+// Columns per thread of crt_kernel: 8 keeps it at 111 VGPRs (4 wavefronts per SIMD); 16 needed
+// 179 and measured 1.16x slower at 8192^2.
+constexpr int kCrtCols = 8;
+// CPT consecutive columns per thread (np is a multiple of 256): per residue plane one load of
+// CPT residues (16 bytes for CPT = 16 int8); NCH = number of CRT chunks.  A workgroup covers
+// 256 * CPT columns of a row; the results go through LDS (one padding double per 16) so the
+// stores to C are 16 bytes per lane and contiguous across the wavefront.  Grid x covers
+// np / (256 CPT), y strides over rows.
 __device__ __forceinline__ int crt_pos(int g) { return g + (g >> 4); }
 
 template <typename R, int NCH>
@@ -727,57 +735,70 @@ __global__ __launch_bounds__(256) void crt_kernel(int m, int n, int mp, int np, 
                                                   const int* __restrict__ s, const int* __restrict__ t,
                                                   const CrtConsts* __restrict__ cc, double* __restrict__ C,
                                                   long ldc) {
-  __shared__ double stage[4096 + 256];
-  const int jb = blockIdx.x * 4096;
-  const int j0 = jb + threadIdx.x * 16;
+  constexpr int CPT = kCrtCols;
+  constexpr int SPAN = 256 * CPT;
+  constexpr int WPL = CPT * (int)sizeof(R) / 4;  // 32-bit words per plane and thread
+  constexpr int EPW = 4 / (int)sizeof(R);        // residues per word
+  __shared__ double stage[SPAN + SPAN / 16];
+  const int jb = blockIdx.x * SPAN;
+  const int j0 = jb + threadIdx.x * CPT;
   const bool active = j0 < n;
   const long plane = plane_elems((long)mp * np, sizeof(R));
-  int tj[16];
+  int tj[CPT];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) tj[u] = (j0 + u < n) ? t[j0 + u] : 0;
+  for (int u = 0; u < CPT; ++u) tj[u] = (j0 + u < n) ? t[j0 + u] : 0;
   for (int i = blockIdx.y; i < m; i += gridDim.y) {
-    double out[16];
+    double out[CPT];
     if (active) {
       const R* src = Cres + (long)i * np + j0;
       // S[c] = sum_l r_l w[c][l]: every term a multiple of chunk c's unit, below 2^53 units: exact
-      double S[NCH][16];
+      double S[NCH][CPT];
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch)
 #pragma unroll
-        for (int u = 0; u < 16; ++u) S[ch][u] = 0.0;
+        for (int u = 0; u < CPT; ++u) S[ch][u] = 0.0;
       // planes in groups of 8: a group's loads are issued before its first multiply-add
-      constexpr int WPL = 4 * (int)sizeof(R);  // 32-bit words per plane and thread
       constexpr int GRP = 8;
       for (int l0 = 0; l0 < nmod; l0 += GRP) {
         int raw[GRP][WPL];
 #pragma unroll
         for (int g = 0; g < GRP; ++g)
-          if (l0 + g < nmod)
+          if (l0 + g < nmod) {
+            const R* q = src + (l0 + g) * plane;
+            if constexpr (WPL == 4) {
+              const int4 v = *reinterpret_cast<const int4*>(q);
+              raw[g][0] = v.x; raw[g][1] = v.y; raw[g][2] = v.z; raw[g][3] = v.w;
+            } else if constexpr (WPL == 2) {
+              const int2 v = *reinterpret_cast<const int2*>(q);
+              raw[g][0] = v.x; raw[g][1] = v.y;
+            } else {
 #pragma unroll
-            for (int h = 0; h < WPL / 4; ++h) {
-              const int4 q = reinterpret_cast<const int4*>(src + (l0 + g) * plane)[h];
-              raw[g][4 * h] = q.x; raw[g][4 * h + 1] = q.y; raw[g][4 * h + 2] = q.z; raw[g][4 * h + 3] = q.w;
+              for (int h = 0; h < WPL / 4; ++h) {
+                const int4 v = reinterpret_cast<const int4*>(q)[h];
+                raw[g][4 * h] = v.x; raw[g][4 * h + 1] = v.y; raw[g][4 * h + 2] = v.z; raw[g][4 * h + 3] = v.w;
+              }
             }
+          }
 #pragma unroll
         for (int g = 0; g < GRP; ++g) {
           if (l0 + g < nmod) {
-            double r[16];
+            double r[CPT];
 #pragma unroll
-            for (int u = 0; u < 16; ++u)
-              r[u] = sizeof(R) == 1 ? (double)(int8_t)(raw[g][u / 4] >> (8 * (u % 4)))
-                                    : (double)(int16_t)(raw[g][u / 2] >> (16 * (u % 2)));
+            for (int u = 0; u < CPT; ++u)
+              r[u] = sizeof(R) == 1 ? (double)(int8_t)(raw[g][u / EPW] >> (8 * (u % EPW)))
+                                    : (double)(int16_t)(raw[g][u / EPW] >> (16 * (u % EPW)));
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) {
               const double wl = cc->w[ch][l0 + g];
 #pragma unroll
-              for (int u = 0; u < 16; ++u) S[ch][u] = fma(r[u], wl, S[ch][u]);
+              for (int u = 0; u < CPT; ++u) S[ch][u] = fma(r[u], wl, S[ch][u]);
             }
           }
         }
       }
       const int si = s[i];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
+      for (int u = 0; u < CPT; ++u) {
         // X = sum_c S[c] = C' + q M with |C'| <= M/4: q from the leading chunk, then the chunks
         // of X - q M (each exact) summed with error-free transformations, rounded once
         const double q = rint(S[0][u] * cc->Minv);
@@ -797,11 +818,11 @@ __global__ __launch_bounds__(256) void crt_kernel(int m, int n, int mp, int np, 
     __syncthreads();  // the previous row's stores have read the stage
     if (active)
 #pragma unroll
-      for (int u = 0; u < 16; ++u) stage[crt_pos(threadIdx.x * 16 + u)] = out[u];
+      for (int u = 0; u < CPT; ++u) stage[crt_pos(threadIdx.x * CPT + u)] = out[u];
     __syncthreads();
     double* dst = C + (long)i * ldc + jb;
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
+    for (int v = 0; v < CPT / 2; ++v) {
       const int g = 2 * threadIdx.x + 512 * v;
       const double a = stage[crt_pos(g)], b = stage[crt_pos(g + 1)];
       if (jb + g + 1 < n && ((uintptr_t)(dst + g) & 15) == 0) {
@@ -865,7 +886,16 @@ bzg_status plan_fp64acc(int64_t k, int32_t moduli, int mode, Fp64AccPlan& pl, co
   BZG_REQUIRE(pl.P >= 1, BZG_ERR_UNSUPPORTED, "fp64acc: too few moduli for this k");
   pl.P = std::min(pl.P, 104);  // |A'| <= 2^104 keeps the residue split exact
   BZG_REQUIRE(mode == 0 || k <= (1LL << 17), BZG_ERR_UNSUPPORTED, "fp64acc int8: k > 2^17");
-  build_crt(ms, pl.host, mode == 1 ? 40 : 39);
+  // the CRT constants depend only on the moduli: built once per (slice type, count)
+  static std::mutex mu;
+  static CrtConsts cache[2][kMaxModuli + 1];
+  static bool have[2][kMaxModuli + 1] = {};
+  std::lock_guard<std::mutex> lock(mu);
+  if (!have[mode][nm]) {
+    build_crt(ms, cache[mode][nm], mode == 1 ? 40 : 39);
+    have[mode][nm] = true;
+  }
+  pl.host = cache[mode][nm];
   return BZG_OK;
 }
 
@@ -927,8 +957,9 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
     hipLaunchKernelGGL(fp64acc_stats_kernel, dim3(1), dim3(1024), 0, ctx->stream, (int)m, (int)n, (int)k, rmax,
                        cmaxB, acol, brn2, sout);
     BZG_LAUNCH_CHECK();
-    double hs[5];
-    BZG_HIP_CHECK(hipMemcpyAsync(hs, sout, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+    if (!ctx->host_stats) BZG_HIP_CHECK(hipHostMalloc((void**)&ctx->host_stats, 8 * sizeof(double)));
+    double* hs = ctx->host_stats;  // pinned: the readback is one small DMA
+    BZG_HIP_CHECK(hipMemcpyAsync(hs, sout, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     st = plan_fp64acc(k, 0, mode, pl, hs);
   } else {
@@ -954,35 +985,34 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   const size_t a_bytes = (size_t)pl.nmod * plane_elems(mp * kp * esz, 1);
   const size_t b_bytes = (size_t)pl.nmod * plane_elems(np * kp * esz, 1);
   const size_t c_bytes = (size_t)pl.nmod * plane_elems(mp * np * (mode == 0 ? 2 : 1), 1);
-  const size_t misc_bytes = 4096 + (size_t)(mp + np) * sizeof(int);
+  const size_t misc_bytes = 4096 + ((size_t)(mp + np) * sizeof(int) + 255) / 256 * 256;
   const size_t total = a_bytes + b_bytes + c_bytes + misc_bytes;
   void* ws = nullptr;
   st = workspace(ctx, 4, total, &ws);
   if (st) return st;
   char* base = static_cast<char*>(ws);
   CrtConsts* dcc = reinterpret_cast<CrtConsts*>(base);
-  int* sA = reinterpret_cast<int*>(base + 4096);
-  int* tB = sA + mp;
+  int* sA = reinterpret_cast<int*>(base + 4096);  // row exponents of A, written by residues_a
+  int* tB = sA + mp;                              // column exponents of B, by residues_b
   char* Ares = base + misc_bytes;
   char* Bres = Ares + a_bytes;
   char* Cres = Bres + b_bytes;
   static_assert(sizeof(CrtConsts) <= 4096, "CRT constants block");
-  BZG_HIP_CHECK(hipMemcpyAsync(dcc, &pl.host, sizeof(CrtConsts), hipMemcpyHostToDevice, ctx->stream));
+  // the constants block keeps the last plan's constants: upload only when they change
+  if (ctx->fp64acc_cc_gen != ctx->ws[4].generation || ctx->fp64acc_cc_key != mode * 64 + pl.nmod) {
+    BZG_HIP_CHECK(hipMemcpyAsync(dcc, &pl.host, sizeof(CrtConsts), hipMemcpyHostToDevice, ctx->stream));
+    ctx->fp64acc_cc_gen = ctx->ws[4].generation;
+    ctx->fp64acc_cc_key = mode * 64 + pl.nmod;
+  }
 
-  hipLaunchKernelGGL(row_scale_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, (int)m, rmax,
-                     pl.P, sA);
-  BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
-                     cmaxB, pl.P, tB);
-  BZG_LAUNCH_CHECK();
   const int rm = pl.P > 56 ? 2 : (pl.P > 50 ? 1 : 0);
   const dim3 ga((unsigned)((kp + 4095) / 4096), (unsigned)std::min<long>(mp, 16384));
   const dim3 gb((unsigned)(np / 32), (unsigned)((kp + 127) / 128));
 #define BZG_RES(MD, RMV, T)                                                                          \
   hipLaunchKernelGGL((residues_a_kernel<MD, RMV>), ga, dim3(256), 0, ctx->stream, (int)m, (int)k, A, (long)lda, \
-                     sA, (int)mp, (int)kp, pl.nmod, dcc, reinterpret_cast<T*>(Ares));                  \
+                     rmax, pl.P, sA, (int)mp, (int)kp, pl.nmod, dcc, reinterpret_cast<T*>(Ares));     \
   hipLaunchKernelGGL((residues_b_kernel<MD, RMV>), gb, dim3(256), 0, ctx->stream, (int)k, (int)n, B, (long)ldb, \
-                     tB, (int)np, (int)kp, pl.nmod, dcc, reinterpret_cast<T*>(Bres))
+                     cmaxB, pl.P, tB, (int)np, (int)kp, pl.nmod, dcc, reinterpret_cast<T*>(Bres))
   if (mode == 0) {
     if (rm == 0) { BZG_RES(0, 0, __bf16); } else if (rm == 1) { BZG_RES(0, 1, __bf16); } else { BZG_RES(0, 2, __bf16); }
   } else {
@@ -1013,7 +1043,7 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                        reinterpret_cast<int8_t*>(Cres), dcc);
   BZG_LAUNCH_CHECK();
   }
-  const dim3 gc((unsigned)((n + 4095) / 4096), (unsigned)std::min<long>(m, 16384));
+  const dim3 gc((unsigned)((n + 256 * kCrtCols - 1) / (256 * kCrtCols)), (unsigned)std::min<long>(m, 16384));
   const int nch = pl.host.nch;
   BZG_REQUIRE(nch >= 2 && nch <= kMaxChunks, BZG_ERR_UNSUPPORTED, "fp64acc: CRT chunk count out of range");
 #define BZG_CRT(R, NC)                                                                             \
