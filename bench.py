@@ -409,7 +409,9 @@ def gemm_leg(b, rec, args):
                      "variant": head, "slices": slices,
                      "slice_arith": ("INT8 x INT8 -> INT32 residues mod m<=255" if head.endswith("int8")
                                      else "BF16 x BF16 -> FP32 residues mod m<=511"),
-                     "recombination": "FP64 CRT (exact 36-bit-chunk partial sums)",
+                     "recombination": ("FP64 CRT, weights split in " + ("40" if head.endswith("int8") else "39")
+                                       + "-bit chunks (exact partial sums), one final rounding"),
+                     "moduli_choice": "automatic: smallest count with estimated rel. Frobenius error <= 8e-15",
                      "rel_frob_err": res[head]["rel_frob_err"]}
     rec["native_fp64"] = res["native_fp64"]
     rec["fp64acc_vs_native"] = res[head]["vs_native"]
