@@ -15,7 +15,8 @@
 //   * Admitted work is spread over `streams` HIP streams (default priority), one library
 //     context per stream, so requests that arrive while others run overlap on the device.
 //   * Solve requests admitted together run as ONE batched launch (bzg_dsgesv_batched: FP32 LU +
-//     FP64 refinement, FP64 fallback), longest system first.
+//     FP64 refinement, FP64 fallback), longest system first.  Small GEMM requests (up to 2^31
+//     flops) waiting together likewise run as one grouped launch (bzg_dgemm_grouped).
 //   * Completion is detected with HIP events (no stream synchronisation in the loop).
 // Every kernel computes each request's result independently of what else is in flight (GEMM
 // tiles depend only on the shape, SpMV on the plan, a solve on its own system), so a result
