@@ -344,13 +344,44 @@ class Bench:
         t = self.timed(one, steps, 2)
         ms, launches = self.ctx.profile_read("csrmv")
         self.ctx.profile(False)
+        # back-to-back device throughput: 8 rounds over the rotated copies captured in one HIP
+        # graph and replayed (no host launch gap between calls), timed with events
+        s = torch.cuda.Stream()
+        gctx = bz.Context(stream=s.cuda_stream)
+        calls = 8 * copies
+
+        def many():
+            for i in range(calls):
+                mm = mats[i % copies]
+                gctx.call(fn, plan, vp(mm[0]), vp(mm[1]), vp(mm[2]), vp(dx), vp(dy))
+        with torch.cuda.stream(s):
+            many()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            many()
+        g.replay()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 5
+        ev[0].record()
+        for _ in range(reps):
+            g.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        gt = ev[0].elapsed_time(ev[1]) / 1e3 / (reps * calls)
+        del g
+        gctx.close()
         buf = C.create_string_buffer(256)
         bz.lib().bzg_csr_plan_describe(plan, buf, 256)
         bz.lib().bzg_csr_plan_destroy(plan)
         kt = ms / launches / 1e3
-        return {"gbps": by / t / 1e9, "kernel_gbps": by / kt / 1e9, "frac_hbm": by / kt / 1e9 / PEAK["hbm"],
-                "us": t * 1e6, "kernel_us": kt * 1e6, "bytes": by, "nnz": nnz, "copies_rotated": copies,
-                "plan": buf.value.decode()}
+        return {"gbps": by / t / 1e9, "kernel_gbps": by / kt / 1e9,
+                "graph_gbps": by / gt / 1e9, "frac_hbm": by / gt / 1e9 / PEAK["hbm"],
+                "us": t * 1e6, "kernel_us": kt * 1e6, "graph_us": gt * 1e6, "bytes": by, "nnz": nnz,
+                "copies_rotated": copies, "plan": buf.value.decode(),
+                "timing": ("us/gbps: host-launched calls (ctypes); kernel_*: HIP events around each call; "
+                           "graph_* and frac_hbm: calls replayed back to back from a HIP graph")}
 
     # -- CPU baseline -------------------------------------------------------------------------
     def cpu_baseline(self):
