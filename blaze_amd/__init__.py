@@ -41,6 +41,7 @@ SIGNATURES = {
     "bzg_gemm_tuning_clear": (i32, []),
     "bzg_sgemm": (i32, [vp, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64]),
     "bzg_igemm": (i32, [vp, i64, i64, i64, i32, vp, i64, vp, i64, i32, vp, i64]),
+    "bzg_dtranspose": (i32, [vp, i64, i64, vp, i64, vp, i64]),
     "bzg_dgemv": (i32, [vp, i64, i64, f64, vp, i64, vp, f64, vp]),
     "bzg_sgemv": (i32, [vp, i64, i64, f32, vp, i64, vp, f32, vp]),
     "bzg_igemv": (i32, [vp, i64, i64, i32, vp, i64, vp, i32, vp]),
@@ -53,6 +54,8 @@ SIGNATURES = {
     "bzg_scsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_icsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bzg_dgetrf_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
+    "bzg_dgetrs_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dsgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, i32, vp]),
     "bzg_batched_order": (i32, [vp, i64, vp, vp]),
     "bzg_solver_autotune": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, i64, vp, i64, vp, C.POINTER(i32), C.POINTER(f32)]),

@@ -145,10 +145,44 @@ bzg_status gemv_dispatch(bzg_context* ctx, int64_t m, int64_t n, T alpha, const 
   return launch_gemv<T, 1, 1>(ctx, m, n, alpha, A, lda, x, beta, y);
 }
 
+// B = A^T for a row-major m x n A (lda) into n x m B (ldb): 64 x 64 tiles through LDS (one
+// padding element per row), reads and writes both along rows.
+__global__ __launch_bounds__(256) void transpose_kernel(int64_t m, int64_t n, const double* __restrict__ A,
+                                                        int64_t lda, double* __restrict__ B, int64_t ldb) {
+  __shared__ double tile[64][65];
+  const int64_t i0 = (int64_t)blockIdx.y * 64, j0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  for (int r = ty; r < 64; r += 4) {
+    const int64_t i = i0 + r, j = j0 + tx;
+    if (i < m && j < n) tile[r][tx] = A[i * lda + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int64_t j = j0 + r, i = i0 + tx;
+    if (j < n && i < m) B[j * ldb + i] = tile[tx][r];
+  }
+}
+
 }  // namespace
 }  // namespace bzg
 
 extern "C" {
+
+bzg_status bzg_dtranspose(bzg_context* ctx, int64_t m, int64_t n, const double* A, int64_t lda, double* B,
+                          int64_t ldb) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "transpose: null context");
+  BZG_REQUIRE(m >= 0 && n >= 0, BZG_ERR_INVALID_ARGUMENT, "transpose: negative size");
+  if (m == 0 || n == 0) return BZG_OK;
+  BZG_REQUIRE(A && B, BZG_ERR_INVALID_ARGUMENT, "transpose: null pointer");
+  BZG_REQUIRE(lda >= n && ldb >= m, BZG_ERR_INVALID_ARGUMENT, "transpose: leading dimension too small");
+  BZG_REQUIRE((m + 63) / 64 < 65536, BZG_ERR_UNSUPPORTED, "transpose: too many rows");
+  BZG_REQUIRE(static_cast<const void*>(A) != static_cast<const void*>(B), BZG_ERR_INVALID_ARGUMENT,
+              "transpose: in-place transposition is not supported");
+  hipLaunchKernelGGL(bzg::transpose_kernel, dim3((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64)), dim3(256),
+                     0, ctx->stream, m, n, A, lda, B, ldb);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
 
 bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,
                      int64_t lda, const double* x, double beta, double* y) {

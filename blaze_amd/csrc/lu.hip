@@ -497,7 +497,10 @@ __device__ __forceinline__ int next_system(int* counter, Misc<NM>& misc) {
   return s;
 }
 
-// ---- plain FP64 (or FP32) dgesv: A and B in place --------------------------------------
+// ---- plain FP64 (or FP32) dgesv / dgetrf / dgetrs: A and B in place ---------------------
+// MODE 0 (gesv): factor A, solve B.  MODE 1 (getrf): factor A only (rv may be null).
+// MODE 2 (getrs): A holds factors and ipiv its 1-based pivots (as written by MODE 1); the row
+// interchanges are applied to B in pivot order (LAPACK dlaswp) and B is solved in place.
 // NM = NSMALL: takes the systems of order <= NSMALL from the (ordered) index range and skips
 // the others; NM = NMAX: takes the systems listed in big_list (orders NSMALL+1 .. NMAX).
 // Systems outside 1..NMAX / 0..RMAX were marked info = -1 by classify_kernel.
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32
                                                           const int32_t* __restrict__ order,
                                                           const int32_t* __restrict__ big_list,
                                                           const int* __restrict__ big_count,
-                                                          int* counter) {
+                                                          int* counter, int mode) {
   __shared__ union {
     FactorShm<T, NM> f;
     SolveShm<T, NM> s;
@@ -525,13 +528,31 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(int count, const int32
     const int idx = next_system(counter, misc);
     if (idx >= count) break;
     const int s = (NM != NSMALL) ? big_list[idx] : (order ? order[idx] : idx);
-    const int n = nv[s], nrhs = rv[s];
+    const int n = nv[s], nrhs = rv ? rv[s] : 0;
     if (NM == NSMALL && (n < 1 || n > NSMALL || nrhs < 0 || nrhs > RMAX)) continue;
     T* a = A + a_off[s];
-    T* b = B + b_off[s];
-    const int inf = lu_factor<T, NM>(a, n, n, ipiv ? ipiv + p_off[s] : nullptr, sh.f, misc);
-    if (tid == 0) info[s] = inf;
-    if (inf == 0) {
+    T* b = (mode != 1) ? B + b_off[s] : nullptr;
+    int inf = 0;
+    if (mode != 2) {
+      inf = lu_factor<T, NM>(a, n, n, ipiv ? ipiv + p_off[s] : nullptr, sh.f, misc);
+    } else {
+      // permutation from the pivots: the interchanges i <-> ipiv[i] - 1 in order
+      if (tid == 0) {
+        const int32_t* pv = ipiv + p_off[s];
+        for (int i = 0; i < n; ++i) misc.perm[i] = i;
+        for (int i = 0; i < n; ++i) {
+          const int q = pv[i] - 1;
+          if (q != i && q >= 0 && q < n) {
+            const int t = misc.perm[i];
+            misc.perm[i] = misc.perm[q];
+            misc.perm[q] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0 && info) info[s] = inf;
+    if (inf == 0 && mode != 1) {
       for (int e = tid; e < n * RMAX; e += NT) {
         const int i = e / RMAX, c = e % RMAX;
         sh.s.x[e] = (c < nrhs) ? b[(long)misc.perm[i] * nrhs + c] : T(0);
@@ -762,9 +783,9 @@ __global__ __launch_bounds__(256) void classify_kernel(int count, const int32_t*
                                                        int32_t* __restrict__ big_list, int* big_count) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= count) return;
-  const int n = nv[s], nrhs = rv[s];
+  const int n = nv[s], nrhs = rv ? rv[s] : 0;
   if (n < 1 || n > NMAX || nrhs < 0 || nrhs > RMAX) {
-    info[s] = -1;
+    if (info) info[s] = -1;
     if (iter) iter[s] = 0;
   } else if (n > NSMALL) {
     big_list[atomicAdd(big_count, 1)] = s;
@@ -810,6 +831,48 @@ bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_h
   return BZG_OK;
 }
 
+}  // extern "C"
+
+namespace bzg {
+namespace {
+// Batched LU entry points (MODE as in gesv_batched_kernel).
+bzg_status lu_batched(bzg_context* ctx, int mode, int64_t count, const int32_t* n, const int32_t* nrhs,
+                      const int64_t* a_off, const int64_t* b_off, const int64_t* p_off, double* A,
+                      double* B, int32_t* ipiv, int32_t* info, const int32_t* order) {
+  // Scratch: slot 0 = counters [0] small-kernel work counter, [1] big count, [2] big-kernel
+  // work counter; slot 3 = list of the systems of order > NSMALL
+  void *cnt = nullptr, *lst = nullptr;
+  bzg_status st = workspace(ctx, 0, 256, &cnt);
+  if (!st) st = workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
+  if (st) return st;
+  int* counters = static_cast<int*>(cnt);
+  int32_t* big = static_cast<int32_t*>(lst);
+  ProfScope prof(ctx, mode == 0 ? "dgesv" : (mode == 1 ? "dgetrf" : "dgetrs"));
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 8);
+  BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(classify_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (int)count, n, nrhs, info, (int32_t*)nullptr, big, counters + 1);
+  BZG_LAUNCH_CHECK();
+  // the few systems of order > NSMALL first (their kernel holds 2 workgroups per CU), then
+  // the rest at full occupancy; persistent grids never larger than the batch
+  const int64_t cap = std::max<int64_t>(count, 1);
+  hipLaunchKernelGGL((gesv_batched_kernel<double, NMAX>),
+                     dim3((unsigned)std::min<int64_t>(ctx->num_cus * 2, cap)), dim3(NT), 0,
+                     ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
+                     (const int32_t*)big, (const int*)(counters + 1), counters + 2, mode);
+  BZG_LAUNCH_CHECK();
+  const int grid = (int)std::min<int64_t>(ctx->num_cus * g_tuning.wg_per_cu_fp64, cap);
+  hipLaunchKernelGGL((gesv_batched_kernel<double, NSMALL>), dim3(grid), dim3(NT), 0,
+                     ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
+                     (const int32_t*)big, (const int*)(counters + 1), counters + 0, mode);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+}  // namespace
+}  // namespace bzg
+
+extern "C" {
+
 bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,
                              const int64_t* a_off, const int64_t* b_off, const int64_t* p_off,
                              double* A, double* B, int32_t* ipiv, int32_t* info,
@@ -821,35 +884,32 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, 
               "dgesv_batched: null array");
   BZG_REQUIRE(ipiv == nullptr || p_off != nullptr, BZG_ERR_INVALID_ARGUMENT,
               "dgesv_batched: ipiv without p_off");
-  // Scratch: slot 0 = counters [0] small-kernel work counter, [1] big count, [2] big-kernel
-  // work counter; slot 3 = list of the systems of order > NSMALL
-  void *cnt = nullptr, *lst = nullptr;
-  bzg_status st = bzg::workspace(ctx, 0, 256, &cnt);
-  if (!st) st = bzg::workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
-  if (st) return st;
-  int* counters = static_cast<int*>(cnt);
-  int32_t* big = static_cast<int32_t*>(lst);
-  bzg::ProfScope prof(ctx, "dgesv");
-  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 8);
-  BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bzg::classify_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
-                     (int)count, n, nrhs, info, (int32_t*)nullptr, big, counters + 1);
-  BZG_LAUNCH_CHECK();
-  // the few systems of order > NSMALL first (their kernel holds 2 workgroups per CU), then
-  // the rest at full occupancy
-  // persistent grids never larger than the batch (a few systems: a few workgroups)
-  const int64_t cap = std::max<int64_t>(count, 1);
-  hipLaunchKernelGGL((bzg::gesv_batched_kernel<double, bzg::NMAX>),
-                     dim3((unsigned)std::min<int64_t>(ctx->num_cus * 2, cap)), dim3(bzg::NT), 0,
-                     ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
-                     (const int32_t*)big, (const int*)(counters + 1), counters + 2);
-  BZG_LAUNCH_CHECK();
-  const int grid = (int)std::min<int64_t>(ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64, cap);
-  hipLaunchKernelGGL((bzg::gesv_batched_kernel<double, bzg::NSMALL>), dim3(grid), dim3(bzg::NT), 0,
-                     ctx->stream, (int)count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
-                     (const int32_t*)big, (const int*)(counters + 1), counters + 0);
-  BZG_LAUNCH_CHECK();
-  return BZG_OK;
+  return bzg::lu_batched(ctx, 0, count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order);
+}
+
+bzg_status bzg_dgetrf_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+                              const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info,
+                              const int32_t* order) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dgetrf_batched: null context");
+  BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "dgetrf_batched: bad count");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(n && a_off && p_off && A && ipiv && info, BZG_ERR_INVALID_ARGUMENT,
+              "dgetrf_batched: null array");
+  return bzg::lu_batched(ctx, 1, count, n, nullptr, a_off, nullptr, p_off, A, nullptr, ipiv, info, order);
+}
+
+bzg_status bzg_dgetrs_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,
+                              const int64_t* a_off, const int64_t* b_off, const int64_t* p_off,
+                              const double* LU, const int32_t* ipiv, double* B, int32_t* info,
+                              const int32_t* order) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dgetrs_batched: null context");
+  BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "dgetrs_batched: bad count");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(n && nrhs && a_off && b_off && p_off && LU && ipiv && B && info, BZG_ERR_INVALID_ARGUMENT,
+              "dgetrs_batched: null array");
+  // the kernel only reads the factors and the pivots in this mode
+  return bzg::lu_batched(ctx, 2, count, n, nrhs, a_off, b_off, p_off, const_cast<double*>(LU), B,
+                         const_cast<int32_t*>(ipiv), info, order);
 }
 
 bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,

@@ -80,3 +80,81 @@ def dsgesv_batched(ctx: bz.Context, As, Bs, tol: float = 0.0, max_iter: int = 30
     Xh = dX.cpu().numpy()
     Xs = [Xh[b_off[i]:b_off[i] + n[i] * r[i]].reshape(n[i], r[i]) for i in range(len(As))]
     return Xs, dit.cpu().numpy()[:len(As)], dinfo.cpu().numpy()[:len(As)]
+
+
+def getrf_batched(ctx: bz.Context, As, lpt: bool = True):
+    """Batched factorisation (dgetrf per system). Returns (LUs, ipivs, infos)."""
+    import torch
+    n = np.array([a.shape[0] for a in As], np.int32)
+    a_off = np.zeros(len(As), np.int64); p_off = np.zeros(len(As), np.int64)
+    if len(As):
+        a_off[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+        p_off[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    Ap = np.concatenate([np.ascontiguousarray(a, np.float64).ravel() for a in As])
+    dn, da, dp, dA = (_dev(torch, x) for x in (n, a_off, p_off, Ap))
+    dpiv = torch.zeros(max(int(n.sum()), 1), dtype=torch.int32, device="cuda")
+    dinfo = torch.full((max(len(As), 1),), -99, dtype=torch.int32, device="cuda")
+    order = order_longest_first(ctx, n) if lpt else None
+    ctx.call("bzg_dgetrf_batched", len(As), _p(dn), _p(da), _p(dp), _p(dA), _p(dpiv), _p(dinfo),
+             _p(order) if order is not None else C.c_void_p(0))
+    ctx.sync()
+    Ah, ph = dA.cpu().numpy(), dpiv.cpu().numpy()
+    LUs = [Ah[a_off[i]:a_off[i] + n[i] * n[i]].reshape(n[i], n[i]) for i in range(len(As))]
+    ipivs = [ph[p_off[i]:p_off[i] + n[i]] for i in range(len(As))]
+    return LUs, ipivs, dinfo.cpu().numpy()[:len(As)]
+
+
+def getrs_batched(ctx: bz.Context, LUs, ipivs, Bs, lpt: bool = True):
+    """Batched solve with given factors (dgetrs per system). Returns (Xs, infos)."""
+    import torch
+    n, r, a_off, b_off, Ap, Bp = pack(LUs, Bs)
+    p_off = np.zeros(len(LUs), np.int64)
+    if len(LUs):
+        p_off[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    Pp = np.concatenate([np.asarray(p, np.int32) for p in ipivs])
+    dn, dr, da, db, dp, dA, dB, dpiv = (_dev(torch, x) for x in (n, r, a_off, b_off, p_off, Ap, Bp, Pp))
+    dinfo = torch.full((max(len(LUs), 1),), -99, dtype=torch.int32, device="cuda")
+    order = order_longest_first(ctx, n) if lpt else None
+    ctx.call("bzg_dgetrs_batched", len(LUs), _p(dn), _p(dr), _p(da), _p(db), _p(dp), _p(dA), _p(dpiv), _p(dB),
+             _p(dinfo), _p(order) if order is not None else C.c_void_p(0))
+    ctx.sync()
+    Xh = dB.cpu().numpy()
+    return [Xh[b_off[i]:b_off[i] + n[i] * r[i]].reshape(n[i], r[i]) for i in range(len(LUs))], \
+        dinfo.cpu().numpy()[:len(LUs)]
+
+
+def lu_rowmajor(ctx: bz.Context, A):
+    """lu(A, L, U, P) with the reference's row-major convention (blaze/math/dense/LU.h:98-140,
+    222-330): the column-major getrf sees A^T, so A = L * U * P with L lower triangular (its
+    diagonal from the factorisation), U unit upper triangular and P the column permutation
+    P(i, permut[i]) = 1.  Square A only.  Runs bzg_dtranspose + bzg_dgetrf_batched on the device."""
+    import torch
+    A = np.ascontiguousarray(A, np.float64)
+    n = A.shape[0]
+    assert A.shape == (n, n)
+    dA = _dev(torch, A)
+    dT = torch.empty_like(dA)
+    ctx.call("bzg_dtranspose", n, n, _p(dA), n, _p(dT), n)
+    LUs, ipivs, infos = getrf_batched_device(ctx, dT, n)
+    R = LUs.T  # the factors of A^T, read back in A's row-major layout
+    L = np.tril(R)
+    U = np.triu(R, 1) + np.eye(n)
+    permut = np.arange(n)
+    for i, p in enumerate(ipivs - 1):
+        if p != i:
+            permut[[i, p]] = permut[[p, i]]
+    P = np.zeros((n, n))
+    P[np.arange(n), permut] = 1.0
+    return L, U, P, int(infos)
+
+
+def getrf_batched_device(ctx: bz.Context, dA, n: int):
+    """One n x n system already on the device (factored in place). Returns (LU, ipiv, info)."""
+    import torch
+    dn = _dev(torch, np.array([n], np.int32))
+    dz = _dev(torch, np.zeros(1, np.int64))
+    dpiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dinfo = torch.full((1,), -99, dtype=torch.int32, device="cuda")
+    ctx.call("bzg_dgetrf_batched", 1, _p(dn), _p(dz), _p(dz), _p(dA), _p(dpiv), _p(dinfo), C.c_void_p(0))
+    ctx.sync()
+    return dA.cpu().numpy().reshape(n, n), dpiv.cpu().numpy(), dinfo.cpu().numpy()[0]

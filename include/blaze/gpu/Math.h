@@ -715,6 +715,59 @@ inline DMatDVecSolveExpr<double> solve(const DynamicMatrix<double>& A, const Dyn
   return {A, b};
 }
 
+// ---------------------------------------------------------------------------------------
+// lu(A, L, U, P) (reference: blaze/math/dense/LU.h:98-140, 222-330) for a square row-major
+// matrix.  The reference hands the row-major array to the column-major getrf, i.e. factors
+// A^T with row interchanges, which makes A = L * U * P: L lower triangular (its diagonal from
+// the factorisation), U unit upper triangular, P the column permutation P(i, permut[i]) = 1.
+// Here: bzg_dtranspose + bzg_dgetrf_batched on the device, the triangles and the permutation
+// are assembled from the factors on the host.  Like the reference, it does not fail for a
+// singular A.  Square matrices of order <= 512.
+// ---------------------------------------------------------------------------------------
+inline void lu(const DynamicMatrix<double>& A, DynamicMatrix<double>& L, DynamicMatrix<double>& U,
+               DynamicMatrix<double>& P) {
+  const size_t n = A.rows();
+  if (A.rows() != A.columns())
+    throw std::invalid_argument("lu: this build supports square matrices");
+  if (n > 512) throw std::invalid_argument("lu: this build supports n <= 512");
+  L.resize(n, n, false);
+  U.resize(n, n, false);
+  P.resize(n, n, false);
+  if (n == 0) return;
+  detail::DeviceBuffer<double> T(n * n);
+  detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "lu: transpose");
+  const int32_t nn = (int32_t)n;
+  const int64_t zero = 0;
+  detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
+  detail::DeviceBuffer<int64_t> doff;
+  dn.allocate(1); dpiv.allocate(n); dinfo.allocate(1); doff.allocate(1);
+  detail::h2d(dn.ptr, &nn, 1);
+  detail::h2d(doff.ptr, &zero, 1);
+  detail::check(bzg_dgetrf_batched(context(), 1, dn.ptr, doff.ptr, doff.ptr, T.ptr, dpiv.ptr, dinfo.ptr, nullptr),
+                "lu: getrf");
+  std::vector<double> f(n * n), l(n * n, 0.0), u(n * n, 0.0), pm(n * n, 0.0);
+  std::vector<int32_t> ipiv(n);
+  detail::d2h(f.data(), T.ptr, n * n);
+  detail::d2h(ipiv.data(), dpiv.ptr, n);
+  // f holds the factors of A^T row-major; R = f^T in A's layout
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t j = 0; j < i; ++j) l[i * n + j] = f[j * n + i];
+    l[i * n + i] = f[i * n + i];
+    u[i * n + i] = 1.0;
+    for (size_t j = i + 1; j < n; ++j) u[i * n + j] = f[j * n + i];
+  }
+  std::vector<size_t> permut(n);
+  for (size_t i = 0; i < n; ++i) permut[i] = i;
+  for (size_t i = 0; i < n; ++i) {
+    const size_t q = (size_t)(ipiv[i] - 1);
+    if (q != i) std::swap(permut[q], permut[i]);
+  }
+  for (size_t i = 0; i < n; ++i) pm[i * n + permut[i]] = 1.0;
+  L.upload(l.data());
+  U.upload(u.data());
+  P.upload(pm.data());
+}
+
 template <typename T>
 DynamicVector<T>& DynamicVector<T>::operator=(const DMatDVecSolveExpr<T>& e) {
   static_assert(std::is_same<T, double>::value, "solve() is provided for double");

@@ -20,6 +20,8 @@
  *        (SMatDVecMultExpr.h:204-207 -> SVecDVecInnerExpr.h:127-135)
  *   bzg_dgesv_batched                  <- solve() (blaze/math/dense/LSE.h:6343-6373) ->
  *        gesv (blaze/math/lapack/gesv.h:136-175) -> dgesv_ (blaze/math/lapack/clapack/gesv.h:207)
+ *   bzg_dgetrf_batched / bzg_dgetrs_batched <- getrf / getrs (blaze/math/lapack/clapack/getrf.h:190-197,
+ *        clapack/getrs.h:207-215), the factorisation under lu() (blaze/math/dense/LU.h:98-140)
  *   bzg_dsgesv_batched                 <- same call site; LAPACK dsgesv semantics (mixed precision)
  *   bzg_dgemm_fp64acc[_i8]             <- no reference function: FP64-accurate GEMM from BF16
  *        (INT8) residue slices; drop-in for bzg_dgemm at the same call site
@@ -131,6 +133,11 @@ bzg_status bzg_gemm_autotune(bzg_context* ctx, int32_t fp32, int64_t m, int64_t 
 bzg_status bzg_gemm_tuning_clear(void);
 
 /* ---- dense GEMV:  y = alpha * A * x + beta * y  (row-major A, device pointers) ---------- */
+/* B = A^T for a row-major m x n A into the n x m B (ldb >= m), not in place.  No reference
+ * function of its own: the storage-order change lu() makes when it hands a row-major matrix
+ * to the column-major getrf (blaze/math/dense/LU.h:98-140, lapack/getrf.h). */
+bzg_status bzg_dtranspose(bzg_context* ctx, int64_t m, int64_t n, const double* A, int64_t lda, double* B,
+                          int64_t ldb);
 bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,
                      int64_t lda, const double* x, double beta, double* y);
 bzg_status bzg_sgemv(bzg_context* ctx, int64_t m, int64_t n, float alpha, const float* A,
@@ -184,7 +191,8 @@ bzg_status bzg_icsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t*
  * pivot.  Row-major storage means the factorisation is of A itself (not of A^T), with
  * partial pivoting by rows exactly as dgetf2: the first row of maximal |a|.
  * 1 <= n[i] <= 512, 0 <= nrhs[i] <= 8; a system outside that range gets info[i] = -1.
- * Orders up to 256 run at 8 workgroups per CU, orders 257..512 on a second kernel instance
+ * Orders up to 256 run on persistent workgroups (up to 4 resident per CU; the grid size is
+ * set by bzg_solver_autotune, default 8 per CU), orders 257..512 on a second kernel instance
  * with twice the LDS (2 per CU).
  * `order` (device, may be NULL) lists the systems in the order the persistent workgroups
  * should take them -- bzg_batched_order() gives longest-first; NULL = index order.
@@ -193,6 +201,21 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
                              const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
                              const int64_t* p_off, double* A, double* B, int32_t* ipiv,
                              int32_t* info, const int32_t* order);
+/* Factorisation only (LAPACK dgetrf per system, replaces getrf, clapack/getrf.h:190-197):
+ * A_i := L U in place (unit-lower L below the diagonal), ipiv_i 1-based, info as above.
+ * The factors and pivots are exactly those bzg_dgesv_batched computes for the same A_i. */
+bzg_status bzg_dgetrf_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+                              const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info,
+                              const int32_t* order);
+/* Solve with factors (LAPACK dgetrs per system, replaces getrs, clapack/getrs.h:207-215):
+ * LU / ipiv as written by bzg_dgetrf_batched; the interchanges are applied to B_i in pivot
+ * order, then B_i := U^-1 L^-1 B_i in place.  info[i] = 0, or -1 outside the size range.
+ * A zero on U's diagonal is not checked (as in dgetrs).  Same result, bit for bit, as the
+ * solve phase of bzg_dgesv_batched. */
+bzg_status bzg_dgetrs_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,
+                              const int64_t* a_off, const int64_t* b_off, const int64_t* p_off,
+                              const double* LU, const int32_t* ipiv, double* B, int32_t* info,
+                              const int32_t* order);
 /* Mixed precision: FP32 LU with partial pivoting + FP64 iterative refinement.  A and B are
  * read only; X (same layout as B) receives the solution.  Per system (LAPACK dsgesv):
  * iter[i] >= 0  converged after iter[i] refinement steps;

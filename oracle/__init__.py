@@ -125,6 +125,16 @@ def getrf(A: np.ndarray):
     return LU, ip, info
 
 
+def getrs(LU: np.ndarray, ipiv: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """dgetrs ('N'): X from the factors and 1-based pivots of getrf (B is n x nrhs)."""
+    LU = np.ascontiguousarray(LU, np.float64)
+    ip = np.ascontiguousarray(ipiv, np.int32)
+    X = np.array(B, np.float64, order="C", copy=True)
+    n, nrhs = X.shape
+    orc().orc_getrs(i32(n), i32(nrhs), _p(LU), i32(n), _p(ip), _p(X), i32(nrhs))
+    return X
+
+
 def gesv(A: np.ndarray, B: np.ndarray):
     """(X, LU, ipiv, info) of A X = B (B is n x nrhs)."""
     LU = np.array(A, np.float64, order="C", copy=True)

@@ -234,6 +234,31 @@ int main() {
     for (size_t q = 0; q < n * r; ++q) ok = ok && relaxed(g[q], b[q]);
     check(ok, "solve(A,X,B) with 5 right-hand sides");
   }
+  // lu(A, L, U, P): lu/DenseTest.h:159-170 -- the decomposition reproduces A; for a row-major
+  // matrix A = L * U * P with L lower, U unit upper and P a permutation (LU.h:166-172)
+  for (size_t n : {1, 2, 3, 9, 40, 130}) {
+    std::vector<double> a(n * n);
+    for (auto& v : a) v = U(rng);
+    DynamicMatrix<double> A(n, n, a.data()), L, Um, P, LU, LUP;
+    lu(A, L, Um, P);
+    LU = L * Um;
+    LUP = LU * P;
+    auto l = L.toVector(), u = Um.toVector(), pm = P.toVector(), r = LUP.toVector();
+    bool ok = L.rows() == n && Um.columns() == n && P.rows() == n;
+    for (size_t i = 0; i < n; ++i) {
+      size_t ones = 0;
+      for (size_t j = 0; j < n; ++j) {
+        if (j > i) ok = ok && l[i * n + j] == 0.0;
+        if (j < i) ok = ok && u[i * n + j] == 0.0;
+        if (j == i) ok = ok && u[i * n + j] == 1.0;
+        ok = ok && (pm[i * n + j] == 0.0 || pm[i * n + j] == 1.0);
+        ones += pm[i * n + j] == 1.0;
+        ok = ok && relaxed(r[i * n + j], a[i * n + j], 1e-12);
+      }
+      ok = ok && ones == 1;
+    }
+    check(ok, "lu(A,L,U,P), A == L*U*P, N = " + std::to_string(n));
+  }
   {
     DynamicMatrix<double> S3{{1, 2, 3}, {2, 4, 6}, {0, 1, 1}};
     DynamicMatrix<double> S9(9, 9, 1.0);

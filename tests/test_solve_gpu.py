@@ -9,6 +9,8 @@ blocked update differs from the unblocked oracle); solutions within 1e-10 relati
 diagonally dominant systems; backward error <= 1e-12 for the mixed-precision solver
 (its convergence target); exact singularity reported as info = first zero pivot.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -141,3 +143,71 @@ def test_orders_above_256_and_out_of_range():
     assert list(infob) == [-1, -1, 0]
     _, itb, infob2 = solve.dsgesv_batched(ctx(), bad_A, bad_B)
     assert list(infob2) == [-1, -1, 0] and itb[0] == 0 and itb[1] == 0
+
+
+def test_getrf_getrs_match_gesv_and_oracle():
+    # dgetrf factors / pivots == the oracle's dgetf2 (pivots identical, factors within the
+    # blocked-update rounding), dgetrs with those factors == the gesv solution bit for bit
+    # (same kernels), and == the oracle's dgetrs on the same factors within rounding
+    from blaze_amd import solve
+    rng = np.random.default_rng(46)
+    sizes = [1, 2, 3, 7, 16, 33, 64, 100, 255, 256, 300, 512]
+    As = [rng.uniform(-1, 1, (n, n)) for n in sizes]
+    Bs = [rng.uniform(-1, 1, (n, int(rng.integers(1, 9)))) for n in sizes]
+    LUs, ipivs, infos = solve.getrf_batched(ctx(), As)
+    Xg, infog, LUg, ipg = solve.gesv_batched(ctx(), As, Bs, with_ipiv=True)
+    Xs, infos2 = solve.getrs_batched(ctx(), LUs, ipivs, Bs)
+    for A, B, LU, ip, X, Xr, LU2, ip2, n in zip(As, Bs, LUs, ipivs, Xs, Xg, LUg, ipg, sizes):
+        LUo, ipo, info_o = oracle.getrf(A)
+        assert info_o == 0
+        assert np.array_equal(ip, ipo), n
+        assert np.array_equal(LU, LU2) and np.array_equal(ip, ip2), n   # same factors as gesv
+        assert np.abs(LU - LUo).max() <= 1e-12 * max(1, np.abs(LUo).max()) * n, n
+        assert np.array_equal(X, Xr), n                                  # same solve as gesv
+        Xo = oracle.getrs(LU, ip, B)
+        assert np.allclose(X, Xo, rtol=1e-10, atol=1e-12), n
+    assert (infos == 0).all() and (infos2 == 0).all() and (infog == 0).all()
+
+
+def test_getrf_singular_and_out_of_range():
+    from blaze_amd import solve
+    A = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [1.0, 0.0, 1.0]])  # rank 2
+    LUs, ipivs, infos = solve.getrf_batched(ctx(), [A, np.eye(4)])
+    _, ipo, info_o = oracle.getrf(A)
+    assert infos[0] == info_o > 0 and infos[1] == 0
+    assert np.array_equal(ipivs[0], ipo)
+
+
+def test_lu_rowmajor_convention():
+    # lu(A, L, U, P) for a row-major matrix (LU.h:98-140, 222-330): A = L * U * P, L lower,
+    # U unit upper, P a column permutation -- the factorisation of A^T by the oracle's dgetf2
+    from blaze_amd import solve
+    rng = np.random.default_rng(47)
+    for n in (1, 2, 5, 17, 64, 200):
+        A = rng.uniform(-1, 1, (n, n))
+        L, U, P, info = solve.lu_rowmajor(ctx(), A)
+        assert info == 0
+        assert np.array_equal(L, np.tril(L)) and np.array_equal(U, np.triu(U))
+        assert np.array_equal(np.diag(U), np.ones(n))
+        assert np.array_equal(P @ P.T, np.eye(n)) and set(np.unique(P)) <= {0.0, 1.0}
+        assert np.abs(L @ U @ P - A).max() <= 1e-13 * n * max(1, np.abs(A).max()), n
+        LUo, ipo, _ = oracle.getrf(np.ascontiguousarray(A.T))
+        assert np.abs(np.tril(LUo.T) - L).max() <= 1e-12 * n, n     # L = U_o^T
+        permut = np.arange(n)
+        for i, p in enumerate(ipo - 1):
+            if p != i:
+                permut[[i, p]] = permut[[p, i]]
+        assert np.array_equal(np.argmax(P, axis=1), permut), n
+
+
+def test_transpose():
+    import torch
+    rng = np.random.default_rng(48)
+    for m, n, pad in ((1, 1, 0), (3, 70, 1), (129, 65, 3), (1000, 7, 0)):
+        A = rng.uniform(-1, 1, (m, n + pad))
+        dA = torch.from_numpy(A).cuda()
+        dB = torch.full((n, m + pad), 7.0, dtype=torch.float64, device="cuda")
+        ctx().call("bzg_dtranspose", m, n, C.c_void_p(dA.data_ptr()), n + pad, C.c_void_p(dB.data_ptr()), m + pad)
+        B = dB.cpu().numpy()
+        assert np.array_equal(B[:, :m], A[:, :n].T)
+        assert (B[:, m:] == 7.0).all()
