@@ -27,6 +27,7 @@
 #ifndef BLAZE_GPU_MATH_H
 #define BLAZE_GPU_MATH_H
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <initializer_list>
@@ -542,6 +543,12 @@ class CompressedMatrix {
     if (i < cur_row_) throw std::logic_error("Rows must be finalized in order");
     close_rows_before(i + 1);
   }
+  // set / insert (CompressedMatrix.h:2089-2135): set overwrites element (i, j) or inserts it;
+  // insert throws std::invalid_argument("Bad access index") when the element exists.  Either
+  // closes the append/finalize fill (every row counts as finalized afterwards) and costs
+  // O(nonZeros()) on the host; the device copy is rebuilt on the next product.
+  void set(size_t i, size_t j, const T& value) { put(i, j, value, false); }
+  void insert(size_t i, size_t j, const T& value) { put(i, j, value, true); }
   // Bulk CSR (rowptr n+1, colidx, values) -- the packed form, e.g. from a file reader.
   void assignCSR(size_t m, size_t n, const int32_t* rowptr, const int32_t* col, const T* val) {
     resize(m, n);
@@ -572,6 +579,21 @@ class CompressedMatrix {
   const T* d_values() const { upload(); return d_val_.ptr; }
 
  private:
+  void put(size_t i, size_t j, const T& value, bool must_be_new) {
+    if (i >= m_ || j >= n_) throw std::out_of_range("Invalid compressed matrix access index");
+    close_rows_before(m_);
+    const size_t b = rowptr_[i], e = rowptr_[i + 1];
+    const size_t pos = std::lower_bound(col_.begin() + b, col_.begin() + e, (int32_t)j) - col_.begin();
+    drop_device();
+    if (pos < e && col_[pos] == (int32_t)j) {
+      if (must_be_new) throw std::invalid_argument("Bad access index");
+      val_[pos] = value;
+      return;
+    }
+    col_.insert(col_.begin() + pos, (int32_t)j);
+    val_.insert(val_.begin() + pos, value);
+    for (size_t r = i + 1; r <= m_; ++r) ++rowptr_[r];
+  }
   // rows < i are complete: their end offsets are the current element count
   void close_rows_before(size_t i) {
     while (cur_row_ < i) {

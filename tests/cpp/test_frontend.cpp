@@ -234,6 +234,37 @@ int main() {
     for (size_t q = 0; q < n * r; ++q) ok = ok && relaxed(g[q], b[q]);
     check(ok, "solve(A,X,B) with 5 right-hand sides");
   }
+  // set / insert (CompressedMatrix.h:2089-2135): random order, overwrite, duplicate insert
+  {
+    const size_t m = 50, n = 40;
+    CompressedMatrix<double> S(m, n);
+    std::vector<double> dense(m * n, 0.0), x(n);
+    for (auto& v : x) v = U(rng);
+    std::uniform_int_distribution<size_t> Ri(0, m - 1), Rj(0, n - 1);
+    for (int t = 0; t < 300; ++t) {
+      const size_t i = Ri(rng), j = Rj(rng);
+      const double v = U(rng);
+      S.set(i, j, v);
+      dense[i * n + j] = v;
+    }
+    bool ok = throws<std::invalid_argument>([&] {
+      size_t i = 0, j = 0;
+      while (dense[i * n + j] == 0.0) { ++j; if (j == n) { j = 0; ++i; } }
+      S.insert(i, j, 1.0);
+    }, "Bad access index");
+    size_t fi = 0, fj = 0;
+    while (dense[fi * n + fj] != 0.0) { ++fj; if (fj == n) { fj = 0; ++fi; } }
+    S.insert(fi, fj, 2.5);
+    dense[fi * n + fj] = 2.5;
+    DynamicVector<double> X(n, x.data()), Y;
+    Y = S * X;
+    auto got = Y.toVector();
+    auto want = host_gemm(dense, x, m, 1, n);
+    for (size_t i = 0; i < m; ++i) ok = ok && relaxed(got[i], want[i], 1e-12);
+    for (size_t i = 0; i < m; ++i)
+      for (size_t j = 0; j < n; ++j) ok = ok && S(i, j) == dense[i * n + j];
+    check(ok, "CompressedMatrix set/insert in random order, then S * x");
+  }
   // lu(A, L, U, P): lu/DenseTest.h:159-170 -- the decomposition reproduces A; for a row-major
   // matrix A = L * U * P with L lower, U unit upper and P a permutation (LU.h:166-172)
   for (size_t n : {1, 2, 3, 9, 40, 130}) {
