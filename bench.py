@@ -383,6 +383,61 @@ class Bench:
                 "timing": ("us/gbps: host-launched calls (ctypes); kernel_*: HIP events around each call; "
                            "graph_* and frac_hbm: calls replayed back to back from a HIP graph")}
 
+    # -- conjugate gradients (the reference's CG benchmark loop, blazemark CG.cpp) -------------
+    def cg_leg(self, N=2048, iterations=50, reps=5):
+        """5-point matrix N^2 x N^2, b = 0, random start (as blazemark); the whole bzg_dcg call
+        (SpMV + 3 streaming kernels per iteration) captured in a HIP graph and replayed."""
+        torch, bz = self.torch, self.bz
+        n = N * N
+        # rows in the order top, left, centre, right, bottom (CG.cpp:86-103)
+        i, j = np.divmod(np.arange(n, dtype=np.int64), N)
+        cnt = 1 + (i > 0) + (j > 0) + (j < N - 1) + (i < N - 1)
+        rp = np.zeros(n + 1, np.int64); rp[1:] = np.cumsum(cnt)
+        ci = np.empty(int(rp[-1]), np.int32); va = np.empty(int(rp[-1]))
+        pos = rp[:-1].copy()
+        for cond, col, v in ((i > 0, (i - 1) * N + j, -1.0), (j > 0, i * N + j - 1, -1.0),
+                             (np.ones(n, bool), i * N + j, 4.0), (j < N - 1, i * N + j + 1, -1.0),
+                             (i < N - 1, (i + 1) * N + j, -1.0)):
+            ci[pos[cond]] = col[cond]; va[pos[cond]] = v; pos[cond] += 1
+        rp = rp.astype(np.int32)
+        nnz = int(rp[-1])
+        d = [self.dev(a) for a in (rp, ci, va)]
+        b = torch.zeros(n, dtype=torch.float64, device="cuda")
+        x0 = self.dev(self.uniform(n, 31))
+        x = x0.clone()
+        plan = C.c_void_p()
+        bz.check("plan", bz.lib().bzg_csr_plan_create(self.ctx.handle, n, n, rp.ctypes.data_as(C.c_void_p),
+                                                      C.byref(plan)))
+        s = torch.cuda.Stream()
+        gctx = bz.Context(stream=s.cuda_stream)
+
+        def call():
+            gctx.call("bzg_dcg", plan, n, vp(d[0]), vp(d[1]), vp(d[2]), vp(b), vp(x), iterations, None)
+        with torch.cuda.stream(s):
+            call()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            call()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        x.copy_(x0); g.replay(); torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            g.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        t_call = ev[0].elapsed_time(ev[1]) / 1e3 / reps
+        del g
+        gctx.close()
+        bz.lib().bzg_csr_plan_destroy(plan)
+        spmv_bytes = nnz * 12 + (n + 1) * 4 + 2 * n * 8
+        it_bytes = spmv_bytes + 88 * n  # + d.h (16 n), x/r update (48 n), d update (24 n)
+        t_it = t_call / (iterations + 1)
+        return {"N": N, "unknowns": n, "nnz": nnz, "iterations": iterations, "ms_per_call": t_call * 1e3,
+                "us_per_iteration": t_it * 1e6, "bytes_per_iteration": it_bytes,
+                "gbps": it_bytes / t_it / 1e9, "frac_hbm": it_bytes / t_it / 1e9 / PEAK["hbm"],
+                "timing": "one bzg_dcg call captured in a HIP graph, replayed; per iteration = call / (iterations + 1)"}
+
     # -- CPU baseline -------------------------------------------------------------------------
     def cpu_baseline(self):
         # The reference's own GEMM (Blaze DMatDMatMultExpr, OpenMP) compiled from its headers
@@ -472,8 +527,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--systems", type=int, default=100_000)
-    ap.add_argument("--legs", default="gemm,solve,spmv,serve,cpu",
-                    help="comma list of gemm, solve, spmv, serve, cpu, copy (1 GiB copy for counter calibration)")
+    ap.add_argument("--legs", default="gemm,solve,spmv,cg,serve,cpu",
+                    help="comma list of gemm, solve, spmv, cg, serve, cpu, copy (1 GiB copy for counter calibration)")
     ap.add_argument("--variant", default="auto", choices=["auto", "bf16", "int8"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -502,6 +557,12 @@ def main():
             log(f"[spmv] {json.dumps(rec['spmv'])}")
         except Exception as e:
             rec["spmv"] = {"error": repr(e)}
+    if "cg" in legs:
+        try:
+            rec["cg"] = b.cg_leg()
+            log(f"[cg] {json.dumps(rec['cg'])}")
+        except Exception as e:
+            rec["cg"] = {"error": repr(e)}
     if "serve" in legs:
         try:
             from blaze_amd import serve_bench

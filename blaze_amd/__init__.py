@@ -53,6 +53,7 @@ SIGNATURES = {
     "bzg_dcsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_scsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_icsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+    "bzg_dcg": (i32, [vp, vp, i64, vp, vp, vp, vp, vp, i32, vp]),
     "bzg_dgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrf_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrs_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -18,6 +18,8 @@
  *        and cblas_dgemv (blaze/math/blas/cblas/gemv.h:153)
  *   bzg_dcsrmv / bzg_scsrmv / bzg_icsrmv <- SMatDVecMultExpr element evaluation
  *        (SMatDVecMultExpr.h:204-207 -> SVecDVecInnerExpr.h:127-135)
+ *   bzg_dcg                            <- the CG loop of blazemark/src/blaze/CG.cpp:114-135 (its
+ *        SpMV, dot products and vector updates, resident on the device)
  *   bzg_dgesv_batched                  <- solve() (blaze/math/dense/LSE.h:6343-6373) ->
  *        gesv (blaze/math/lapack/gesv.h:136-175) -> dgesv_ (blaze/math/lapack/clapack/gesv.h:207)
  *   bzg_dgetrf_batched / bzg_dgetrs_batched <- getrf / getrs (blaze/math/lapack/clapack/getrf.h:190-197,
@@ -181,6 +183,20 @@ bzg_status bzg_scsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t*
                       const int32_t* colidx, const float* val, const float* x, float* y);
 bzg_status bzg_icsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,
                       const int32_t* colidx, const int32_t* val, const int32_t* x, int32_t* y);
+
+/* ---- conjugate gradients on a resident CSR matrix ----------------------------------------
+ * The loop of the reference's CG benchmark (blazemark/src/blaze/CG.cpp:114-135):
+ *   r = A x - b;  delta = r.r;  d = -r;  then `iterations` times:
+ *   h = A d;  alpha = delta / (d.h);  x += alpha d;  r += alpha h;  beta = r.r;
+ *   d = (beta / delta) d - r;  delta = beta.
+ * A is n x n CSR (device arrays) with a plan from bzg_csr_plan_create; b and x are device
+ * vectors (x: start in, result out).  The products are bzg_dcsrmv's; dot products are
+ * deterministic (fixed partial-sum order), and no scalar leaves the device inside the loop, so
+ * the call can be captured in a HIP graph.  rr_out (host, may be NULL) receives r.r after the
+ * last iteration (one stream synchronisation).  Scratch (3 n doubles) is cached in the context. */
+bzg_status bzg_dcg(bzg_context* ctx, const bzg_csr_plan* plan, int64_t n, const int32_t* rowptr,
+                   const int32_t* colidx, const double* val, const double* b, double* x, int32_t iterations,
+                   double* rr_out);
 
 /* ---- batched dense solves (LAPACK dgesv / dsgesv semantics per system) ------------------
  * `count` independent systems A_i X_i = B_i of order n[i] with nrhs[i] right-hand sides.

@@ -125,6 +125,39 @@ def getrf(A: np.ndarray):
     return LU, ip, info
 
 
+def cg(rowptr, col, val, b, x0, iterations: int):
+    """The CG loop of blazemark/src/blaze/CG.cpp:114-135 (products by csrmv, the reference's
+    sequential row sums; dot products by numpy).  Returns (x, r.r after the last iteration)."""
+    x = np.array(x0, np.float64, copy=True)
+    r = csrmv(rowptr, col, val, x) - b
+    delta = float(r @ r)
+    d = -r
+    for _ in range(iterations):
+        h = csrmv(rowptr, col, val, d)
+        alpha = delta / float(d @ h)
+        x += alpha * d
+        r += alpha * h
+        beta = float(r @ r)
+        d = (beta / delta) * d - r
+        delta = beta
+    return x, delta
+
+
+def laplace2d(N: int):
+    """The 5-point matrix of blazemark/src/blaze/CG.cpp:86-103 (row-major CSR, diagonal 4,
+    neighbours -1, rows in the order top, left, centre, right, bottom)."""
+    rp, ci, va = [0], [], []
+    for i in range(N):
+        for j in range(N):
+            if i > 0: ci.append((i - 1) * N + j); va.append(-1.0)
+            if j > 0: ci.append(i * N + j - 1); va.append(-1.0)
+            ci.append(i * N + j); va.append(4.0)
+            if j < N - 1: ci.append(i * N + j + 1); va.append(-1.0)
+            if i < N - 1: ci.append((i + 1) * N + j); va.append(-1.0)
+            rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(va)
+
+
 def getrs(LU: np.ndarray, ipiv: np.ndarray, B: np.ndarray) -> np.ndarray:
     """dgetrs ('N'): X from the factors and 1-based pivots of getrf (B is n x nrhs)."""
     LU = np.ascontiguousarray(LU, np.float64)

@@ -304,3 +304,25 @@ def test_oracle_matches_compiled_reference_kernels():
         xr, rcr = oracle.ref_solve_small(M, b)
         xo, rco = oracle.solve_closed(M, b)
         assert rcr == rco == 0 and np.allclose(xo, xr, rtol=1e-14, atol=1e-16)
+
+
+def test_cg_oracle_converges_to_direct_solve():
+    # pinned by the mathematics: on the SPD 5-point matrix, CG in exact arithmetic reaches the
+    # solution in at most n steps; in FP64 it must agree with a direct solve to ~1e-10
+    rp, ci, va = oracle.laplace2d(8)
+    n = 64
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, ci[rp[i]:rp[i + 1]]] = va[rp[i]:rp[i + 1]]
+    assert np.array_equal(A, A.T) and (np.linalg.eigvalsh(A) > 0).all()
+    assert np.array_equal(np.diag(A), np.full(n, 4.0)) and A.sum() == 4 * n - 2 * 2 * 8 * 7
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, n)
+    x, rr = oracle.cg(rp, ci, va, b, rng.uniform(-1, 1, n), 120)
+    xs = np.linalg.solve(A, b)
+    assert np.abs(x - xs).max() <= 1e-10 * np.abs(xs).max()
+    assert rr <= 1e-20 * (b @ b)
+    # b = 0 (the benchmark's setting): x is driven to 0
+    x0 = rng.uniform(-1, 1, n)
+    x, rr = oracle.cg(rp, ci, va, np.zeros(n), x0, 120)
+    assert np.abs(x).max() <= 1e-10
