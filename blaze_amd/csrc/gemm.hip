@@ -88,7 +88,10 @@ struct GemmShm {
 // One BM x BN tile (tm, tn) of C = alpha A B + beta C by the 4 wavefronts of the workgroup.
 // Every element of C is summed in the same k order (slabs of kBK, MFMA k-steps of 4) whatever
 // the tile shape, so all tile shapes give bit-identical results.
-template <typename T, int BM, int BN, int WM, int WN, bool VEC>
+// TA / TB: A (B) is given transposed, i.e. stored k x m (n x k) row-major with leading
+// dimension lda >= m (ldb >= k); the slabs are then read along m (k) and transposed on the
+// way into LDS, so the MFMA loop and its summation order are the same for all four forms.
+template <typename T, int BM, int BN, int WM, int WN, bool VEC, bool TA = false, bool TB = false>
 __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T* __restrict__ A,
                                           long lda, const T* __restrict__ B, long ldb, T beta,
                                           T* __restrict__ C, long ldc, int tm, int tn,
@@ -107,6 +110,10 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
   constexpr int A_RPP = kThreads / A_VPR;        // A rows per pass
   constexpr int B_VPR = BN / 2;
   constexpr int B_RPP = kThreads / B_VPR;
+  constexpr int AT_VPR = BM / 2;                 // transposed A: vectors per stored k row
+  constexpr int AT_RPP = kThreads / AT_VPR;
+  constexpr int BT_VPR = kBK / 2;                // transposed B: vectors per stored n row
+  constexpr int BT_RPP = kThreads / BT_VPR;
   static_assert(NA >= 1 && NB >= 1, "tile too small for 256 threads");
   static_assert((BM / WM) * (BN / WN) == 4, "four wavefronts per workgroup");
 
@@ -130,49 +137,79 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
 
   V2 ra[NA], rb[NB];
 
+  // thread-local (row, column) of vector i in the stored slab
+  auto a_rc = [&](int i, int& r, int& c) {
+    if (TA) { r = tid / AT_VPR + i * AT_RPP; c = (tid % AT_VPR) * 2; }  // r: k, c: m
+    else    { r = tid / A_VPR + i * A_RPP;   c = (tid % A_VPR) * 2; }   // r: m, c: k
+  };
+  auto b_rc = [&](int i, int& r, int& c) {
+    if (TB) { r = tid / BT_VPR + i * BT_RPP; c = (tid % BT_VPR) * 2; }  // r: n, c: k
+    else    { r = tid / B_VPR + i * B_RPP;   c = (tid % B_VPR) * 2; }   // r: k, c: n
+  };
   auto load_slab = [&](int k0) {
     const bool full = (m0 + BM <= M) && (n0 + BN <= N) && (k0 + kBK <= K);
     if (VEC && full) {
 #pragma unroll
       for (int i = 0; i < NA; ++i) {
-        const int r = tid / A_VPR + i * A_RPP, c = (tid % A_VPR) * 2;
-        ra[i] = *reinterpret_cast<const V2*>(A + (long)(m0 + r) * lda + k0 + c);
+        int r, c;
+        a_rc(i, r, c);
+        ra[i] = TA ? *reinterpret_cast<const V2*>(A + (long)(k0 + r) * lda + m0 + c)
+                   : *reinterpret_cast<const V2*>(A + (long)(m0 + r) * lda + k0 + c);
       }
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
-        const int r = tid / B_VPR + i * B_RPP, c = (tid % B_VPR) * 2;
-        rb[i] = *reinterpret_cast<const V2*>(B + (long)(k0 + r) * ldb + n0 + c);
+        int r, c;
+        b_rc(i, r, c);
+        rb[i] = TB ? *reinterpret_cast<const V2*>(B + (long)(n0 + r) * ldb + k0 + c)
+                   : *reinterpret_cast<const V2*>(B + (long)(k0 + r) * ldb + n0 + c);
       }
     } else {
 #pragma unroll
       for (int i = 0; i < NA; ++i) {
-        const int r = tid / A_VPR + i * A_RPP, c = (tid % A_VPR) * 2;
-        const int gr = m0 + r, gc = k0 + c;
-        const bool rin = gr < M;
-        ra[i].x = (rin && gc < K) ? A[(long)gr * lda + gc] : T(0);
-        ra[i].y = (rin && gc + 1 < K) ? A[(long)gr * lda + gc + 1] : T(0);
+        int r, c;
+        a_rc(i, r, c);
+        // stored row gr, stored columns gc, gc + 1 and their bounds
+        const int gr = TA ? k0 + r : m0 + r, gc = TA ? m0 + c : k0 + c;
+        const bool rin = gr < (TA ? K : M);
+        const int cl = TA ? M : K;
+        ra[i].x = (rin && gc < cl) ? A[(long)gr * lda + gc] : T(0);
+        ra[i].y = (rin && gc + 1 < cl) ? A[(long)gr * lda + gc + 1] : T(0);
       }
 #pragma unroll
       for (int i = 0; i < NB; ++i) {
-        const int r = tid / B_VPR + i * B_RPP, c = (tid % B_VPR) * 2;
-        const int gr = k0 + r, gc = n0 + c;
-        const bool rin = gr < K;
-        rb[i].x = (rin && gc < N) ? B[(long)gr * ldb + gc] : T(0);
-        rb[i].y = (rin && gc + 1 < N) ? B[(long)gr * ldb + gc + 1] : T(0);
+        int r, c;
+        b_rc(i, r, c);
+        const int gr = TB ? n0 + r : k0 + r, gc = TB ? k0 + c : n0 + c;
+        const bool rin = gr < (TB ? N : K);
+        const int cl = TB ? K : N;
+        rb[i].x = (rin && gc < cl) ? B[(long)gr * ldb + gc] : T(0);
+        rb[i].y = (rin && gc + 1 < cl) ? B[(long)gr * ldb + gc + 1] : T(0);
       }
     }
   };
   auto store_slab = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
-      const int r = tid / A_VPR + i * A_RPP, c = (tid % A_VPR) * 2;
-      As[buf][r * LDA_S + c] = ra[i].x;
-      As[buf][r * LDA_S + c + 1] = ra[i].y;
+      int r, c;
+      a_rc(i, r, c);
+      if (TA) {  // (k = r, m = c, c + 1) -> As[m][k]
+        As[buf][c * LDA_S + r] = ra[i].x;
+        As[buf][(c + 1) * LDA_S + r] = ra[i].y;
+      } else {
+        As[buf][r * LDA_S + c] = ra[i].x;
+        As[buf][r * LDA_S + c + 1] = ra[i].y;
+      }
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const int r = tid / B_VPR + i * B_RPP, c = (tid % B_VPR) * 2;
-      *reinterpret_cast<V2*>(&Bs[buf][r * LDB_S + c]) = rb[i];
+      int r, c;
+      b_rc(i, r, c);
+      if (TB) {  // (n = r, k = c, c + 1) -> Bs[k][n]
+        Bs[buf][c * LDB_S + r] = rb[i].x;
+        Bs[buf][(c + 1) * LDB_S + r] = rb[i].y;
+      } else {
+        *reinterpret_cast<V2*>(&Bs[buf][r * LDB_S + c]) = rb[i];
+      }
     }
   };
 
@@ -221,7 +258,7 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
       }
 }
 
-template <typename T, int BM, int BN, int WM, int WN, bool VEC>
+template <typename T, int BM, int BN, int WM, int WN, bool VEC, bool TA = false, bool TB = false>
 __global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, int K, T alpha,
                                                              const T* __restrict__ A, long lda,
                                                              const T* __restrict__ B, long ldb,
@@ -232,7 +269,7 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_mfma_kernel(int M, int N, in
   const int tiles_n = (N + BN - 1) / BN;
   int tm, tn;
   tile_coords(tiles_m, tiles_n, tm, tn);
-  gemm_tile<T, BM, BN, WM, WN, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, sh);
+  gemm_tile<T, BM, BN, WM, WN, VEC, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, sh);
 }
 
 // ---- grouped GEMM: many independent products of uneven sizes in one persistent launch ----
@@ -379,12 +416,14 @@ __global__ __launch_bounds__(256) void igemm_kernel(int M, int N, int K, uint32_
 
 template <typename T>
 bzg_status check_gemm_args(bzg_context* ctx, int64_t m, int64_t n, int64_t k, const T* A,
-                           int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc) {
+                           int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, bool ta = false,
+                           bool tb = false) {
   BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemm: null context");
   BZG_REQUIRE(m >= 0 && n >= 0 && k >= 0, BZG_ERR_INVALID_ARGUMENT, "gemm: negative size");
   BZG_REQUIRE(m < (1LL << 31) && n < (1LL << 31) && k < (1LL << 31), BZG_ERR_UNSUPPORTED,
               "gemm: dimension >= 2^31");
-  BZG_REQUIRE(lda >= (k > 0 ? k : 1) && ldb >= (n > 0 ? n : 1) && ldc >= (n > 0 ? n : 1),
+  const int64_t arow = ta ? m : k, brow = tb ? k : n;  // stored row lengths of A and B
+  BZG_REQUIRE(lda >= (arow > 0 ? arow : 1) && ldb >= (brow > 0 ? brow : 1) && ldc >= (n > 0 ? n : 1),
               BZG_ERR_INVALID_ARGUMENT, "gemm: leading dimension smaller than row length");
   if (m > 0 && n > 0) BZG_REQUIRE(C != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemm: null C");
   if (m > 0 && n > 0 && k > 0)
@@ -392,7 +431,7 @@ bzg_status check_gemm_args(bzg_context* ctx, int64_t m, int64_t n, int64_t k, co
   return BZG_OK;
 }
 
-template <typename T, int BM, int BN, int WM, int WN>
+template <typename T, int BM, int BN, int WM, int WN, bool TA = false, bool TB = false>
 bzg_status launch_tile(bzg_context* ctx, int m, int n, int k, T alpha, const T* A, long lda,
                        const T* B, long ldb, T beta, T* C, long ldc) {
   const bool vec = ((uintptr_t)A % (2 * sizeof(T)) == 0) && ((uintptr_t)B % (2 * sizeof(T)) == 0) &&
@@ -402,10 +441,10 @@ bzg_status launch_tile(bzg_context* ctx, int m, int n, int k, T alpha, const T* 
   BZG_REQUIRE(tiles < (1L << 31), BZG_ERR_UNSUPPORTED, "gemm: too many tiles");
   dim3 grid((unsigned)tiles), block(kThreads);
   if (vec)
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, true>), grid, block, 0, ctx->stream,
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, true, TA, TB>), grid, block, 0, ctx->stream,
                        m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   else
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, false>), grid, block, 0, ctx->stream,
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, BM, BN, WM, WN, false, TA, TB>), grid, block, 0, ctx->stream,
                        m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
@@ -443,14 +482,24 @@ int heuristic_shape(int64_t m, int64_t n) {
   return 0;
 }
 
+template <typename T, bool TA = false, bool TB = false>
+bzg_status launch_shape_t(bzg_context* ctx, int shape, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
+                          int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
+  if (shape == 2)
+    return launch_tile<T, 128, 128, 64, 64, TA, TB>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (shape == 1)
+    return launch_tile<T, 64, 64, 32, 32, TA, TB>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  return launch_tile<T, 32, 32, 16, 16, TA, TB>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
 template <typename T>
 bzg_status launch_shape(bzg_context* ctx, int shape, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
-                        int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
-  if (shape == 2)
-    return launch_tile<T, 128, 128, 64, 64>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-  if (shape == 1)
-    return launch_tile<T, 64, 64, 32, 32>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-  return launch_tile<T, 32, 32, 16, 16>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+                        int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, bool ta = false,
+                        bool tb = false) {
+  if (ta && tb) return launch_shape_t<T, true, true>(ctx, shape, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (ta) return launch_shape_t<T, true, false>(ctx, shape, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (tb) return launch_shape_t<T, false, true>(ctx, shape, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  return launch_shape_t<T, false, false>(ctx, shape, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 // Tile choice by size only (never by load), so a product's result never depends on what
@@ -458,8 +507,9 @@ bzg_status launch_shape(bzg_context* ctx, int shape, int64_t m, int64_t n, int64
 // (large tiles cut L2->LDS traffic; small tiles keep >= ~256 workgroups in flight).
 template <typename T>
 bzg_status gemm_dispatch(bzg_context* ctx, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
-                         int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
-  bzg_status st = check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+                         int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc, bool ta = false,
+                         bool tb = false) {
+  bzg_status st = check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc, ta, tb);
   if (st != BZG_OK) return st;
   if (m == 0 || n == 0) return BZG_OK;
   if (k == 0) {
@@ -475,7 +525,7 @@ bzg_status gemm_dispatch(bzg_context* ctx, int64_t m, int64_t n, int64_t k, T al
     auto it = g_tune.find(tune_key(sizeof(T) == 8 ? 0 : 1, m, n, k));
     if (it != g_tune.end()) shape = it->second;
   }
-  return launch_shape<T>(ctx, shape, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  return launch_shape<T>(ctx, shape, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ta, tb);
 }
 
 template <typename T>
@@ -575,6 +625,18 @@ bzg_status bzg_sgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, float al
                      const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
                      float* C, int64_t ldc) {
   return bzg::gemm_dispatch<float>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+bzg_status bzg_dgemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        double alpha, const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                        double* C, int64_t ldc) {
+  return bzg::gemm_dispatch<double>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, transa != 0, transb != 0);
+}
+
+bzg_status bzg_sgemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                        float* C, int64_t ldc) {
+  return bzg::gemm_dispatch<float>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, transa != 0, transb != 0);
 }
 
 bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,

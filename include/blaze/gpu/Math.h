@@ -122,18 +122,25 @@ template <typename T> class CompressedMatrix;
 // type dispatch onto the C ABI
 // ---------------------------------------------------------------------------------------
 namespace detail {
+// ta / tb: the operand is used transposed (stored k x m / n x k, leading dimension = its columns)
 inline bzg_status gemm(int64_t m, int64_t n, int64_t k, double al, const double* A, int64_t lda,
-                       const double* B, int64_t ldb, double be, double* C, int64_t ldc) {
-  if (fp64_accurate_flag() && al == 1.0 && be == 0.0 && k > 0)
+                       const double* B, int64_t ldb, double be, double* C, int64_t ldc, bool ta = false,
+                       bool tb = false) {
+  if (!ta && !tb && fp64_accurate_flag() && al == 1.0 && be == 0.0 && k > 0)
     return bzg_dgemm_fp64acc_i8(context(), m, n, k, A, lda, B, ldb, C, ldc, 0);
+  if (ta || tb) return bzg_dgemm_ex(context(), ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
   return bzg_dgemm(context(), m, n, k, al, A, lda, B, ldb, be, C, ldc);
 }
 inline bzg_status gemm(int64_t m, int64_t n, int64_t k, float al, const float* A, int64_t lda,
-                       const float* B, int64_t ldb, float be, float* C, int64_t ldc) {
+                       const float* B, int64_t ldb, float be, float* C, int64_t ldc, bool ta = false,
+                       bool tb = false) {
+  if (ta || tb) return bzg_sgemm_ex(context(), ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
   return bzg_sgemm(context(), m, n, k, al, A, lda, B, ldb, be, C, ldc);
 }
 inline bzg_status gemm(int64_t m, int64_t n, int64_t k, int al, const int* A, int64_t lda,
-                       const int* B, int64_t ldb, int be, int* C, int64_t ldc) {
+                       const int* B, int64_t ldb, int be, int* C, int64_t ldc, bool ta = false,
+                       bool tb = false) {
+  if (ta || tb) throw std::invalid_argument("trans(): int matrix products are row-major only in this build");
   return bzg_igemm(context(), m, n, k, al, A, lda, B, ldb, be, C, ldc);
 }
 inline bzg_status gemv(int64_t m, int64_t n, double al, const double* A, int64_t lda,
@@ -182,8 +189,15 @@ struct DMatDMatMultExpr {
   const DynamicMatrix<T>& lhs;
   const DynamicMatrix<T>& rhs;
   T scale;
-  size_t rows() const { return lhs.rows(); }
-  size_t columns() const { return rhs.columns(); }
+  bool ta = false, tb = false;  // lhs / rhs used transposed (trans(A) * B and friends)
+  size_t rows() const { return ta ? lhs.columns() : lhs.rows(); }
+  size_t columns() const { return tb ? rhs.rows() : rhs.columns(); }
+  size_t inner() const { return ta ? lhs.rows() : lhs.columns(); }
+};
+// trans(A): A^T as an operand of a product (DMatTransExpr; a column-major view of A's storage).
+template <typename T>
+struct DMatTransExpr {
+  const DynamicMatrix<T>& mat;
 };
 template <typename T>
 struct DMatDVecMultExpr {
@@ -333,16 +347,16 @@ class DynamicMatrix {
 
  private:
   void assign(const DMatDMatMultExpr<T>& e) {
-    const size_t m = e.lhs.rows(), k = e.lhs.columns(), n = e.rhs.columns();
+    const size_t m = e.rows(), k = e.inner(), n = e.columns();
     resize(m, n, false);
     if (m == 0 || n == 0) return;
-    detail::check(detail::gemm(m, n, k, e.scale, e.lhs.data(), std::max<size_t>(k, 1), e.rhs.data(),
-                               std::max<size_t>(n, 1), T(0), buf_.ptr, n),
+    detail::check(detail::gemm(m, n, k, e.scale, e.lhs.data(), std::max<size_t>(e.lhs.columns(), 1), e.rhs.data(),
+                               std::max<size_t>(e.rhs.columns(), 1), T(0), buf_.ptr, n, e.ta, e.tb),
                   "C = A * B");
     detail::check(bzg_synchronize(context()), "sync");
   }
   DynamicMatrix& accumulate(const DMatDMatMultExpr<T>& e, T sign) {
-    if (e.lhs.rows() != m_ || e.rhs.columns() != n_)
+    if (e.rows() != m_ || e.columns() != n_)
       throw std::invalid_argument("Matrix sizes do not match");
     if (e.lhs.isAliased(this) || e.rhs.isAliased(this)) {
       DynamicMatrix tmp(e);
@@ -353,10 +367,11 @@ class DynamicMatrix {
       upload(a.data());
       return *this;
     }
-    const size_t k = e.lhs.columns();
+    const size_t k = e.inner();
     if (m_ && n_)
-      detail::check(detail::gemm(m_, n_, k, sign * e.scale, e.lhs.data(), std::max<size_t>(k, 1),
-                                 e.rhs.data(), std::max<size_t>(n_, 1), T(1), buf_.ptr, n_),
+      detail::check(detail::gemm(m_, n_, k, sign * e.scale, e.lhs.data(), std::max<size_t>(e.lhs.columns(), 1),
+                                 e.rhs.data(), std::max<size_t>(e.rhs.columns(), 1), T(1), buf_.ptr, n_,
+                                 e.ta, e.tb),
                     "C += A * B");
     detail::check(bzg_synchronize(context()), "sync");
     return *this;
@@ -654,11 +669,32 @@ inline DMatDMatMultExpr<T> operator*(const DynamicMatrix<T>& A, const DynamicMat
 }
 template <typename T>
 inline DMatDMatMultExpr<T> operator*(T s, const DMatDMatMultExpr<T>& e) {
-  return {e.lhs, e.rhs, s * e.scale};
+  return {e.lhs, e.rhs, s * e.scale, e.ta, e.tb};
 }
 template <typename T>
 inline DMatDMatMultExpr<T> operator*(const DMatDMatMultExpr<T>& e, T s) {
-  return {e.lhs, e.rhs, e.scale * s};
+  return {e.lhs, e.rhs, e.scale * s, e.ta, e.tb};
+}
+template <typename T>
+inline DMatTransExpr<T> trans(const DynamicMatrix<T>& A) {
+  return {A};
+}
+// trans(A) * B, A * trans(B), trans(A) * trans(B) (TDMatDMatMultExpr, DMatTDMatMultExpr,
+// TDMatTDMatMultExpr): one bzg_*gemm_ex call on the stored operands, no transposed copies
+template <typename T>
+inline DMatDMatMultExpr<T> operator*(const DMatTransExpr<T>& A, const DynamicMatrix<T>& B) {
+  if (A.mat.rows() != B.rows()) throw std::invalid_argument("Matrix sizes do not match");
+  return {A.mat, B, T(1), true, false};
+}
+template <typename T>
+inline DMatDMatMultExpr<T> operator*(const DynamicMatrix<T>& A, const DMatTransExpr<T>& B) {
+  if (A.columns() != B.mat.columns()) throw std::invalid_argument("Matrix sizes do not match");
+  return {A, B.mat, T(1), false, true};
+}
+template <typename T>
+inline DMatDMatMultExpr<T> operator*(const DMatTransExpr<T>& A, const DMatTransExpr<T>& B) {
+  if (A.mat.rows() != B.mat.columns()) throw std::invalid_argument("Matrix sizes do not match");
+  return {A.mat, B.mat, T(1), true, true};
 }
 template <typename T>
 inline DMatDVecMultExpr<T> operator*(const DynamicMatrix<T>& A, const DynamicVector<T>& x) {

@@ -14,6 +14,8 @@
  * Reference interfaces replaced (paths relative to the Blaze source tree):
  *   bzg_dgemm / bzg_sgemm / bzg_igemm  <- DMatDMatMultExpr assign (blaze/math/expressions/
  *        DMatDMatMultExpr.h:516-569) and its BLAS seam cblas_dgemm (blaze/math/blas/cblas/gemm.h:161)
+ *   bzg_dgemm_ex / bzg_sgemm_ex        <- the same products over column-major (transposed) operands
+ *        (DMatTDMatMultExpr.h, TDMatDMatMultExpr.h, TDMatTDMatMultExpr.h; CBLAS transposes)
  *   bzg_dgemv / bzg_sgemv / bzg_igemv  <- DMatDVecMultExpr assign (DMatDVecMultExpr.h:408-458)
  *        and cblas_dgemv (blaze/math/blas/cblas/gemv.h:153)
  *   bzg_dcsrmv / bzg_scsrmv / bzg_icsrmv <- SMatDVecMultExpr element evaluation
@@ -101,6 +103,19 @@ bzg_status bzg_sgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, float al
 bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,
                      const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
                      int32_t beta, int32_t* C, int64_t ldc);
+/* C = alpha * op(A) * op(B) + beta * C with op(A) = A or A^T (transa != 0: A is stored k x m,
+ * lda >= m) and op(B) = B or B^T (transb != 0: B is stored n x k, ldb >= k).  Replaces the
+ * products over the other storage orders (DMatTDMatMultExpr.h, TDMatDMatMultExpr.h,
+ * TDMatTDMatMultExpr.h): a column-major m x k matrix is a row-major k x m one.  The
+ * transposed slabs are reordered on the way into LDS, so every form sums each C element in
+ * the same order as bzg_dgemm on explicitly transposed copies (bit-identical results). */
+bzg_status bzg_dgemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        double alpha, const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                        double* C, int64_t ldc);
+bzg_status bzg_sgemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+                        float* C, int64_t ldc);
+
 
 /* ---- grouped GEMM: many independent products of uneven sizes in one launch ------------
  * No single reference function: a batch of DMatDMatMultExpr assignments

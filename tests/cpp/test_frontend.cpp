@@ -234,6 +234,32 @@ int main() {
     for (size_t q = 0; q < n * r; ++q) ok = ok && relaxed(g[q], b[q]);
     check(ok, "solve(A,X,B) with 5 right-hand sides");
   }
+  // products over the other storage orders: trans(A) * B, A * trans(B), trans(A) * trans(B)
+  // (TDMatDMatMultExpr, DMatTDMatMultExpr, TDMatTDMatMultExpr) == the product of transposed copies
+  {
+    const size_t m = 37, n = 29, k = 45;
+    std::vector<double> a(k * m), b(n * k), at(m * k), bt(k * n), c0(m * n);
+    for (auto& v : a) v = U(rng);
+    for (auto& v : b) v = U(rng);
+    for (auto& v : c0) v = U(rng);
+    for (size_t i = 0; i < k; ++i)
+      for (size_t j = 0; j < m; ++j) at[j * k + i] = a[i * m + j];  // at = a^T (m x k)
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = 0; j < k; ++j) bt[j * n + i] = b[i * k + j];  // bt = b^T (k x n)
+    DynamicMatrix<double> A(k, m, a.data()), B(n, k, b.data()), At(m, k, at.data()), Bt(k, n, bt.data());
+    DynamicMatrix<double> Bk(k, n, bt.data()), Am(m, k, at.data());
+    DynamicMatrix<double> C1, C2, C3, R, C4(m, n, c0.data());
+    C1 = trans(A) * Bk;
+    C2 = Am * trans(B);
+    C3 = trans(A) * trans(B);
+    R = At * Bt;
+    C4 += 2.0 * (trans(A) * trans(B));
+    auto r = R.toVector(), g1 = C1.toVector(), g2 = C2.toVector(), g3 = C3.toVector(), g4 = C4.toVector();
+    bool ok = C1.rows() == m && C1.columns() == n && g1 == r && g2 == r && g3 == r;
+    for (size_t q = 0; q < m * n; ++q) ok = ok && relaxed(g4[q], c0[q] + 2.0 * r[q], 1e-12);
+    ok = ok && throws<std::invalid_argument>([&] { C1 = trans(A) * B; }, "Matrix sizes do not match");
+    check(ok, "trans(A)*B, A*trans(B), trans(A)*trans(B), C += s*(trans(A)*trans(B))");
+  }
   // set / insert (CompressedMatrix.h:2089-2135): random order, overwrite, duplicate insert
   {
     const size_t m = 50, n = 40;

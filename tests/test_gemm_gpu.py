@@ -159,3 +159,48 @@ def test_deterministic_repeat():
     A = rng.uniform(-1, 1, (700, 900)); B = rng.uniform(-1, 1, (900, 650))
     a = gpu_gemm(A, B); b = gpu_gemm(A, B)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_transposed_operands_bit_identical(dt):
+    # the four storage-order forms (DMatTDMatMultExpr / TDMatDMatMultExpr / TDMatTDMatMultExpr):
+    # bzg_*gemm_ex on transposed storage == bzg_*gemm on explicitly transposed copies, bit for
+    # bit (same k order), and within Higham's bound of the oracle
+    fn = "bzg_dgemm_ex" if dt == np.float64 else "bzg_sgemm_ex"
+    u = 2.0 ** -53 if dt == np.float64 else 2.0 ** -24
+    rng = np.random.default_rng(33)
+    for (m, n, k, pad) in [(1, 1, 1, 0), (7, 5, 3, 1), (32, 32, 16, 0), (65, 130, 47, 2), (300, 260, 129, 1),
+                           (1024, 1024, 256, 0)]:
+        A = rng.uniform(-1, 1, (m, k)).astype(dt)
+        B = rng.uniform(-1, 1, (k, n)).astype(dt)
+        ref = gpu_gemm(A, B)
+        want = oracle.gemm(A.astype(np.float64), B.astype(np.float64))
+        bound = 2 * k * u * (np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))) + 1e-300
+        assert np.all(np.abs(ref - want) <= bound)
+        for ta in (0, 1):
+            for tb in (0, 1):
+                As = np.zeros((k, m + pad), dt) if ta else np.zeros((m, k + pad), dt)
+                Bs = np.zeros((n, k + pad), dt) if tb else np.zeros((k, n + pad), dt)
+                if ta: As[:, :m] = A.T
+                else: As[:, :k] = A
+                if tb: Bs[:, :k] = B.T
+                else: Bs[:, :n] = B
+                dA, dB = dev(As), dev(Bs)
+                dC = full((m, n), dt, 3.0)
+                ctx().call(fn, ta, tb, m, n, k, 1.0, p(dA), As.shape[1], p(dB), Bs.shape[1], 0.0, p(dC), n)
+                got = host(dC)
+                assert np.array_equal(got, ref), (m, n, k, ta, tb)
+
+
+def test_transposed_alpha_beta_and_argument_checks():
+    rng = np.random.default_rng(34)
+    m, n, k = 50, 40, 30
+    A = rng.uniform(-1, 1, (k, m)); B = rng.uniform(-1, 1, (n, k)); C0 = rng.uniform(-1, 1, (m, n))
+    dA, dB, dC = dev(A), dev(B), dev(C0)
+    ctx().call("bzg_dgemm_ex", 1, 1, m, n, k, -2.0, p(dA), m, p(dB), k, 0.5, p(dC), n)
+    want = -2.0 * (A.T @ B.T) + 0.5 * C0
+    bound = (2 * k + 4) * 2.0 ** -53 * (2 * np.abs(A.T) @ np.abs(B.T) + 0.5 * np.abs(C0))
+    assert np.all(np.abs(host(dC) - want) <= bound)
+    import blaze_amd as bz
+    with pytest.raises(bz.BlazeGpuError):   # transposed A needs lda >= m
+        ctx().call("bzg_dgemm_ex", 1, 0, m, n, k, 1.0, p(dA), m - 1, p(dB), n, 0.0, p(dC), n)
