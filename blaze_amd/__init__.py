@@ -44,6 +44,8 @@ SIGNATURES = {
     "bzg_dtranspose": (i32, [vp, i64, i64, vp, i64, vp, i64]),
     "bzg_dgemm_ex": (i32, [vp, i32, i32, i64, i64, i64, f64, vp, i64, vp, i64, f64, vp, i64]),
     "bzg_sgemm_ex": (i32, [vp, i32, i32, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64]),
+    "bzg_dgemv_t": (i32, [vp, i64, i64, f64, vp, i64, vp, f64, vp]),
+    "bzg_sgemv_t": (i32, [vp, i64, i64, f32, vp, i64, vp, f32, vp]),
     "bzg_dgemv": (i32, [vp, i64, i64, f64, vp, i64, vp, f64, vp]),
     "bzg_sgemv": (i32, [vp, i64, i64, f32, vp, i64, vp, f32, vp]),
     "bzg_igemv": (i32, [vp, i64, i64, i32, vp, i64, vp, i32, vp]),

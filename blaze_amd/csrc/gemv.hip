@@ -10,6 +10,8 @@
 // run to run.  One load of x is reused for the R rows, which cuts L2 traffic for x by R.
 #include "common.hpp"
 
+#include <algorithm>
+
 namespace bzg {
 namespace {
 
@@ -145,6 +147,81 @@ bzg_status gemv_dispatch(bzg_context* ctx, int64_t m, int64_t n, T alpha, const 
   return launch_gemv<T, 1, 1>(ctx, m, n, alpha, A, lda, x, beta, y);
 }
 
+// Transposed GEMV, y = alpha * A^T * x + beta * y for a row-major m x n A (y has n entries):
+// the column-major form of the product (TDMatDVecMultExpr).  Stage 1: workgroup (cx, ry)
+// takes columns [512 cx, 512 cx + 512) -- two per thread, 16-byte loads along the stored
+// rows -- and the rows ry, ry + RY, ... ; x[i] is the same for the whole workgroup (a scalar
+// load).  Each thread keeps its two column sums in row order and writes them to
+// part[ry][j].  Stage 2 adds the RY partials of column j in index order, so the result is
+// identical run to run.
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_t_partial_kernel(int M, int N, const T* __restrict__ A, long lda,
+                                                             const T* __restrict__ x, T* __restrict__ part) {
+  using AR = Arith<T>;
+  const int j = (blockIdx.x * 256 + threadIdx.x) * 2;
+  const int RY = gridDim.y;
+  T s0 = T(0), s1 = T(0);
+  if (j < N) {
+    const bool pair = j + 1 < N;
+#pragma unroll 8
+    for (int i = blockIdx.y; i < M; i += RY) {
+      const T xi = x[i];
+      const T* row = A + (long)i * lda + j;
+      const T a0 = row[0];
+      const T a1 = pair ? row[1] : T(0);
+      s0 = AR::add(s0, AR::mul(a0, xi));
+      s1 = AR::add(s1, AR::mul(a1, xi));
+    }
+    part[(long)blockIdx.y * N + j] = s0;
+    if (pair) part[(long)blockIdx.y * N + j + 1] = s1;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void gemv_t_reduce_kernel(int N, int RY, T alpha, const T* __restrict__ part,
+                                                            T beta, T* __restrict__ y) {
+  using AR = Arith<T>;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= N) return;
+  T s = T(0);
+  for (int r = 0; r < RY; ++r) s = AR::add(s, part[(long)r * N + j]);
+  // beta == 0 never reads y (the BLAS convention of the non-transposed kernel)
+  y[j] = (beta == T(0)) ? AR::mul(alpha, s) : AR::add(AR::mul(alpha, s), AR::mul(beta, y[j]));
+}
+
+template <typename T>
+bzg_status gemv_t_dispatch(bzg_context* ctx, int64_t m, int64_t n, T alpha, const T* A, int64_t lda,
+                           const T* x, T beta, T* y) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemv: null context");
+  BZG_REQUIRE(m >= 0 && n >= 0, BZG_ERR_INVALID_ARGUMENT, "gemv: negative size");
+  BZG_REQUIRE(m < (1LL << 31) && n < (1LL << 31), BZG_ERR_UNSUPPORTED, "gemv: dimension >= 2^31");
+  BZG_REQUIRE(lda >= (n > 0 ? n : 1), BZG_ERR_INVALID_ARGUMENT, "gemv: lda < n");
+  if (n == 0) return BZG_OK;
+  BZG_REQUIRE(y != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemv: null y");
+  if (m == 0) {
+    hipLaunchKernelGGL(gemv_scale_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (int)n, beta, y);
+    BZG_LAUNCH_CHECK();
+    return BZG_OK;
+  }
+  BZG_REQUIRE(A != nullptr && x != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemv: null A or x");
+  // enough workgroups to fill the chip: row partitions for narrow matrices, at most one
+  // partition per 16 rows and never more than 512
+  const int64_t cx = (n + 511) / 512;
+  const int64_t ry = std::max<int64_t>(1, std::min<int64_t>({(int64_t)512, (m + 15) / 16,
+                                                            (2048 + cx - 1) / cx}));
+  void* w = nullptr;
+  bzg_status st = workspace(ctx, 11, (size_t)ry * n * sizeof(T), &w);
+  if (st) return st;
+  T* part = static_cast<T*>(w);
+  hipLaunchKernelGGL(gemv_t_partial_kernel<T>, dim3((unsigned)cx, (unsigned)ry), dim3(256), 0, ctx->stream, (int)m,
+                     (int)n, A, (long)lda, x, part);
+  hipLaunchKernelGGL(gemv_t_reduce_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
+                     (int)ry, alpha, part, beta, y);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
 // B = A^T for a row-major m x n A (lda) into n x m B (ldb): 64 x 64 tiles through LDS (one
 // padding element per row), reads and writes both along rows.
 __global__ __launch_bounds__(256) void transpose_kernel(int64_t m, int64_t n, const double* __restrict__ A,
@@ -182,6 +259,15 @@ bzg_status bzg_dtranspose(bzg_context* ctx, int64_t m, int64_t n, const double* 
                      0, ctx->stream, m, n, A, lda, B, ldb);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
+}
+
+bzg_status bzg_dgemv_t(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A, int64_t lda,
+                       const double* x, double beta, double* y) {
+  return bzg::gemv_t_dispatch<double>(ctx, m, n, alpha, A, lda, x, beta, y);
+}
+bzg_status bzg_sgemv_t(bzg_context* ctx, int64_t m, int64_t n, float alpha, const float* A, int64_t lda,
+                       const float* x, float beta, float* y) {
+  return bzg::gemv_t_dispatch<float>(ctx, m, n, alpha, A, lda, x, beta, y);
 }
 
 bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,

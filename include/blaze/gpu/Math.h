@@ -143,16 +143,20 @@ inline bzg_status gemm(int64_t m, int64_t n, int64_t k, int al, const int* A, in
   if (ta || tb) throw std::invalid_argument("trans(): int matrix products are row-major only in this build");
   return bzg_igemm(context(), m, n, k, al, A, lda, B, ldb, be, C, ldc);
 }
+// m x n = the STORED shape of A; ta: y = A^T x (y has n entries) instead of A x
 inline bzg_status gemv(int64_t m, int64_t n, double al, const double* A, int64_t lda,
-                       const double* x, double be, double* y) {
+                       const double* x, double be, double* y, bool ta = false) {
+  if (ta) return bzg_dgemv_t(context(), m, n, al, A, lda, x, be, y);
   return bzg_dgemv(context(), m, n, al, A, lda, x, be, y);
 }
 inline bzg_status gemv(int64_t m, int64_t n, float al, const float* A, int64_t lda,
-                       const float* x, float be, float* y) {
+                       const float* x, float be, float* y, bool ta = false) {
+  if (ta) return bzg_sgemv_t(context(), m, n, al, A, lda, x, be, y);
   return bzg_sgemv(context(), m, n, al, A, lda, x, be, y);
 }
 inline bzg_status gemv(int64_t m, int64_t n, int al, const int* A, int64_t lda, const int* x,
-                       int be, int* y) {
+                       int be, int* y, bool ta = false) {
+  if (ta) throw std::invalid_argument("trans(): int matrix products are row-major only in this build");
   return bzg_igemv(context(), m, n, al, A, lda, x, be, y);
 }
 inline bzg_status csrmv(const bzg_csr_plan* p, const int32_t* rp, const int32_t* ci,
@@ -204,7 +208,8 @@ struct DMatDVecMultExpr {
   const DynamicMatrix<T>& mat;
   const DynamicVector<T>& vec;
   T scale;
-  size_t size() const { return mat.rows(); }
+  bool ta = false;  // trans(A) * x (TDMatDVecMultExpr)
+  size_t size() const { return ta ? mat.columns() : mat.rows(); }
 };
 template <typename T>
 struct SMatDVecMultExpr {
@@ -472,7 +477,7 @@ class DynamicVector {
     return *this;
   }
   DynamicVector& operator+=(const DMatDVecMultExpr<T>& e) {
-    if (e.mat.rows() != n_) throw std::invalid_argument("Vector sizes do not match");
+    if (e.size() != n_) throw std::invalid_argument("Vector sizes do not match");
     if (e.vec.isAliased(this)) {
       DynamicVector tmp;
       tmp.assign(e);
@@ -482,8 +487,8 @@ class DynamicVector {
       return *this;
     }
     if (n_)
-      detail::check(detail::gemv(n_, e.mat.columns(), e.scale, e.mat.data(),
-                                 std::max<size_t>(e.mat.columns(), 1), e.vec.data(), T(1), buf_.ptr),
+      detail::check(detail::gemv(e.mat.rows(), e.mat.columns(), e.scale, e.mat.data(),
+                                 std::max<size_t>(e.mat.columns(), 1), e.vec.data(), T(1), buf_.ptr, e.ta),
                     "y += A * x");
     detail::check(bzg_synchronize(context()), "sync");
     return *this;
@@ -492,10 +497,10 @@ class DynamicVector {
 
   void assign(const DMatDVecMultExpr<T>& e) {
     const size_t m = e.mat.rows(), n = e.mat.columns();
-    resize(m, false);
-    if (m)
+    resize(e.size(), false);
+    if (e.size())
       detail::check(detail::gemv(m, n, e.scale, e.mat.data(), std::max<size_t>(n, 1), e.vec.data(), T(0),
-                                 buf_.ptr),
+                                 buf_.ptr, e.ta),
                     "y = A * x");
     detail::check(bzg_synchronize(context()), "sync");
   }
@@ -695,6 +700,11 @@ template <typename T>
 inline DMatDMatMultExpr<T> operator*(const DMatTransExpr<T>& A, const DMatTransExpr<T>& B) {
   if (A.mat.rows() != B.mat.columns()) throw std::invalid_argument("Matrix sizes do not match");
   return {A.mat, B.mat, T(1), true, true};
+}
+template <typename T>
+inline DMatDVecMultExpr<T> operator*(const DMatTransExpr<T>& A, const DynamicVector<T>& x) {
+  if (A.mat.rows() != x.size()) throw std::invalid_argument("Matrix and vector sizes do not match");
+  return {A.mat, x, T(1), true};
 }
 template <typename T>
 inline DMatDVecMultExpr<T> operator*(const DynamicMatrix<T>& A, const DynamicVector<T>& x) {

@@ -155,6 +155,14 @@ bzg_status bzg_gemm_tuning_clear(void);
  * to the column-major getrf (blaze/math/dense/LU.h:98-140, lapack/getrf.h). */
 bzg_status bzg_dtranspose(bzg_context* ctx, int64_t m, int64_t n, const double* A, int64_t lda, double* B,
                           int64_t ldb);
+/* y = alpha * A^T * x + beta * y for the row-major m x n A (x: m entries, y: n entries): the
+ * product with a column-major matrix (TDMatDVecMultExpr, blaze/math/expressions/
+ * TDMatDVecMultExpr.h; cblas_dgemv with a transpose).  Deterministic two-stage column sums.
+ * beta == 0 never reads y.  Scratch is cached in the context. */
+bzg_status bzg_dgemv_t(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A, int64_t lda,
+                       const double* x, double beta, double* y);
+bzg_status bzg_sgemv_t(bzg_context* ctx, int64_t m, int64_t n, float alpha, const float* A, int64_t lda,
+                       const float* x, float beta, float* y);
 bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,
                      int64_t lda, const double* x, double beta, double* y);
 bzg_status bzg_sgemv(bzg_context* ctx, int64_t m, int64_t n, float alpha, const float* A,

@@ -260,6 +260,26 @@ int main() {
     ok = ok && throws<std::invalid_argument>([&] { C1 = trans(A) * B; }, "Matrix sizes do not match");
     check(ok, "trans(A)*B, A*trans(B), trans(A)*trans(B), C += s*(trans(A)*trans(B))");
   }
+  // y = trans(A) * x (TDMatDVecMultExpr)
+  {
+    const size_t m = 300, n = 77;
+    std::vector<double> a(m * n), x(m);
+    for (auto& v : a) v = U(rng);
+    for (auto& v : x) v = U(rng);
+    DynamicMatrix<double> A(m, n, a.data());
+    DynamicVector<double> X(m, x.data()), Y;
+    Y = trans(A) * X;
+    auto g = Y.toVector();
+    bool ok = Y.size() == n;
+    for (size_t j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (size_t i = 0; i < m; ++i) s += a[i * n + j] * x[i];
+      ok = ok && relaxed(g[j], s, 1e-12);
+    }
+    ok = ok && throws<std::invalid_argument>([&] { DynamicVector<double> Z(n, 1.0); Y = trans(A) * Z; },
+                                             "Matrix and vector sizes do not match");
+    check(ok, "y = trans(A) * x");
+  }
   // set / insert (CompressedMatrix.h:2089-2135): random order, overwrite, duplicate insert
   {
     const size_t m = 50, n = 40;

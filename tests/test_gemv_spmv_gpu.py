@@ -229,3 +229,34 @@ def test_spmv_no_nonzeros_null_arrays():
         dy = full((1000,), dt, -5)
         ctx().call(fn, plan.h, p(drp), C.c_void_p(0), C.c_void_p(0), C.c_void_p(0), p(dy))
         assert not host(dy).any()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_gemv_transposed(dt):
+    # y = alpha A^T x + beta y (TDMatDVecMultExpr): within Higham's bound of the oracle's
+    # gemv on the transposed copy, deterministic, beta = 0 never reads y
+    fn = "bzg_dgemv_t" if dt == np.float64 else "bzg_sgemv_t"
+    u = 2.0 ** -53 if dt == np.float64 else 2.0 ** -24
+    rng = np.random.default_rng(35)
+    for (m, n, pad) in [(1, 1, 0), (3, 7, 1), (100, 513, 2), (4096, 300, 0), (20000, 1030, 1), (64, 20000, 0)]:
+        A = np.zeros((m, n + pad), dt); A[:, :n] = rng.uniform(-1, 1, (m, n))
+        x = rng.uniform(-1, 1, m).astype(dt)
+        y0 = rng.uniform(-1, 1, n).astype(dt)
+        dA, dx = dev(A), dev(x)
+        want = oracle.gemv(np.ascontiguousarray(A[:, :n].T).astype(np.float64), x.astype(np.float64))
+        absv = np.abs(A[:, :n].T).astype(np.float64) @ np.abs(x).astype(np.float64)
+        dy = full((n,), dt, float("nan"))
+        ctx().call(fn, m, n, 1.0, p(dA), n + pad, p(dx), 0.0, p(dy))
+        got = host(dy)
+        assert np.all(np.abs(got - want) <= (m + 2) * u * absv + 1e-300), (m, n)
+        dy2 = full((n,), dt, float("nan"))
+        ctx().call(fn, m, n, 1.0, p(dA), n + pad, p(dx), 0.0, p(dy2))
+        assert np.array_equal(host(dy2), got)
+        dy3 = dev(y0)
+        ctx().call(fn, m, n, -0.5, p(dA), n + pad, p(dx), 2.0, p(dy3))
+        w3 = -0.5 * want + 2.0 * y0.astype(np.float64)
+        assert np.all(np.abs(host(dy3) - w3) <= (m + 4) * u * (0.5 * absv + 2 * np.abs(y0)) + 1e-300)
+    # m = 0: y = beta y
+    dy = dev(np.ones(5, dt))
+    ctx().call(fn, 0, 5, 1.0, p(dev(np.zeros(5, dt))), 5, p(dev(np.zeros(1, dt))), 3.0, p(dy))
+    assert np.array_equal(host(dy), np.full(5, 3.0, dt))
