@@ -33,23 +33,28 @@ struct Arith<int32_t> {  // wrap-around arithmetic, like the unsigned reinterpre
 };
 
 template <typename T, int G, int R, bool VEC>
-__global__ __launch_bounds__(256) void gemv_kernel(int M, int N, T alpha, const T* __restrict__ A,
+__global__ __launch_bounds__(256) void gemv_kernel(int M, int N, int KC, T alpha, const T* __restrict__ A,
                                                    long lda, const T* __restrict__ x, T beta,
-                                                   T* __restrict__ y) {
+                                                   T* __restrict__ y, T* __restrict__ part) {
+  // Columns [c0, c1) of the rows: the whole row when the grid has one column chunk, else
+  // chunk blockIdx.y of KC columns (KC a multiple of VW), whose sums go to part[chunk][row].
   using AR = Arith<T>;
   constexpr int VW = 16 / sizeof(T);
-  constexpr int STEP = VW;  // elements per lane per load
   const int lane_in_group = threadIdx.x % G;
   const long group = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const long row0 = group * R;
   if (row0 >= M) return;  // the whole group exits together: no partial shuffles
+  const long c0l = (long)blockIdx.y * KC;
+  const int c0 = (int)c0l;
+  const int c1 = (int)min((long)N, c0l + KC);
   T acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = T(0);
 
+  int cs = c0;  // first column of the scalar pass
   if (VEC) {
-    const int nvec = N / VW;
-    for (int v = lane_in_group; v < nvec; v += G) {
+    const int vend = c1 / VW;  // c0 is a multiple of VW
+    for (int v = c0 / VW + lane_in_group; v < vend; v += G) {
       T xv[VW];
       *reinterpret_cast<i4v*>(xv) = *reinterpret_cast<const i4v*>(x + (long)v * VW);
 #pragma unroll
@@ -63,22 +68,14 @@ __global__ __launch_bounds__(256) void gemv_kernel(int M, int N, T alpha, const 
         }
       }
     }
-    // Tail columns (N not a multiple of VW).
-    for (int c = nvec * VW + lane_in_group; c < N; c += G) {
-      const T xc = x[c];
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        if (row0 + r < M) acc[r] = AR::add(acc[r], AR::mul(A[(row0 + r) * lda + c], xc));
-    }
-  } else {
-    for (int c = lane_in_group; c < N; c += G) {
-      const T xc = x[c];
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        if (row0 + r < M) acc[r] = AR::add(acc[r], AR::mul(A[(row0 + r) * lda + c], xc));
-    }
+    cs = vend * VW;  // tail columns (N not a multiple of VW)
   }
-  (void)STEP;
+  for (int c = cs + lane_in_group; c < c1; c += G) {
+    const T xc = x[c];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (row0 + r < M) acc[r] = AR::add(acc[r], AR::mul(A[(row0 + r) * lda + c], xc));
+  }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
 #pragma unroll
@@ -89,13 +86,61 @@ __global__ __launch_bounds__(256) void gemv_kernel(int M, int N, T alpha, const 
     for (int r = 0; r < R; ++r) {
       const long row = row0 + r;
       if (row < M) {
-        T v = AR::mul(alpha, acc[r]);
-        if (beta != T(0)) v = AR::add(v, AR::mul(beta, y[row]));
-        y[row] = v;
+        if (gridDim.y > 1) {
+          part[(long)blockIdx.y * M + row] = acc[r];
+        } else {
+          T v = AR::mul(alpha, acc[r]);
+          if (beta != T(0)) v = AR::add(v, AR::mul(beta, y[row]));
+          y[row] = v;
+        }
       }
     }
   }
 }
+
+// y[j] = alpha * (sum of the P partials part[r][j]) + beta * y[j]; beta == 0 never reads y
+// (BLAS's convention).  A workgroup is CW columns by L = 256 / CW lanes: lane l adds the
+// partials r = l, l + L, ... in order, then the L lane sums meet in a fixed pairwise tree
+// through LDS.  The shape (CW, L) depends on N and P only, so results are identical run to run.
+template <typename T>
+__global__ __launch_bounds__(256) void partials_reduce_kernel(int N, int P, int cw_log2, T alpha,
+                                                              const T* __restrict__ part, T beta,
+                                                              T* __restrict__ y) {
+  using AR = Arith<T>;
+  __shared__ T red[256];
+  const int CW = 1 << cw_log2, L = 256 >> cw_log2;
+  const int c = threadIdx.x & (CW - 1), l = threadIdx.x >> cw_log2;
+  const long j = (long)blockIdx.x * CW + c;
+  T s = T(0);
+  if (j < N) {
+#pragma unroll 8
+    for (int r = l; r < P; r += L) s = AR::add(s, part[(long)r * N + j]);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = L / 2; h >= 1; h >>= 1) {
+    if (l < h) red[threadIdx.x] = AR::add(red[threadIdx.x], red[threadIdx.x + h * CW]);
+    __syncthreads();
+  }
+  if (l == 0 && j < N) {
+    s = red[c];
+    y[j] = (beta == T(0)) ? AR::mul(alpha, s) : AR::add(AR::mul(alpha, s), AR::mul(beta, y[j]));
+  }
+}
+
+// Launch of partials_reduce_kernel: about 32 partials per lane.
+template <typename T>
+void launch_partials_reduce(bzg_context* ctx, int N, int P, T alpha, const T* part, T beta, T* y) {
+  int l_log2 = 0;
+  while (l_log2 < 8 && (32 << l_log2) < P) ++l_log2;
+  const int cw_log2 = 8 - l_log2;
+  const int cw = 1 << cw_log2;
+  hipLaunchKernelGGL(partials_reduce_kernel<T>, dim3((unsigned)((N + cw - 1) / cw)), dim3(256), 0, ctx->stream, N,
+                     P, cw_log2, alpha, part, beta, y);
+}
+
+constexpr int kGemvTargetGroups = 2048;  // workgroups that keep every CU streaming
+constexpr int kGemvMaxParts = 2048;      // partial sums per output entry
 
 template <typename T, int G, int R>
 bzg_status launch_gemv(bzg_context* ctx, int m, int n, T alpha, const T* A, long lda, const T* x,
@@ -106,12 +151,33 @@ bzg_status launch_gemv(bzg_context* ctx, int m, int n, T alpha, const T* A, long
   const long threads = groups * G;
   const long blocks = (threads + 255) / 256;
   BZG_REQUIRE(blocks < (1L << 31), BZG_ERR_UNSUPPORTED, "gemv: too many rows");
+  // Few rows, long rows: split the columns into chunks of at least 2048 (a multiple of a
+  // group's G*VW stride) so the grid still fills the chip; the chunk sums are then added in
+  // chunk order.  The split depends on m and n only, so results are identical run to run.
+  int kc = n, ks = 1;
+  if (blocks < kGemvTargetGroups / 2 && n >= 4096) {
+    const long want = std::min<long>({(long)kGemvMaxParts, (kGemvTargetGroups + blocks - 1) / blocks, n / 2048L});
+    if (want > 1) {
+      const long q = (long)G * VW;
+      kc = (int)(((n + want - 1) / want + q - 1) / q * q);
+      ks = (n + kc - 1) / kc;
+    }
+  }
+  T* part = nullptr;
+  if (ks > 1) {
+    void* w = nullptr;
+    bzg_status st = workspace(ctx, 11, (size_t)ks * m * sizeof(T), &w);
+    if (st) return st;
+    part = static_cast<T*>(w);
+  }
+  const dim3 grid((unsigned)blocks, (unsigned)ks);
   if (vec)
-    hipLaunchKernelGGL((gemv_kernel<T, G, R, true>), dim3((unsigned)blocks), dim3(256), 0,
-                       ctx->stream, m, n, alpha, A, lda, x, beta, y);
+    hipLaunchKernelGGL((gemv_kernel<T, G, R, true>), grid, dim3(256), 0, ctx->stream, m, n, kc, alpha, A, lda, x,
+                       beta, y, part);
   else
-    hipLaunchKernelGGL((gemv_kernel<T, G, R, false>), dim3((unsigned)blocks), dim3(256), 0,
-                       ctx->stream, m, n, alpha, A, lda, x, beta, y);
+    hipLaunchKernelGGL((gemv_kernel<T, G, R, false>), grid, dim3(256), 0, ctx->stream, m, n, kc, alpha, A, lda, x,
+                       beta, y, part);
+  if (ks > 1) launch_partials_reduce<T>(ctx, m, ks, alpha, part, beta, y);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }
@@ -148,45 +214,64 @@ bzg_status gemv_dispatch(bzg_context* ctx, int64_t m, int64_t n, T alpha, const 
 }
 
 // Transposed GEMV, y = alpha * A^T * x + beta * y for a row-major m x n A (y has n entries):
-// the column-major form of the product (TDMatDVecMultExpr).  Stage 1: workgroup (cx, ry)
-// takes columns [512 cx, 512 cx + 512) -- two per thread, 16-byte loads along the stored
-// rows -- and the rows ry, ry + RY, ... ; x[i] is the same for the whole workgroup (a scalar
-// load).  Each thread keeps its two column sums in row order and writes them to
-// part[ry][j].  Stage 2 adds the RY partials of column j in index order, so the result is
-// identical run to run.
-template <typename T>
+// the column-major form of the product (TDMatDVecMultExpr).  Stage 1: a workgroup is CT
+// column threads by RL = 256 / CT row lanes; column thread c owns VW adjacent columns (one
+// 16-byte load per row when aligned), workgroup (cx, ry) covers columns
+// [cx CT VW, (cx + 1) CT VW) and, in row lane l, the rows (ry RL + l) + k RY RL.  x[i] is the
+// same for every column thread of a row lane.  Each thread keeps its column sums in row order;
+// the RL row lanes are added in lane order through LDS and written to part[ry][j].  Stage 2
+// adds the RY partials of column j in index order, so the result is identical run to run.
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void gemv_t_partial_kernel(int M, int N, const T* __restrict__ A, long lda,
-                                                             const T* __restrict__ x, T* __restrict__ part) {
+                                                             const T* __restrict__ x, T* __restrict__ part,
+                                                             int ct_log2) {
   using AR = Arith<T>;
-  const int j = (blockIdx.x * 256 + threadIdx.x) * 2;
-  const int RY = gridDim.y;
-  T s0 = T(0), s1 = T(0);
+  constexpr int VW = 16 / sizeof(T);
+  __shared__ T red[256 * VW];
+  const int CT = 1 << ct_log2, RL = 256 >> ct_log2;
+  const int ct = threadIdx.x & (CT - 1), rl = threadIdx.x >> ct_log2;
+  const long j = ((long)blockIdx.x * CT + ct) * VW;
+  const long step = (long)gridDim.y * RL;
+  T s[VW];
+#pragma unroll
+  for (int e = 0; e < VW; ++e) s[e] = T(0);
   if (j < N) {
-    const bool pair = j + 1 < N;
+    if (VEC && j + VW <= N) {
 #pragma unroll 8
-    for (int i = blockIdx.y; i < M; i += RY) {
-      const T xi = x[i];
-      const T* row = A + (long)i * lda + j;
-      const T a0 = row[0];
-      const T a1 = pair ? row[1] : T(0);
-      s0 = AR::add(s0, AR::mul(a0, xi));
-      s1 = AR::add(s1, AR::mul(a1, xi));
+      for (long i = (long)blockIdx.y * RL + rl; i < M; i += step) {
+        const T xi = x[i];
+        T av[VW];
+        *reinterpret_cast<i4v*>(av) = __builtin_nontemporal_load(reinterpret_cast<const i4v*>(A + i * lda + j));
+#pragma unroll
+        for (int e = 0; e < VW; ++e) s[e] = AR::add(s[e], AR::mul(av[e], xi));
+      }
+    } else {
+      const int w = (int)min((long)VW, N - j);
+#pragma unroll 4
+      for (long i = (long)blockIdx.y * RL + rl; i < M; i += step) {
+        const T xi = x[i];
+        const T* row = A + i * lda + j;
+#pragma unroll
+        for (int e = 0; e < VW; ++e)
+          if (e < w) s[e] = AR::add(s[e], AR::mul(row[e], xi));
+      }
     }
-    part[(long)blockIdx.y * N + j] = s0;
-    if (pair) part[(long)blockIdx.y * N + j + 1] = s1;
   }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void gemv_t_reduce_kernel(int N, int RY, T alpha, const T* __restrict__ part,
-                                                            T beta, T* __restrict__ y) {
-  using AR = Arith<T>;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= N) return;
-  T s = T(0);
-  for (int r = 0; r < RY; ++r) s = AR::add(s, part[(long)r * N + j]);
-  // beta == 0 never reads y (the BLAS convention of the non-transposed kernel)
-  y[j] = (beta == T(0)) ? AR::mul(alpha, s) : AR::add(AR::mul(alpha, s), AR::mul(beta, y[j]));
+  if (RL > 1) {
+#pragma unroll
+    for (int e = 0; e < VW; ++e) red[threadIdx.x * VW + e] = s[e];
+    __syncthreads();
+    if (rl == 0) {
+      for (int l = 1; l < RL; ++l)
+#pragma unroll
+        for (int e = 0; e < VW; ++e) s[e] = AR::add(s[e], red[(l * CT + ct) * VW + e]);
+    }
+  }
+  if (rl == 0) {
+#pragma unroll
+    for (int e = 0; e < VW; ++e)
+      if (j + e < N) part[(long)blockIdx.y * N + j + e] = s[e];
+  }
 }
 
 template <typename T>
@@ -205,19 +290,29 @@ bzg_status gemv_t_dispatch(bzg_context* ctx, int64_t m, int64_t n, T alpha, cons
     return BZG_OK;
   }
   BZG_REQUIRE(A != nullptr && x != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemv: null A or x");
-  // enough workgroups to fill the chip: row partitions for narrow matrices, at most one
-  // partition per 16 rows and never more than 512
-  const int64_t cx = (n + 511) / 512;
-  const int64_t ry = std::max<int64_t>(1, std::min<int64_t>({(int64_t)512, (m + 15) / 16,
-                                                            (2048 + cx - 1) / cx}));
+  constexpr int VW = 16 / sizeof(T);
+  const bool vec = ((uintptr_t)A % 16 == 0) && (lda % VW == 0);
+  // column threads: the next power of two covering the row, at most 256 (the workgroup)
+  int ct_log2 = 0;
+  while (ct_log2 < 8 && ((int64_t)VW << ct_log2) < n) ++ct_log2;
+  const int64_t ct = 1 << ct_log2, rl = 256 / ct;
+  const int64_t cx = (n + ct * VW - 1) / (ct * VW);
+  // row partitions: enough workgroups to fill the chip, at least 16 rows per row lane
+  const int64_t ry = std::max<int64_t>(1, std::min<int64_t>({(int64_t)kGemvMaxParts, (m + 16 * rl - 1) / (16 * rl),
+                                                            (kGemvTargetGroups + cx - 1) / cx}));
+  BZG_REQUIRE(cx < (1LL << 31), BZG_ERR_UNSUPPORTED, "gemv: too many columns");
   void* w = nullptr;
   bzg_status st = workspace(ctx, 11, (size_t)ry * n * sizeof(T), &w);
   if (st) return st;
   T* part = static_cast<T*>(w);
-  hipLaunchKernelGGL(gemv_t_partial_kernel<T>, dim3((unsigned)cx, (unsigned)ry), dim3(256), 0, ctx->stream, (int)m,
-                     (int)n, A, (long)lda, x, part);
-  hipLaunchKernelGGL(gemv_t_reduce_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
-                     (int)ry, alpha, part, beta, y);
+  const dim3 grid((unsigned)cx, (unsigned)ry);
+  if (vec)
+    hipLaunchKernelGGL((gemv_t_partial_kernel<T, true>), grid, dim3(256), 0, ctx->stream, (int)m, (int)n, A,
+                       (long)lda, x, part, ct_log2);
+  else
+    hipLaunchKernelGGL((gemv_t_partial_kernel<T, false>), grid, dim3(256), 0, ctx->stream, (int)m, (int)n, A,
+                       (long)lda, x, part, ct_log2);
+  launch_partials_reduce<T>(ctx, (int)n, (int)ry, alpha, part, beta, y);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }

@@ -65,6 +65,29 @@ def test_gemv_alpha_beta():
     assert np.allclose(got, -1.5 * (A @ x) + 2.0 * y0, rtol=1e-13, atol=1e-13)
 
 
+def test_gemv_wide_column_split():
+    # few rows, long rows: the columns run as chunks whose sums are added in chunk order
+    # (aligned and unaligned leading dimensions, tails shorter than a vector, alpha/beta,
+    # int32 wrap-around exact, identical results run to run)
+    rng = np.random.default_rng(23)
+    for (m, n, lda) in [(1, 4096, 4096), (64, 100000, 100000), (64, 100000, 100003), (700, 9000, 9001),
+                        (17, 65537, 65540)]:
+        A = rng.uniform(-1, 1, (m, n)); x = rng.uniform(-1, 1, n); y0 = rng.uniform(-1, 1, m)
+        absv = np.abs(A) @ np.abs(x)
+        got = gpu_gemv(A, x, lda=lda)
+        assert np.all(np.abs(got - A @ x) <= (n + 2) * 2.0 ** -53 * absv + 1e-300), (m, n, lda)
+        assert np.array_equal(gpu_gemv(A, x, lda=lda), got)
+        g2 = gpu_gemv(A, x, alpha=0.5, beta=-2.0, y0=y0, lda=lda)
+        assert np.all(np.abs(g2 - (0.5 * (A @ x) - 2.0 * y0)) <= (n + 4) * 2.0 ** -53 * (absv + 2 * np.abs(y0)))
+        Af = A.astype(np.float32); xf = x.astype(np.float32)
+        gf = gpu_gemv(Af, xf, lda=lda)
+        bf = (n + 2) * 2.0 ** -24 * (np.abs(Af).astype(np.float64) @ np.abs(xf))
+        assert np.all(np.abs(gf - Af.astype(np.float64) @ xf) <= bf), (m, n, lda)
+        Ai = rng.integers(-2 ** 31, 2 ** 31, (m, n), dtype=np.int64).astype(np.int32)
+        xi = rng.integers(-2 ** 31, 2 ** 31, n, dtype=np.int64).astype(np.int32)
+        assert np.array_equal(gpu_gemv(Ai, xi, lda=lda), oracle.gemv(Ai, xi)), (m, n, lda)
+
+
 # ---- SpMV ---------------------------------------------------------------------------------
 class Plan:
     def __init__(self, rowptr, n_cols):
@@ -238,7 +261,8 @@ def test_gemv_transposed(dt):
     fn = "bzg_dgemv_t" if dt == np.float64 else "bzg_sgemv_t"
     u = 2.0 ** -53 if dt == np.float64 else 2.0 ** -24
     rng = np.random.default_rng(35)
-    for (m, n, pad) in [(1, 1, 0), (3, 7, 1), (100, 513, 2), (4096, 300, 0), (20000, 1030, 1), (64, 20000, 0)]:
+    for (m, n, pad) in [(1, 1, 0), (3, 7, 1), (100, 513, 2), (4096, 300, 0), (20000, 1030, 1), (64, 20000, 0),
+                        (100000, 1, 0), (50000, 3, 1), (30000, 64, 0), (30000, 65, 0), (9000, 130, 3)]:
         A = np.zeros((m, n + pad), dt); A[:, :n] = rng.uniform(-1, 1, (m, n))
         x = rng.uniform(-1, 1, m).astype(dt)
         y0 = rng.uniform(-1, 1, n).astype(dt)
