@@ -383,6 +383,51 @@ class Bench:
                 "timing": ("us/gbps: host-launched calls (ctypes); kernel_*: HIP events around each call; "
                            "graph_* and frac_hbm: calls replayed back to back from a HIP graph")}
 
+    # -- dense GEMV, y = A * x and y = trans(A) * x (DMatDVecMultExpr / TDMatDVecMultExpr) ------
+    def gemv_leg(self, m=16384, n=16384, reps=5):
+        """Row-major m x n A; calls replayed back to back from a HIP graph (two copies of A, so
+        the 2 GiB FP64 working set is well beyond the 256 MB MALL), timed with events.  Bytes per
+        call: m n + m + n elements (A once, x, y)."""
+        torch, bz = self.torch, self.bz
+        out = {}
+        for dt, pre in ((torch.float64, "d"), (torch.float32, "s")):
+            mats = [torch.empty(m, n, dtype=dt, device="cuda").uniform_(-1, 1) for _ in range(2)]
+            xs = {False: torch.empty(n, dtype=dt, device="cuda").uniform_(-1, 1),
+                  True: torch.empty(m, dtype=dt, device="cuda").uniform_(-1, 1)}
+            ys = {False: torch.empty(m, dtype=dt, device="cuda"), True: torch.empty(n, dtype=dt, device="cuda")}
+            by = (m * n + m + n) * mats[0].element_size()
+            for tr in (False, True):
+                fn = f"bzg_{pre}gemv" + ("_t" if tr else "")
+                s = torch.cuda.Stream()
+                gctx = bz.Context(stream=s.cuda_stream)
+                calls = 16
+
+                def many():
+                    for i in range(calls):
+                        gctx.call(fn, m, n, 1.0, vp(mats[i % 2]), n, vp(xs[tr]), 0.0, vp(ys[tr]))
+                with torch.cuda.stream(s):
+                    many()
+                torch.cuda.synchronize()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=s):
+                    many()
+                g.replay()
+                torch.cuda.synchronize()
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                ev[0].record()
+                for _ in range(reps):
+                    g.replay()
+                ev[1].record()
+                torch.cuda.synchronize()
+                us = ev[0].elapsed_time(ev[1]) * 1e3 / (reps * calls)
+                del g
+                gctx.close()
+                key = f"{'trans_' if tr else ''}{pre}gemv_{m}x{n}"
+                out[key] = {"us": us, "gbps": by / us / 1e3, "frac_hbm": by / us / 1e3 / PEAK["hbm"], "bytes": by}
+            del mats
+        out["timing"] = "calls replayed back to back from a HIP graph, events around the replays"
+        return out
+
     # -- conjugate gradients (the reference's CG benchmark loop, blazemark CG.cpp) -------------
     def cg_leg(self, N=2048, iterations=50, reps=5):
         """5-point matrix N^2 x N^2, b = 0, random start (as blazemark); the whole bzg_dcg call
@@ -527,8 +572,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--systems", type=int, default=100_000)
-    ap.add_argument("--legs", default="gemm,solve,spmv,cg,serve,cpu",
-                    help="comma list of gemm, solve, spmv, cg, serve, cpu, copy (1 GiB copy for counter calibration)")
+    ap.add_argument("--legs", default="gemm,solve,spmv,gemv,cg,serve,cpu",
+                    help="comma list of gemm, solve, spmv, gemv, cg, serve, cpu, copy (1 GiB copy for counter "
+                         "calibration)")
     ap.add_argument("--variant", default="auto", choices=["auto", "bf16", "int8"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -557,6 +603,12 @@ def main():
             log(f"[spmv] {json.dumps(rec['spmv'])}")
         except Exception as e:
             rec["spmv"] = {"error": repr(e)}
+    if "gemv" in legs:
+        try:
+            rec["gemv"] = b.gemv_leg()
+            log(f"[gemv] {json.dumps(rec['gemv'])}")
+        except Exception as e:
+            rec["gemv"] = {"error": repr(e)}
     if "cg" in legs:
         try:
             rec["cg"] = b.cg_leg()

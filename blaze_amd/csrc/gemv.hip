@@ -205,9 +205,12 @@ bzg_status gemv_dispatch(bzg_context* ctx, int64_t m, int64_t n, T alpha, const 
   }
   BZG_REQUIRE(A != nullptr && x != nullptr, BZG_ERR_INVALID_ARGUMENT, "gemv: null A or x");
   // Lanes per row from the row length: a full wavefront for long rows, fewer lanes (and
-  // more rows per wavefront) for short ones, so no lane idles on a short row.
+  // more rows per wavefront) for short ones, so no lane idles on a short row.  Long rows of
+  // double run two rows per wavefront (twice the workgroups; 6.9 against 6.2 TB/s with four
+  // at 16384^2), 4-byte types four (wide FP32 matrices lose up to 12 % with two).
   constexpr int VW = 16 / sizeof(T);
-  if (n >= 64 * VW * 2) return launch_gemv<T, 64, 4>(ctx, m, n, alpha, A, lda, x, beta, y);
+  constexpr int RLONG = sizeof(T) == 8 ? 2 : 4;
+  if (n >= 64 * VW * 2) return launch_gemv<T, 64, RLONG>(ctx, m, n, alpha, A, lda, x, beta, y);
   if (n >= 16 * VW * 2) return launch_gemv<T, 16, 2>(ctx, m, n, alpha, A, lda, x, beta, y);
   if (n >= 4 * VW) return launch_gemv<T, 4, 1>(ctx, m, n, alpha, A, lda, x, beta, y);
   return launch_gemv<T, 1, 1>(ctx, m, n, alpha, A, lda, x, beta, y);
