@@ -75,6 +75,7 @@ SIGNATURES = {
     "bzg_submit_gemm": (i32, [vp, i64, i64, i64, vp, i64, vp, i64, C.POINTER(i64)]),
     "bzg_submit_spmv": (i32, [vp, i32, vp, C.POINTER(i64)]),
     "bzg_submit_solve": (i32, [vp, i32, i32, vp, vp, C.POINTER(i64)]),
+    "bzg_server_hold": (i32, [vp, i32]),
     "bzg_poll": (i32, [vp, i64, C.POINTER(i32)]),
     "bzg_wait": (i32, [vp, i64, vp]),
     "bzg_release": (i32, [vp, i64]),

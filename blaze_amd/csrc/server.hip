@@ -165,6 +165,7 @@ struct bzg_server {
   int64_t next_id = 1;
   bzg_server_stats st{};
   bool stop = false;
+  bool hold = false;  // admissions paused by bzg_server_hold (mu)
   bzg_status fatal = BZG_OK;
   std::thread th;
 };
@@ -351,6 +352,7 @@ void dispatcher(bzg_server* s) {
   (void)hipSetDevice(s->device);
   for (;;) {
     bool progressed = false;
+    bool held = false;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       while (!s->incoming.empty()) {
@@ -358,6 +360,7 @@ void dispatcher(bzg_server* s) {
         s->incoming.pop_front();
       }
       if (s->stop && s->pending.empty() && s->inflight.empty()) break;
+      held = s->hold && !s->stop;  // destroy admits and finishes everything
     }
     // ---- completions ----
     for (size_t gi = 0; gi < s->inflight.size();) {
@@ -400,7 +403,7 @@ void dispatcher(bzg_server* s) {
       progressed = true;
     }
     // ---- admissions ----
-    while (!s->pending.empty() && (int)s->inflight.size() < s->cfg.max_inflight) {
+    while (!held && !s->pending.empty() && (int)s->inflight.size() < s->cfg.max_inflight) {
       Request* head = s->pending.front();
       std::unique_lock<std::mutex> lk(s->mu);
       const size_t need = request_bytes(s, head);
@@ -682,6 +685,16 @@ bzg_status bzg_poll(bzg_server* s, int64_t id, int32_t* done) {
   auto it = s->reqs.find(id);
   BZG_REQUIRE(it != s->reqs.end(), BZG_ERR_INVALID_ARGUMENT, "poll: unknown request");
   *done = it->second->done ? 1 : 0;
+  return BZG_OK;
+}
+
+bzg_status bzg_server_hold(bzg_server* s, int32_t hold) {
+  BZG_REQUIRE(s != nullptr, BZG_ERR_INVALID_ARGUMENT, "server_hold: null server");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->hold = hold != 0;
+  }
+  s->cv_work.notify_one();
   return BZG_OK;
 }
 

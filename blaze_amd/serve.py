@@ -71,6 +71,10 @@ class Server:
                                                                C.byref(rid)))
         return rid.value
 
+    def hold(self, on: bool) -> None:
+        """Queue without admitting while on; lifting the hold admits the queued burst at once."""
+        bz.check("bzg_server_hold", bz.lib().bzg_server_hold(self.h, 1 if on else 0))
+
     def poll(self, rid: int) -> bool:
         d = C.c_int32()
         bz.check("bzg_poll", bz.lib().bzg_poll(self.h, rid, C.byref(d)))

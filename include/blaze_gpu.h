@@ -364,6 +364,11 @@ bzg_status bzg_submit_spmv(bzg_server* server, int32_t csr, const double* x, int
  * 1 <= nrhs <= 8. */
 bzg_status bzg_submit_solve(bzg_server* server, int32_t n, int32_t nrhs, const double* A,
                             const double* B, int64_t* id);
+/* hold != 0: requests are still accepted and queued but none is admitted until a call with
+ * hold = 0, so a burst submitted in between is batched as if it had arrived at once (all of its
+ * solves, and all of its small GEMMs, join one launch each as far as the budget allows).  A
+ * wait on a held request returns only after another thread lifts the hold; destroy lifts it. */
+bzg_status bzg_server_hold(bzg_server* server, int32_t hold);
 bzg_status bzg_poll(bzg_server* server, int64_t id, int32_t* done);
 /* Blocks until the request has completed and describes it. */
 bzg_status bzg_wait(bzg_server* server, int64_t id, bzg_request_result* out);

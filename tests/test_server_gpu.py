@@ -104,10 +104,12 @@ def test_results_identical_alone_and_in_mix():
                 ids[i] = _submit(srv, reqs[i])
 
         th = [threading.Thread(target=client, args=(order[j::2],)) for j in range(2)]
+        srv.hold(True)  # the whole burst is queued before anything is admitted
         for t in th:
             t.start()
         for t in th:
             t.join()
+        srv.hold(False)
         mixed = []
         for i, q in enumerate(reqs):
             res = srv.wait(ids[i])
@@ -138,7 +140,9 @@ def test_results_identical_alone_and_in_mix():
                 assert be <= 1e-12, (shape, be)
     finally:
         srv.close()
-    assert st["solve_batches"] >= 1 and st["max_solve_batch"] >= 2
+    # the held burst's solves formed one batch (the alone pass ran them one by one)
+    n_solve = sum(1 for q in reqs if q[0] == "solve")
+    assert n_solve >= 2 and st["max_solve_batch"] == n_solve
 
 
 def test_budget_admission_and_release():
