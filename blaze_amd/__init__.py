@@ -61,6 +61,8 @@ SIGNATURES = {
     "bzg_dgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrf_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrs_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bzg_ddet_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bzg_dinv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dsgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, i32, vp]),
     "bzg_batched_order": (i32, [vp, i64, vp, vp]),
     "bzg_solver_autotune": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, i64, vp, i64, vp, C.POINTER(i32), C.POINTER(f32)]),

@@ -868,10 +868,160 @@ bzg_status lu_batched(bzg_context* ctx, int mode, int64_t count, const int32_t* 
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }
+
+// Determinants from the factors (one thread per system), detNxN's rule
+// (blaze/math/expressions/DMatDetExpr.h:345-357): 0 when getrf reported a zero pivot, else the
+// product, left to right from 1, of U's diagonal with the sign flipped at every row interchange.
+// An empty system has determinant 0 (DMatDetExpr.h:400-402) and info 0; an order outside
+// 1..NMAX keeps info -1 and gets NaN.
+__global__ __launch_bounds__(256) void det_from_lu_kernel(int64_t count, const int32_t* __restrict__ n,
+                                                          const int64_t* __restrict__ a_off,
+                                                          const int64_t* __restrict__ p_off,
+                                                          const double* __restrict__ LU,
+                                                          const int32_t* __restrict__ ipiv,
+                                                          int32_t* __restrict__ info, double* __restrict__ det) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  const int ni = n[i];
+  if (ni == 0) {
+    det[i] = 0.0;
+    info[i] = 0;
+    return;
+  }
+  if (ni < 0 || ni > NMAX || info[i] < 0) {
+    det[i] = __builtin_nan("");
+    return;
+  }
+  if (info[i] > 0) {
+    det[i] = 0.0;
+    return;
+  }
+  const double* a = LU + a_off[i];
+  const int32_t* p = ipiv + p_off[i];
+  double d = 1.0;
+  for (int k = 0; k < ni; ++k) {
+    const double u = a[(int64_t)k * (ni + 1)];
+    d *= (p[k] == k + 1) ? u : -u;
+  }
+  det[i] = d;
+}
+
+// Identity columns [col0, col0 + nr) of each system, nr = min(8, n_i - col0), as the row-major
+// n_i x nr right-hand side at B + b_off[i]; nrhs[i] = nr, or 0 for a system that is done, empty,
+// out of range or singular.  On the first chunk an empty system gets info 0 (dgetri's n = 0).
+__global__ __launch_bounds__(256) void inv_rhs_kernel(int64_t count, int col0, const int32_t* __restrict__ n,
+                                                      int32_t* __restrict__ info, const int64_t* __restrict__ b_off,
+                                                      double* __restrict__ B, int32_t* __restrict__ nrhs) {
+  const int64_t i = blockIdx.x;
+  if (i >= count) return;
+  const int ni = n[i];
+  if (col0 == 0 && ni == 0 && threadIdx.x == 0) info[i] = 0;
+  const bool live = ni > col0 && ni <= NMAX && info[i] == 0;
+  const int nr = live ? min(8, ni - col0) : 0;
+  if (threadIdx.x == 0) nrhs[i] = nr;
+  double* b = B + b_off[i];
+  for (int e = threadIdx.x; e < ni * nr; e += 256) {
+    const int r = e / nr, q = e - r * nr;
+    b[e] = (r == col0 + q) ? 1.0 : 0.0;
+  }
+}
+
+// A_i[:, col0 + q] = X_i[:, q] for the chunk's solutions (row-major n_i x nrhs[i] at B + b_off[i]).
+__global__ __launch_bounds__(256) void inv_scatter_kernel(int64_t count, int col0, const int32_t* __restrict__ n,
+                                                          const int32_t* __restrict__ nrhs,
+                                                          const int64_t* __restrict__ a_off,
+                                                          const int64_t* __restrict__ b_off,
+                                                          const double* __restrict__ B, double* __restrict__ A) {
+  const int64_t i = blockIdx.x;
+  if (i >= count) return;
+  const int nr = nrhs[i];
+  if (nr == 0) return;
+  const int ni = n[i];
+  const double* b = B + b_off[i];
+  double* a = A + a_off[i];
+  for (int e = threadIdx.x; e < ni * nr; e += 256) {
+    const int r = e / nr, q = e - r * nr;
+    a[(int64_t)r * ni + col0 + q] = b[e];
+  }
+}
 }  // namespace
 }  // namespace bzg
 
 extern "C" {
+
+bzg_status bzg_dinv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+                            const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info, const int32_t* order) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dinv_batched: null context");
+  BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "dinv_batched: bad count");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(n && a_off && p_off && A && ipiv && info, BZG_ERR_INVALID_ARGUMENT, "dinv_batched: null array");
+  // orders and offsets on the host (one synchronisation) to size the scratch
+  std::vector<int32_t> hn((size_t)count);
+  std::vector<int64_t> ha((size_t)count);
+  BZG_HIP_CHECK(hipMemcpyAsync(hn.data(), n, (size_t)count * 4, hipMemcpyDeviceToHost, ctx->stream));
+  BZG_HIP_CHECK(hipMemcpyAsync(ha.data(), a_off, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
+  BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  int maxn = 0;
+  int64_t extent = 0, bsz = 0;
+  std::vector<int64_t> hb((size_t)count, 0);
+  for (int64_t i = 0; i < count; ++i) {
+    hb[(size_t)i] = bsz;
+    if (hn[(size_t)i] >= 1 && hn[(size_t)i] <= bzg::NMAX) {
+      BZG_REQUIRE(ha[(size_t)i] >= 0, BZG_ERR_INVALID_ARGUMENT, "dinv_batched: negative offset");
+      maxn = std::max(maxn, hn[(size_t)i]);
+      extent = std::max<int64_t>(extent, ha[(size_t)i] + (int64_t)hn[(size_t)i] * hn[(size_t)i]);
+      bsz += (int64_t)hn[(size_t)i] * 8;
+    }
+  }
+  // A_i := P L U in place (dgetrf)
+  bzg_status st = bzg::lu_batched(ctx, 1, count, n, nullptr, a_off, nullptr, p_off, A, nullptr, ipiv, info, order);
+  if (st) return st;
+  // scratch (slot 9): a copy of the factors, the right-hand-side chunks, b_off, nrhs, getrs info
+  const size_t lu_b = (size_t)extent * 8, rhs_b = (size_t)std::max<int64_t>(bsz, 1) * 8;
+  const size_t off_b = (size_t)count * 8, cnt_b = ((size_t)count * 4 + 7) / 8 * 8;
+  void* w = nullptr;
+  st = bzg::workspace(ctx, 9, lu_b + rhs_b + off_b + 2 * cnt_b, &w);
+  if (st) return st;
+  double* LUw = static_cast<double*>(w);
+  double* Bw = reinterpret_cast<double*>(static_cast<char*>(w) + lu_b);
+  int64_t* bo = reinterpret_cast<int64_t*>(static_cast<char*>(w) + lu_b + rhs_b);
+  int32_t* nr = reinterpret_cast<int32_t*>(static_cast<char*>(w) + lu_b + rhs_b + off_b);
+  int32_t* info2 = reinterpret_cast<int32_t*>(static_cast<char*>(w) + lu_b + rhs_b + off_b + cnt_b);
+  BZG_HIP_CHECK(hipMemcpyAsync(bo, hb.data(), off_b, hipMemcpyHostToDevice, ctx->stream));
+  if (extent > 0) BZG_HIP_CHECK(hipMemcpyAsync(LUw, A, lu_b, hipMemcpyDeviceToDevice, ctx->stream));
+  // X = A^-1 I, eight columns per dgetrs pass, written over the factors in A
+  const int chunks = std::max(1, (maxn + 7) / 8);
+  for (int c = 0; c < chunks; ++c) {
+    hipLaunchKernelGGL(bzg::inv_rhs_kernel, dim3((unsigned)count), dim3(256), 0, ctx->stream, count, 8 * c, n, info,
+                       (const int64_t*)bo, Bw, nr);
+    BZG_LAUNCH_CHECK();
+    if (maxn == 0) break;
+    st = bzg::lu_batched(ctx, 2, count, n, nr, a_off, bo, p_off, LUw, Bw, ipiv, info2, order);
+    if (st) return st;
+    hipLaunchKernelGGL(bzg::inv_scatter_kernel, dim3((unsigned)count), dim3(256), 0, ctx->stream, count, 8 * c, n,
+                       (const int32_t*)nr, a_off, (const int64_t*)bo, (const double*)Bw, A);
+    BZG_LAUNCH_CHECK();
+  }
+  BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host copy of b_off is released on return
+  return BZG_OK;
+}
+
+bzg_status bzg_ddet_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+                            const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info, double* det,
+                            const int32_t* order) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "ddet_batched: null context");
+  BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "ddet_batched: bad count");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(n && a_off && p_off && A && ipiv && info && det, BZG_ERR_INVALID_ARGUMENT,
+              "ddet_batched: null array");
+  bzg_status st = bzg::lu_batched(ctx, 1, count, n, nullptr, a_off, nullptr, p_off, A, nullptr, ipiv, info, order);
+  if (st) return st;
+  hipLaunchKernelGGL(bzg::det_from_lu_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                     count, n, a_off, p_off, A, ipiv, info, det);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
 
 bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,
                              const int64_t* a_off, const int64_t* b_off, const int64_t* p_off,

@@ -104,6 +104,49 @@ def getrf_batched(ctx: bz.Context, As, lpt: bool = True):
     return LUs, ipivs, dinfo.cpu().numpy()[:len(As)]
 
 
+def det_batched(ctx: bz.Context, As, lpt: bool = True):
+    """Batched determinants (det() per system, DMatDetExpr.h:384-424, by the LU rule of detNxN).
+    Returns (dets, infos); each A_i is factored in a device copy."""
+    import torch
+    n = np.array([a.shape[0] for a in As], np.int32)
+    a_off = np.zeros(len(As), np.int64); p_off = np.zeros(len(As), np.int64)
+    if len(As):
+        a_off[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+        p_off[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    Ap = np.concatenate([np.ascontiguousarray(a, np.float64).ravel() for a in As] + [np.zeros(1)])
+    dn, da, dp, dA = (_dev(torch, x) for x in (n, a_off, p_off, Ap))
+    dpiv = torch.zeros(max(int(n.sum()), 1), dtype=torch.int32, device="cuda")
+    dinfo = torch.full((max(len(As), 1),), -99, dtype=torch.int32, device="cuda")
+    ddet = torch.full((max(len(As), 1),), float("nan"), dtype=torch.float64, device="cuda")
+    order = order_longest_first(ctx, n) if lpt and len(As) else None
+    ctx.call("bzg_ddet_batched", len(As), _p(dn), _p(da), _p(dp), _p(dA), _p(dpiv), _p(dinfo), _p(ddet),
+             _p(order) if order is not None else C.c_void_p(0))
+    ctx.sync()
+    return ddet.cpu().numpy()[:len(As)], dinfo.cpu().numpy()[:len(As)]
+
+
+def inv_batched(ctx: bz.Context, As, lpt: bool = True):
+    """Batched inverses (invert() by LU per system, Inversion.h:3337-3347). Returns (results,
+    infos): the inverse where info == 0, the LU factors where info > 0 (singular)."""
+    import torch
+    n = np.array([a.shape[0] for a in As], np.int32)
+    a_off = np.zeros(len(As), np.int64); p_off = np.zeros(len(As), np.int64)
+    if len(As):
+        a_off[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+        p_off[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    Ap = np.concatenate([np.ascontiguousarray(a, np.float64).ravel() for a in As] + [np.zeros(1)])
+    dn, da, dp, dA = (_dev(torch, x) for x in (n, a_off, p_off, Ap))
+    dpiv = torch.zeros(max(int(n.sum()), 1), dtype=torch.int32, device="cuda")
+    dinfo = torch.full((max(len(As), 1),), -99, dtype=torch.int32, device="cuda")
+    order = order_longest_first(ctx, n) if lpt and len(As) else None
+    ctx.call("bzg_dinv_batched", len(As), _p(dn), _p(da), _p(dp), _p(dA), _p(dpiv), _p(dinfo),
+             _p(order) if order is not None else C.c_void_p(0))
+    ctx.sync()
+    Ah = dA.cpu().numpy()
+    return [Ah[a_off[i]:a_off[i] + n[i] * n[i]].reshape(n[i], n[i]) for i in range(len(As))], \
+        dinfo.cpu().numpy()[:len(As)]
+
+
 def getrs_batched(ctx: bz.Context, LUs, ipivs, Bs, lpt: bool = True):
     """Batched solve with given factors (dgetrs per system). Returns (Xs, infos)."""
     import torch

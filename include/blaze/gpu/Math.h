@@ -836,6 +836,63 @@ inline void lu(const DynamicMatrix<double>& A, DynamicMatrix<double>& L, Dynamic
   P.upload(pm.data());
 }
 
+// det(A) (blaze/math/expressions/DMatDetExpr.h:384-424): std::invalid_argument for a non-square
+// matrix, 0 for 0 x 0, otherwise detNxN's LU rule (:324-357) on the device.  Like the reference's
+// getrf on row-major storage, the factorisation is of A^T (same pivots as the reference's).
+inline double det(const DynamicMatrix<double>& A) {
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
+  const size_t n = A.rows();
+  if (n == 0) return 0.0;
+  if (n > 512) throw std::invalid_argument("det: this build supports n <= 512");
+  detail::DeviceBuffer<double> T(n * n);
+  detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "det: transpose");
+  const int32_t nn = (int32_t)n;
+  const int64_t zero = 0;
+  detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
+  detail::DeviceBuffer<int64_t> doff;
+  detail::DeviceBuffer<double> dd;
+  dn.allocate(1); dpiv.allocate(n); dinfo.allocate(1); doff.allocate(1); dd.allocate(1);
+  detail::h2d(dn.ptr, &nn, 1);
+  detail::h2d(doff.ptr, &zero, 1);
+  detail::check(bzg_ddet_batched(context(), 1, dn.ptr, doff.ptr, doff.ptr, T.ptr, dpiv.ptr, dinfo.ptr, dd.ptr,
+                                 nullptr),
+                "det");
+  double d = 0.0;
+  detail::d2h(&d, dd.ptr, 1);
+  return d;
+}
+
+// invert(A) in place and inv(A) (blaze/math/dense/Inversion.h:3337-3347, byLU: getrf + getri):
+// std::invalid_argument for a non-square matrix, std::runtime_error("Inversion of singular matrix
+// failed") when the factorisation meets an exactly zero pivot.  Like the reference's getrf on
+// row-major storage, the factorisation is of A^T; inv(A^T)^T = inv(A).
+inline void invert(DynamicMatrix<double>& A) {
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
+  const size_t n = A.rows();
+  if (n == 0) return;
+  if (n > 512) throw std::invalid_argument("inv: this build supports n <= 512");
+  detail::DeviceBuffer<double> T(n * n);
+  detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "inv: transpose");
+  const int32_t nn = (int32_t)n;
+  const int64_t zero = 0;
+  detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
+  detail::DeviceBuffer<int64_t> doff;
+  dn.allocate(1); dpiv.allocate(n); dinfo.allocate(1); doff.allocate(1);
+  detail::h2d(dn.ptr, &nn, 1);
+  detail::h2d(doff.ptr, &zero, 1);
+  detail::check(bzg_dinv_batched(context(), 1, dn.ptr, doff.ptr, doff.ptr, T.ptr, dpiv.ptr, dinfo.ptr, nullptr),
+                "inv");
+  int32_t info = 0;
+  detail::d2h(&info, dinfo.ptr, 1);
+  if (info != 0) throw std::runtime_error("Inversion of singular matrix failed");
+  detail::check(bzg_dtranspose(context(), n, n, T.ptr, n, A.data(), n), "inv: transpose");
+}
+inline DynamicMatrix<double> inv(const DynamicMatrix<double>& A) {
+  DynamicMatrix<double> R(A);
+  invert(R);
+  return R;
+}
+
 template <typename T>
 DynamicVector<T>& DynamicVector<T>::operator=(const DMatDVecSolveExpr<T>& e) {
   static_assert(std::is_same<T, double>::value, "solve() is provided for double");

@@ -26,6 +26,8 @@
  *        gesv (blaze/math/lapack/gesv.h:136-175) -> dgesv_ (blaze/math/lapack/clapack/gesv.h:207)
  *   bzg_dgetrf_batched / bzg_dgetrs_batched <- getrf / getrs (blaze/math/lapack/clapack/getrf.h:190-197,
  *        clapack/getrs.h:207-215), the factorisation under lu() (blaze/math/dense/LU.h:98-140)
+ *   bzg_ddet_batched                   <- det() (blaze/math/expressions/DMatDetExpr.h:384-424)
+ *   bzg_dinv_batched                   <- invert() / inv() by LU (blaze/math/dense/Inversion.h:3337-3347)
  *   bzg_dsgesv_batched                 <- same call site; LAPACK dsgesv semantics (mixed precision)
  *   bzg_dgemm_fp64acc[_i8]             <- no reference function: FP64-accurate GEMM from BF16
  *        (INT8) residue slices; drop-in for bzg_dgemm at the same call site
@@ -246,6 +248,25 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
 bzg_status bzg_dgetrf_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
                               const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info,
                               const int32_t* order);
+/* Determinants (replaces det(), blaze/math/expressions/DMatDetExpr.h:384-424, whose orders >= 7
+ * take detNxN, :324-357): factors every A_i in place exactly as bzg_dgetrf_batched (ipiv and
+ * info as there), then det[i] = product over k of U_i(k,k), negated for every k with
+ * ipiv_i[k] != k + 1, multiplied left to right from 1; 0 when info[i] > 0.  n[i] = 0 gives
+ * det 0 and info 0 (the reference's 0 x 0 case); an order outside 0..512 gives info -1 and NaN.
+ * The reference's closed forms for orders 2..6 agree with this to rounding. */
+bzg_status bzg_ddet_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+                            const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info, double* det,
+                            const int32_t* order);
+/* Inverses in place (replaces invert()/inv() by LU, blaze/math/dense/Inversion.h:3337-3347:
+ * getrf then getri): A_i := A_i^-1, computed as the factorisation of bzg_dgetrf_batched
+ * (ipiv, info as there) followed by dgetrs solves against the identity, eight columns per pass.
+ * A system with info[i] > 0 (singular; the reference throws "Inversion of singular matrix
+ * failed") keeps its LU factors in A_i; n[i] = 0 gives info 0.  Reads n and a_off back to the
+ * host (one synchronisation) and needs scratch of one copy of the factors plus 8 n_i doubles
+ * per system (cached in the context). */
+bzg_status bzg_dinv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+                            const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info,
+                            const int32_t* order);
 /* Solve with factors (LAPACK dgetrs per system, replaces getrs, clapack/getrs.h:207-215):
  * LU / ipiv as written by bzg_dgetrf_batched; the interchanges are applied to B_i in pivot
  * order, then B_i := U^-1 L^-1 B_i in place.  info[i] = 0, or -1 outside the size range.

@@ -260,6 +260,75 @@ int main() {
     ok = ok && throws<std::invalid_argument>([&] { C1 = trans(A) * B; }, "Matrix sizes do not match");
     check(ok, "trans(A)*B, A*trans(B), trans(A)*trans(B), C += s*(trans(A)*trans(B))");
   }
+  // det (operations/determinant/DenseTest.cpp:285-470): 0 x 0 -> 0, 1 x 1 -> A(0,0) exactly,
+  // unit-triangular 9 x 9 -> 1 exactly, non-square throws; random 8 x 8 against a host LU
+  {
+    bool ok = det(DynamicMatrix<double>(0, 0)) == 0.0;
+    ok = ok && det(DynamicMatrix<double>(1, 1, -2.75)) == -2.75;
+    const size_t n9 = 9;
+    std::vector<double> lo(n9 * n9, 0.0), up(n9 * n9, 0.0);
+    for (size_t i = 0; i < n9; ++i) {
+      lo[i * n9 + i] = up[i * n9 + i] = 1.0;
+      for (size_t j = 0; j < i; ++j) lo[i * n9 + j] = U(rng);
+      for (size_t j = i + 1; j < n9; ++j) up[i * n9 + j] = U(rng);
+    }
+    ok = ok && det(DynamicMatrix<double>(n9, n9, lo.data())) == 1.0 &&
+         det(DynamicMatrix<double>(n9, n9, up.data())) == 1.0;
+    ok = ok && throws<std::invalid_argument>([&] { (void)det(DynamicMatrix<double>(2, 3, 0.0)); },
+                                             "Invalid non-square matrix provided");
+    // random 8 x 8: host Gaussian elimination with partial pivoting in long double
+    const size_t n = 8;
+    std::vector<double> a(n * n);
+    for (auto& v : a) v = U(rng);
+    std::vector<long double> w(a.begin(), a.end());
+    long double ref = 1.0L;
+    for (size_t k = 0; k < n; ++k) {
+      size_t piv = k;
+      for (size_t i = k + 1; i < n; ++i)
+        if (std::fabs((double)w[i * n + k]) > std::fabs((double)w[piv * n + k])) piv = i;
+      if (piv != k) {
+        for (size_t j = 0; j < n; ++j) std::swap(w[k * n + j], w[piv * n + j]);
+        ref = -ref;
+      }
+      ref *= w[k * n + k];
+      for (size_t i = k + 1; i < n; ++i) {
+        const long double l = w[i * n + k] / w[k * n + k];
+        for (size_t j = k; j < n; ++j) w[i * n + j] -= l * w[k * n + j];
+      }
+    }
+    ok = ok && relaxed(det(DynamicMatrix<double>(n, n, a.data())), (double)ref, 1e-12);
+    check(ok, "det");
+  }
+  // inv / invert (Inversion.h:3337-3347): A * inv(A) = I, a power-of-two diagonal inverts
+  // exactly, a singular matrix throws std::runtime_error, a non-square one std::invalid_argument
+  {
+    const size_t n = 37;
+    std::vector<double> a(n * n);
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = 0; j < n; ++j) a[i * n + j] = U(rng) + (i == j ? 4.0 : 0.0);
+    DynamicMatrix<double> A(n, n, a.data());
+    DynamicMatrix<double> X = inv(A);
+    auto g = X.toVector();
+    bool ok = X.rows() == n && X.columns() == n;
+    for (size_t i = 0; i < n && ok; ++i)
+      for (size_t j = 0; j < n; ++j) {
+        long double s = 0.0L;
+        for (size_t k = 0; k < n; ++k) s += (long double)a[i * n + k] * g[k * n + j];
+        ok = ok && std::fabs((double)s - (i == j ? 1.0 : 0.0)) <= 1e-13;
+      }
+    std::vector<double> dg(16, 0.0);
+    for (size_t i = 0; i < 4; ++i) dg[i * 4 + i] = std::ldexp(1.0, (int)i - 2);
+    DynamicMatrix<double> D(4, 4, dg.data());
+    invert(D);
+    auto dv = D.toVector();
+    for (size_t i = 0; i < 4; ++i) ok = ok && dv[i * 4 + i] == std::ldexp(1.0, 2 - (int)i);
+    std::vector<double> sg(9, 1.0);  // rank one
+    ok = ok && throws<std::runtime_error>([&] { DynamicMatrix<double> S(3, 3, sg.data()); invert(S); },
+                                          "Inversion of singular matrix failed");
+    ok = ok && throws<std::invalid_argument>([&] { DynamicMatrix<double> R(2, 3, 1.0); invert(R); },
+                                             "Invalid non-square matrix provided");
+    check(ok, "inv / invert");
+  }
   // y = trans(A) * x (TDMatDVecMultExpr)
   {
     const size_t m = 300, n = 77;
