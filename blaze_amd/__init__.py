@@ -50,6 +50,8 @@ SIGNATURES = {
     "bzg_sgemv": (i32, [vp, i64, i64, f32, vp, i64, vp, f32, vp]),
     "bzg_igemv": (i32, [vp, i64, i64, i32, vp, i64, vp, i32, vp]),
     "bzg_csr_plan_create": (i32, [vp, i64, i64, vp, C.POINTER(vp)]),
+    "bzg_csr_plan_set_sum_start": (i32, [vp, i32]),
+    "bzg_csc_to_csr_host": (i32, [i64, i64, vp, vp, vp, i32, vp, vp, vp]),
     "bzg_csr_plan_destroy": (i32, [vp]),
     "bzg_csr_plan_tune": (i32, [vp, i32, i32, i32]),
     "bzg_csr_plan_describe": (i32, [vp, C.c_char_p, C.c_size_t]),

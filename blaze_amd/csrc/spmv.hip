@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 // Every product is rounded before it is added: never contract v*x + s into an FMA here.
@@ -44,6 +45,7 @@ struct bzg_csr_plan {
   int64_t n_huge = 0;
   int64_t max_row = 0;
   double mean_row = 0;
+  int plus_zero = 0;        // sums start from +0 (column-major product semantics)
   std::vector<int32_t> rowptr_host;  // kept for re-planning by the autotuner
   char desc[256] = {0};
 };
@@ -273,6 +275,14 @@ __global__ __launch_bounds__(256) void csr_rows_kernel(int64_t n_rows, const int
   }
 }
 
+// y += +0 on every entry: turns the -0 that a row of only -0 products leaves into +0 and keeps
+// every other value, so the sums equal those of an accumulation that starts from +0.
+template <typename T>
+__global__ __launch_bounds__(256) void plus_zero_kernel(int64_t n, T* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = y[i] + T(0);
+}
+
 // y = 0 for a matrix without nonzeros (x may then be null, so no SpMV kernel runs).
 template <typename T>
 __global__ __launch_bounds__(256) void zero_rows_kernel(int64_t n, T* __restrict__ y) {
@@ -350,6 +360,11 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
     const int64_t blocks = (plan->n_huge + 7) / 8 * 8;
     hipLaunchKernelGGL((csr_rows_kernel<T, 256, 8>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
                        plan->n_huge, plan->huge_rows, rowptr, colidx, val, x, y);
+    BZG_LAUNCH_CHECK();
+  }
+  if (plan->plus_zero && !std::is_integral<T>::value) {
+    hipLaunchKernelGGL(plus_zero_kernel<T>, dim3((unsigned)((plan->n_rows + 255) / 256)), dim3(256), 0, ctx->stream,
+                       plan->n_rows, y);
     BZG_LAUNCH_CHECK();
   }
   return BZG_OK;
@@ -533,6 +548,45 @@ bzg_status bzg_csr_plan_tune(bzg_csr_plan* p, int32_t lanes, int32_t rows_per_gr
                   " vector-csr lanes=%d rows/group=%d cap=%d rows=%lld%s", lanes, rows_per_group,
                   cap, (long long)p->n_short, tail);
   }
+  return BZG_OK;
+}
+
+bzg_status bzg_csr_plan_set_sum_start(bzg_csr_plan* p, int32_t positive_zero) {
+  BZG_REQUIRE(p != nullptr, BZG_ERR_INVALID_ARGUMENT, "csr_plan_set_sum_start: null plan");
+  p->plus_zero = positive_zero != 0;
+  return BZG_OK;
+}
+
+bzg_status bzg_csc_to_csr_host(int64_t n_rows, int64_t n_cols, const int32_t* colptr, const int32_t* rowidx,
+                               const void* val, int32_t elem_bytes, int32_t* rowptr, int32_t* colidx,
+                               void* val_out) {
+  BZG_REQUIRE(n_rows >= 0 && n_cols >= 0 && n_rows < (1LL << 31), BZG_ERR_INVALID_ARGUMENT,
+              "csc_to_csr_host: bad size");
+  BZG_REQUIRE(colptr && rowptr && (elem_bytes == 4 || elem_bytes == 8), BZG_ERR_INVALID_ARGUMENT,
+              "csc_to_csr_host: bad argument");
+  BZG_REQUIRE(colptr[0] == 0, BZG_ERR_INVALID_ARGUMENT, "csc_to_csr_host: colptr[0] != 0");
+  const int64_t nnz = colptr[n_cols];
+  BZG_REQUIRE(nnz >= 0 && (nnz == 0 || (rowidx && val && colidx && val_out)), BZG_ERR_INVALID_ARGUMENT,
+              "csc_to_csr_host: null array");
+  // counting sort by row; columns are visited in order, so every row lists its columns ascending
+  std::vector<int64_t> cnt((size_t)n_rows + 1, 0);
+  for (int64_t c = 0; c < n_cols; ++c) {
+    BZG_REQUIRE(colptr[c + 1] >= colptr[c], BZG_ERR_INVALID_ARGUMENT, "csc_to_csr_host: colptr decreasing");
+    for (int64_t e = colptr[c]; e < colptr[c + 1]; ++e) {
+      BZG_REQUIRE(rowidx[e] >= 0 && rowidx[e] < n_rows, BZG_ERR_INVALID_ARGUMENT, "csc_to_csr_host: row index");
+      ++cnt[(size_t)rowidx[e] + 1];
+    }
+  }
+  for (int64_t r = 0; r < n_rows; ++r) cnt[(size_t)r + 1] += cnt[(size_t)r];
+  for (int64_t r = 0; r <= n_rows; ++r) rowptr[r] = (int32_t)cnt[(size_t)r];
+  const char* src = static_cast<const char*>(val);
+  char* dst = static_cast<char*>(val_out);
+  for (int64_t c = 0; c < n_cols; ++c)
+    for (int64_t e = colptr[c]; e < colptr[c + 1]; ++e) {
+      const int64_t q = cnt[(size_t)rowidx[e]]++;
+      colidx[q] = (int32_t)c;
+      std::memcpy(dst + q * elem_bytes, src + e * elem_bytes, (size_t)elem_bytes);
+    }
   return BZG_OK;
 }
 

@@ -218,6 +218,16 @@ struct SMatDVecMultExpr {
   size_t size() const { return mat.rows(); }
 };
 template <typename T>
+struct SMatTransExpr {  // trans(S) of a row-major CompressedMatrix: a column-major operand
+  const CompressedMatrix<T>& mat;
+};
+template <typename T>
+struct TSMatDVecMultExpr {  // trans(S) * x (TSMatDVecMultExpr)
+  const CompressedMatrix<T>& mat;
+  const DynamicVector<T>& vec;
+  size_t size() const { return mat.columns(); }
+};
+template <typename T>
 struct DMatDVecSolveExpr {
   const DynamicMatrix<T>& mat;
   const DynamicVector<T>& rhs;
@@ -476,20 +486,33 @@ class DynamicVector {
     }
     return *this;
   }
-  DynamicVector& operator+=(const DMatDVecMultExpr<T>& e) {
+  DynamicVector& operator+=(const DMatDVecMultExpr<T>& e) { return accumulate(e, T(1)); }
+  DynamicVector& operator-=(const DMatDVecMultExpr<T>& e) { return accumulate(e, T(-1)); }
+  DynamicVector& operator=(const TSMatDVecMultExpr<T>& e) {
+    if (e.vec.isAliased(this)) {
+      DynamicVector tmp;
+      tmp.assign(e);
+      swap(tmp);
+    } else {
+      assign(e);
+    }
+    return *this;
+  }
+  // y += s * (A * x) (DMatDVecMultExpr addAssign / subAssign): one GEMV with beta = 1
+  DynamicVector& accumulate(const DMatDVecMultExpr<T>& e, T sign) {
     if (e.size() != n_) throw std::invalid_argument("Vector sizes do not match");
     if (e.vec.isAliased(this)) {
       DynamicVector tmp;
       tmp.assign(e);
       std::vector<T> a = toVector(), b = tmp.toVector();
-      for (size_t i = 0; i < n_; ++i) a[i] += b[i];
+      for (size_t i = 0; i < n_; ++i) a[i] += sign * b[i];
       upload(a.data());
       return *this;
     }
     if (n_)
-      detail::check(detail::gemv(e.mat.rows(), e.mat.columns(), e.scale, e.mat.data(),
+      detail::check(detail::gemv(e.mat.rows(), e.mat.columns(), sign * e.scale, e.mat.data(),
                                  std::max<size_t>(e.mat.columns(), 1), e.vec.data(), T(1), buf_.ptr, e.ta),
-                    "y += A * x");
+                    sign == T(1) ? "y += A * x" : "y -= A * x");
     detail::check(bzg_synchronize(context()), "sync");
     return *this;
   }
@@ -505,6 +528,7 @@ class DynamicVector {
     detail::check(bzg_synchronize(context()), "sync");
   }
   void assign(const SMatDVecMultExpr<T>& e);
+  void assign(const TSMatDVecMultExpr<T>& e);
 
  private:
   size_t n_ = 0;
@@ -597,6 +621,15 @@ class CompressedMatrix {
   const int32_t* d_rowptr() const { upload(); return d_rowptr_.ptr; }
   const int32_t* d_colidx() const { upload(); return d_col_.ptr; }
   const T* d_values() const { upload(); return d_val_.ptr; }
+  // device-side CSR of trans(S) (packed on first use by bzg_csc_to_csr_host; its plan sums from
+  // +0 as the reference's column-major product does)
+  const bzg_csr_plan* t_plan() const {
+    upload_trans();
+    return t_plan_;
+  }
+  const int32_t* d_t_rowptr() const { upload_trans(); return d_trp_.ptr; }
+  const int32_t* d_t_colidx() const { upload_trans(); return d_tcol_.ptr; }
+  const T* d_t_values() const { upload_trans(); return d_tval_.ptr; }
 
  private:
   void put(size_t i, size_t j, const T& value, bool must_be_new) {
@@ -637,12 +670,40 @@ class CompressedMatrix {
     detail::check(bzg_csr_plan_create(context(), m_, n_, rowptr_.data(), &p), "CSR plan");
     self->plan_ = p;
   }
+  void upload_trans() const {
+    if (t_plan_) return;
+    auto* self = const_cast<CompressedMatrix*>(this);
+    self->close_rows_before(m_);
+    const size_t nnz = val_.size();
+    std::vector<int32_t> trp(n_ + 1), tcol(std::max<size_t>(nnz, 1));
+    std::vector<T> tval(std::max<size_t>(nnz, 1));
+    detail::check(bzg_csc_to_csr_host((int64_t)n_, (int64_t)m_, rowptr_.data(), col_.data(), val_.data(),
+                                      (int32_t)sizeof(T), trp.data(), tcol.data(), tval.data()),
+                  "trans(S) pack");
+    self->d_trp_.allocate(n_ + 1);
+    self->d_tcol_.allocate(tcol.size());
+    self->d_tval_.allocate(tval.size());
+    detail::h2d(self->d_trp_.ptr, trp.data(), n_ + 1);
+    if (nnz) {
+      detail::h2d(self->d_tcol_.ptr, tcol.data(), nnz);
+      detail::h2d(self->d_tval_.ptr, tval.data(), nnz);
+    }
+    bzg_csr_plan* p = nullptr;
+    detail::check(bzg_csr_plan_create(context(), n_, m_, trp.data(), &p), "CSR plan of trans(S)");
+    detail::check(bzg_csr_plan_set_sum_start(p, 1), "CSR plan of trans(S)");
+    self->t_plan_ = p;
+  }
   void drop_device() {
     if (plan_) bzg_csr_plan_destroy(plan_);
     plan_ = nullptr;
     d_rowptr_.release();
     d_col_.release();
     d_val_.release();
+    if (t_plan_) bzg_csr_plan_destroy(t_plan_);
+    t_plan_ = nullptr;
+    d_trp_.release();
+    d_tcol_.release();
+    d_tval_.release();
   }
 
   size_t m_ = 0, n_ = 0, cur_row_ = 0;
@@ -651,7 +712,21 @@ class CompressedMatrix {
   bzg_csr_plan* plan_ = nullptr;
   detail::DeviceBuffer<int32_t> d_rowptr_, d_col_;
   detail::DeviceBuffer<T> d_val_;
+  bzg_csr_plan* t_plan_ = nullptr;
+  detail::DeviceBuffer<int32_t> d_trp_, d_tcol_;
+  detail::DeviceBuffer<T> d_tval_;
 };
+
+template <typename T>
+void DynamicVector<T>::assign(const TSMatDVecMultExpr<T>& e) {
+  const size_t n = e.mat.columns();
+  resize(n, false);
+  if (n)
+    detail::check(detail::csrmv(e.mat.t_plan(), e.mat.d_t_rowptr(), e.mat.d_t_colidx(), e.mat.d_t_values(),
+                                e.vec.data(), buf_.ptr),
+                  "y = trans(S) * x");
+  detail::check(bzg_synchronize(context()), "sync");
+}
 
 template <typename T>
 void DynamicVector<T>::assign(const SMatDVecMultExpr<T>& e) {
@@ -710,6 +785,17 @@ template <typename T>
 inline DMatDVecMultExpr<T> operator*(const DynamicMatrix<T>& A, const DynamicVector<T>& x) {
   if (A.columns() != x.size()) throw std::invalid_argument("Matrix and vector sizes do not match");
   return {A, x, T(1)};
+}
+template <typename T>
+inline SMatTransExpr<T> trans(const CompressedMatrix<T>& S) {
+  return {S};
+}
+// trans(S) * x (TSMatDVecMultExpr, TSMatDVecMultExpr.h:329-383): the product over the CSR of
+// trans(S), each entry summed from +0 in the order of S's rows
+template <typename T>
+inline TSMatDVecMultExpr<T> operator*(const SMatTransExpr<T>& S, const DynamicVector<T>& x) {
+  if (S.mat.rows() != x.size()) throw std::invalid_argument("Matrix and vector sizes do not match");
+  return {S.mat, x};
 }
 template <typename T>
 inline SMatDVecMultExpr<T> operator*(const CompressedMatrix<T>& S, const DynamicVector<T>& x) {

@@ -20,6 +20,8 @@
  *        and cblas_dgemv (blaze/math/blas/cblas/gemv.h:153)
  *   bzg_dcsrmv / bzg_scsrmv / bzg_icsrmv <- SMatDVecMultExpr element evaluation
  *        (SMatDVecMultExpr.h:204-207 -> SVecDVecInnerExpr.h:127-135)
+ *   bzg_csr_plan_set_sum_start + bzg_csc_to_csr_host + bzg_dcsrmv <- TSMatDVecMultExpr assign
+ *        (TSMatDVecMultExpr.h:329-383: column-major sparse matrix or trans(S) times a vector)
  *   bzg_dcg                            <- the CG loop of blazemark/src/blaze/CG.cpp:114-135 (its
  *        SpMV, dot products and vector updates, resident on the device)
  *   bzg_dgesv_batched                  <- solve() (blaze/math/dense/LSE.h:6343-6373) ->
@@ -186,6 +188,20 @@ bzg_status bzg_igemv(bzg_context* ctx, int64_t m, int64_t n, int32_t alpha, cons
 typedef struct bzg_csr_plan bzg_csr_plan;
 bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
                                const int32_t* rowptr_host, bzg_csr_plan** plan);
+/* positive_zero != 0: every y entry is the sum started from +0 (column-major products,
+ * TSMatDVecMultExpr's reset-then-accumulate, blaze/math/expressions/TSMatDVecMultExpr.h:329-383)
+ * instead of from the first product (SMatDVecMultExpr, SVecDVecInnerExpr.h:127-135).  The two
+ * differ only in the sign of an all-zero sum: one extra pass adds +0 to y. */
+bzg_status bzg_csr_plan_set_sum_start(bzg_csr_plan* plan, int32_t positive_zero);
+/* Host CSC -> CSR pack (a column-major CompressedMatrix, CompressedMatrix.h:3118, into the
+ * layout the device products read): colptr (n_cols + 1), rowidx, val (elem_bytes 4 or 8) in;
+ * rowptr (n_rows + 1), colidx, val_out (colptr[n_cols] entries) out.  Within every row the
+ * columns come out ascending, so y = A x over the result sums each row in column order -- the
+ * reference's order for a column-major A.  The CSR of trans(S) for a row-major S is this pack
+ * applied to S's CSR read as CSC. */
+bzg_status bzg_csc_to_csr_host(int64_t n_rows, int64_t n_cols, const int32_t* colptr, const int32_t* rowidx,
+                               const void* val, int32_t elem_bytes, int32_t* rowptr, int32_t* colidx,
+                               void* val_out);
 bzg_status bzg_csr_plan_destroy(bzg_csr_plan* plan);
 /* Overrides the vector-CSR shape chosen from the histogram: lanes per row in {1,2,4,8,16,32},
  * rows per lane group in {1,2}, nonzeros loaded per lane per batch in {1,2,4,8}; used by the

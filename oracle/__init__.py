@@ -116,6 +116,15 @@ def csrmv(rowptr, col, val, x) -> np.ndarray:
     return y
 
 
+def tcsrmv(rowptr, col, val, x, n_cols: int) -> np.ndarray:
+    """y = trans(S) * x for a row-major CSR S (oracle.c orc_tcsrmv_*: TSMatDVecMultExpr's loop)."""
+    n = len(rowptr) - 1
+    fn = {np.dtype(np.float64): "orc_tcsrmv_d", np.dtype(np.float32): "orc_tcsrmv_s"}[val.dtype]
+    y = np.empty(n_cols, val.dtype)
+    getattr(orc(), fn)(i64(n), i64(n_cols), _p(rowptr), _p(col), _p(val), _p(x), _p(y))
+    return y
+
+
 def getrf(A: np.ndarray):
     """LU with partial pivoting (dgetf2 rule) of a copy of A: returns (LU, ipiv 1-based, info)."""
     LU = np.array(A, np.float64, order="C", copy=True)
@@ -221,6 +230,15 @@ def ref_gemv(A: np.ndarray, x: np.ndarray) -> np.ndarray:
           np.dtype(np.int32): "ref_igemv"}[A.dtype]
     y = np.empty(m, A.dtype)
     getattr(ref(), fn)(i64(m), i64(n), _p(A), _p(x), _p(y))
+    return y
+
+
+def ref_tcsrmv(rowptr, col, val, x, n_cols: int) -> np.ndarray:
+    """trans(S) * x by the reference's own TSMatDVecMultExpr (compiled reference)."""
+    n = len(rowptr) - 1
+    fn = {np.dtype(np.float64): "ref_dtcsrmv", np.dtype(np.float32): "ref_stcsrmv"}[val.dtype]
+    y = np.empty(n_cols, val.dtype)
+    getattr(ref(), fn)(i64(n), i64(n_cols), _p(rowptr), _p(col), _p(val), _p(x), _p(y))
     return y
 
 

@@ -164,6 +164,28 @@ void orc_csrmv_s(int64_t n_rows, const int32_t* rowptr, const int32_t* col, cons
     y[i] = sp;
   }
 }
+/* y = trans(S) * x for a row-major CSR S (n_rows x n_cols), y of n_cols entries: the
+ * reference's TSMatDVecMultExpr assign (blaze/math/expressions/TSMatDVecMultExpr.h:329-383):
+ * y is reset to +0, then for every row j of S in order and every stored (i, v) of it,
+ * y[i] += v * x[j] (each product rounded before it is added). */
+void orc_tcsrmv_d(int64_t n_rows, int64_t n_cols, const int32_t* rowptr, const int32_t* col,
+                  const double* val, const double* x, double* y) {
+  for (int64_t i = 0; i < n_cols; i++) y[i] = 0.0;
+  for (int64_t j = 0; j < n_rows; j++)
+    for (int32_t t = rowptr[j]; t < rowptr[j + 1]; t++) {
+      volatile double p = val[t] * x[j];
+      y[col[t]] = y[col[t]] + p;
+    }
+}
+void orc_tcsrmv_s(int64_t n_rows, int64_t n_cols, const int32_t* rowptr, const int32_t* col,
+                  const float* val, const float* x, float* y) {
+  for (int64_t i = 0; i < n_cols; i++) y[i] = 0.0f;
+  for (int64_t j = 0; j < n_rows; j++)
+    for (int32_t t = rowptr[j]; t < rowptr[j + 1]; t++) {
+      volatile float p = val[t] * x[j];
+      y[col[t]] = y[col[t]] + p;
+    }
+}
 void orc_csrmv_i32(int64_t n_rows, const int32_t* rowptr, const int32_t* col, const int32_t* val,
                    const int32_t* x, int32_t* y) {
   for (int64_t i = 0; i < n_rows; i++) {

@@ -64,6 +64,17 @@ void csrmv(int64_t n_rows, int64_t n_cols, const int32_t* rowptr, const int32_t*
   yv = S * xv;
   for (int64_t i = 0; i < n_rows; ++i) y[i] = yv[i];
 }
+// y = trans(S) * x for a row-major S: the reference's TSMatDVecMultExpr (a column-major operand)
+template <typename T>
+void tcsrmv(int64_t n_rows, int64_t n_cols, const int32_t* rowptr, const int32_t* col, const T* val,
+            const T* x, T* y) {
+  blaze::CompressedMatrix<T, rowMajor> S;
+  build_csr(S, n_rows, n_cols, rowptr, col, val);
+  blaze::DynamicVector<T, columnVector> xv(n_rows), yv;
+  for (int64_t i = 0; i < n_rows; ++i) xv[i] = x[i];
+  yv = trans(S) * xv;
+  for (int64_t j = 0; j < n_cols; ++j) y[j] = yv[j];
+}
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -85,6 +96,8 @@ void ref_igemv(int64_t m, int64_t n, const int32_t* A, const int32_t* x, int32_t
 void ref_dcsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const double* v, const double* x, double* y) { csrmv(r, c, rp, ci, v, x, y); }
 void ref_scsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const float* v, const float* x, float* y) { csrmv(r, c, rp, ci, v, x, y); }
 void ref_icsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const int32_t* v, const int32_t* x, int32_t* y) { csrmv(r, c, rp, ci, v, x, y); }
+void ref_dtcsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const double* v, const double* x, double* y) { tcsrmv(r, c, rp, ci, v, x, y); }
+void ref_stcsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const float* v, const float* x, float* y) { tcsrmv(r, c, rp, ci, v, x, y); }
 
 // Closed-form solves of LSE.h for 1 <= n <= 6.  Returns 0, or 1 when Blaze throws
 // std::runtime_error("Solving LSE with singular system matrix failed").

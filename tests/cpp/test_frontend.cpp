@@ -9,9 +9,11 @@
 // is the number of failures.
 #include <blaze/gpu/Math.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <random>
 #include <string>
@@ -328,6 +330,68 @@ int main() {
     ok = ok && throws<std::invalid_argument>([&] { DynamicMatrix<double> R(2, 3, 1.0); invert(R); },
                                              "Invalid non-square matrix provided");
     check(ok, "inv / invert");
+  }
+  // y = trans(S) * x (TSMatDVecMultExpr.h:329-383): bit-identical to the reference's loop --
+  // y reset to +0, then rows j of S in order, y[i] += v * x[j] -- including an entry whose
+  // products are all -0 (+0 as in the reference); size mismatch throws
+  {
+    const size_t m = 700, n = 450;
+    CompressedMatrix<double> S(m, n);
+    std::vector<std::vector<std::pair<size_t, double>>> rows(m);
+    std::uniform_int_distribution<int> len(0, 9);
+    for (size_t i = 0; i < m; ++i) {
+      std::vector<size_t> cols;
+      for (int q = len(rng); q > 0; --q) cols.push_back((size_t)(std::uniform_int_distribution<int>(0, (int)n - 2)(rng)));
+      std::sort(cols.begin(), cols.end());
+      cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+      for (size_t j : cols) {
+        const double v = U(rng);
+        S.append(i, j, v);
+        rows[i].push_back({j, v});
+      }
+      if (i % 50 == 3) {  // column n - 1: only -0 products (x = 0 on these rows, v < 0)
+        S.append(i, n - 1, -1.0);
+        rows[i].push_back({n - 1, -1.0});
+      }
+      S.finalize(i);
+    }
+    std::vector<double> x(m);
+    for (size_t i = 0; i < m; ++i) x[i] = (i % 50 == 3) ? 0.0 : U(rng);
+    DynamicVector<double> X(m, x.data()), Y;
+    Y = trans(S) * X;
+    std::vector<double> want(n, 0.0);
+    for (size_t i = 0; i < m; ++i)
+      for (auto& e : rows[i]) {
+        const double prod = e.second * x[i];
+        want[e.first] = want[e.first] + prod;
+      }
+    auto g = Y.toVector();
+    bool ok = Y.size() == n;
+    for (size_t j = 0; j < n && ok; ++j)
+      ok = std::memcmp(&g[j], &want[j], sizeof(double)) == 0;
+    ok = ok && !std::signbit(g[n - 1]) && g[n - 1] == 0.0;
+    ok = ok && throws<std::invalid_argument>([&] { DynamicVector<double> Z(n, 1.0); Y = trans(S) * Z; },
+                                             "Matrix and vector sizes do not match");
+    check(ok, "y = trans(S) * x bit-identical to the reference loop");
+  }
+  // y -= A * x (DMatDVecMultExpr subAssign)
+  {
+    const size_t m = 90, n = 70;
+    std::vector<double> a(m * n), x(n), y0(m);
+    for (auto& v : a) v = U(rng);
+    for (auto& v : x) v = U(rng);
+    for (auto& v : y0) v = U(rng);
+    DynamicMatrix<double> A(m, n, a.data());
+    DynamicVector<double> X(n, x.data()), Y(m, y0.data());
+    Y -= A * X;
+    auto g = Y.toVector();
+    bool ok = true;
+    for (size_t i = 0; i < m; ++i) {
+      double s = 0.0;
+      for (size_t j = 0; j < n; ++j) s += a[i * n + j] * x[j];
+      ok = ok && relaxed(g[i], y0[i] - s, 1e-12);
+    }
+    check(ok, "y -= A * x");
   }
   // y = trans(A) * x (TDMatDVecMultExpr)
   {

@@ -74,6 +74,21 @@ def spmv_fixtures():
                         nnz=np.int64(rp[-1]), y=y, y_f32=y32)
 
 
+def spmv_trans_fixtures():
+    # y = trans(S) * x by the reference's TSMatDVecMultExpr: a Zipf matrix with short rows
+    # (every column of S, i.e. every row of S^T, holds at most a few dozen entries) and the
+    # s = 1.5 matrix of spmv_zipf1e4 (long columns too)
+    out = {}
+    for tag, (seed, n, s, kmax) in {"short": (0x7A11, 20000, 2.0, 20), "zipf": (0x21FF, 10000, 1.5, 2000)}.items():
+        rp, ci, v = zipf(seed, n, s, kmax)
+        x = uniform(seed, 77, n)
+        out[f"{tag}_params"] = np.array([seed, n, kmax], np.int64)
+        out[f"{tag}_s"] = np.float64(s)
+        out[f"{tag}_y"] = oracle.ref_tcsrmv(rp, ci, v, x, n)
+        out[f"{tag}_y_f32"] = oracle.ref_tcsrmv(rp, ci, v.astype(np.float32), x.astype(np.float32), n)
+    np.savez_compressed(HERE / "spmv_trans.npz", x_stream=np.int64(77), **out)
+
+
 def solve_fixtures():
     # closed-form solves of the reference (LSE.h solve1x1 .. solve6x6), well-conditioned
     # systems plus one singular system per size
@@ -103,8 +118,13 @@ def main() -> None:
     if not oracle.have_ref():
         raise SystemExit("oracle/_ref/libblazeref.so is missing (needs the reference tree)")
     oracle.ref().ref_set_threads(1)
+    if len(sys.argv) > 1:  # named fixture sets only, e.g. spmv_trans_fixtures
+        for name in sys.argv[1:]:
+            globals()[name]()
+        return
     gemm_fixtures()
     spmv_fixtures()
+    spmv_trans_fixtures()
     solve_fixtures()
     for f in sorted(HERE.glob("*.npz")):
         print(f"{f.name}: {f.stat().st_size} bytes")

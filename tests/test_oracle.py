@@ -181,6 +181,40 @@ def test_spmv_equals_dense_product_of_same_matrix():
     assert np.array_equal(oracle.csrmv(rp, ci, v.astype(np.int32), x), D @ x)
 
 
+def _trans_case(g, tag):
+    seed, n, kmax = (int(v) for v in g[f"{tag}_params"])
+    rp, ci, v = zipf(seed, n, float(g[f"{tag}_s"]), kmax)
+    x = uniform(seed, int(g["x_stream"]), n)
+    return rp, ci, v, x, n
+
+
+def test_spmv_trans_golden_bit_exact():
+    # trans(S) * x (TSMatDVecMultExpr): the oracle's loop against the reference's own results
+    g = load("spmv_trans.npz")
+    for tag in ("short", "zipf"):
+        rp, ci, v, x, n = _trans_case(g, tag)
+        y = oracle.tcsrmv(rp, ci, v, x, n)
+        assert np.array_equal(y, g[f"{tag}_y"]) and not np.any(np.signbit(y) & (y == 0))
+        assert np.array_equal(oracle.tcsrmv(rp, ci, v.astype(np.float32), x.astype(np.float32), n),
+                              g[f"{tag}_y_f32"])
+
+
+def test_spmv_trans_equals_dense_transpose_product():
+    # integer values: exact, whatever the order; an all-zero product sum is +0 (the reference
+    # starts every entry from a reset +0)
+    rng = np.random.default_rng(5)
+    rp, ci, v = zipf(31, 400, 1.5, 60)
+    vi = rng.integers(-9, 10, len(v)).astype(np.float64)
+    n = len(rp) - 1
+    D = np.zeros((n, n))
+    for i in range(n):
+        D[i, ci[rp[i]:rp[i + 1]]] = vi[rp[i]:rp[i + 1]]
+    x = rng.integers(-9, 10, n).astype(np.float64)
+    assert np.array_equal(oracle.tcsrmv(rp, ci, vi, x, n), D.T @ x)
+    y0 = oracle.tcsrmv(rp, ci, -np.ones_like(vi), np.zeros(n), n)
+    assert np.all(y0 == 0) and not np.any(np.signbit(y0))
+
+
 def test_spmv_empty_rows():
     rp = np.array([0, 0, 2, 2, 3], np.int32)
     ci = np.array([0, 3, 1], np.int32)
@@ -298,6 +332,7 @@ def test_oracle_matches_compiled_reference_kernels():
     rp, ci, v = zipf(77, 3000, 1.5, 500)
     x = rng.uniform(-1, 1, 3000)
     assert np.array_equal(oracle.csrmv(rp, ci, v, x), oracle.ref_csrmv(rp, ci, v, x, 3000))
+    assert np.array_equal(oracle.tcsrmv(rp, ci, v, x, 3000), oracle.ref_tcsrmv(rp, ci, v, x, 3000))
     for n in range(1, 4):
         M = rng.uniform(-1, 1, (n, n)) + n * np.eye(n)
         b = rng.uniform(-1, 1, n)
