@@ -44,6 +44,7 @@ namespace blaze {
 namespace gpu {
 
 constexpr bool rowMajor = false;     // Blaze's storage-order flag values
+constexpr bool columnMajor = true;
 constexpr bool columnVector = false;
 
 // ---------------------------------------------------------------------------------------
@@ -114,9 +115,9 @@ inline bzg_context* context() {
 inline void setFp64AccurateGemm(bool on) { detail::fp64_accurate_flag() = on; }
 inline bool fp64AccurateGemm() { return detail::fp64_accurate_flag(); }
 
-template <typename T> class DynamicMatrix;
+template <typename T, bool SO = rowMajor> class DynamicMatrix;
 template <typename T> class DynamicVector;
-template <typename T> class CompressedMatrix;
+template <typename T, bool SO = rowMajor> class CompressedMatrix;
 
 // ---------------------------------------------------------------------------------------
 // type dispatch onto the C ABI
@@ -236,8 +237,8 @@ struct DMatDVecSolveExpr {
 // ---------------------------------------------------------------------------------------
 // DynamicMatrix
 // ---------------------------------------------------------------------------------------
-template <typename T>
-class DynamicMatrix {
+template <typename T, bool SO>
+class DynamicMatrix {  // row-major (the column-major form is the specialisation below)
   static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value ||
                     std::is_same<T, int>::value,
                 "blaze::gpu::DynamicMatrix supports double, float and int");
@@ -538,8 +539,8 @@ class DynamicVector {
 // ---------------------------------------------------------------------------------------
 // CompressedMatrix (row-major): host build API of the reference, CSR pack on first use
 // ---------------------------------------------------------------------------------------
-template <typename T>
-class CompressedMatrix {
+template <typename T, bool SO>
+class CompressedMatrix {  // row-major (the column-major form is the specialisation below)
  public:
   using ElementType = T;
   CompressedMatrix() = default;
@@ -803,8 +804,8 @@ inline SMatDVecMultExpr<T> operator*(const CompressedMatrix<T>& S, const Dynamic
   return {S, x};
 }
 
-template <typename T>
-DynamicMatrix<T>& DynamicMatrix<T>::operator*=(const DynamicMatrix<T>& rhs) {
+template <typename T, bool SO>
+DynamicMatrix<T, SO>& DynamicMatrix<T, SO>::operator*=(const DynamicMatrix<T, SO>& rhs) {
   if (n_ != rhs.rows()) throw std::invalid_argument("Matrix sizes do not match");
   DynamicMatrix tmp;
   tmp = (*this) * rhs;
@@ -984,6 +985,157 @@ DynamicVector<T>& DynamicVector<T>::operator=(const DMatDVecSolveExpr<T>& e) {
   static_assert(std::is_same<T, double>::value, "solve() is provided for double");
   solve(e.mat, *this, e.rhs);
   return *this;
+}
+
+// ---------------------------------------------------------------------------------------
+// column-major containers (Blaze's columnMajor storage order, DynamicMatrix.h / CompressedMatrix.h:3118)
+// ---------------------------------------------------------------------------------------
+// A column-major m x n matrix stores its columns contiguously: exactly the row-major storage of
+// its n x m transpose.  The column-major containers therefore hold that row-major transpose
+// (`storage()`), and every product is the transposed-operand form of the row-major kernels:
+// A * B with column-major operands runs bzg_*gemm_ex on the stored buffers, A * x runs
+// bzg_*gemv_t, S * x runs the column-order sparse product of trans(storage) (sums from +0 in
+// the order of the columns, TSMatDVecMultExpr.h:329-383).  No data is transposed on the device.
+template <typename T>
+class DynamicMatrix<T, columnMajor> {
+ public:
+  using ElementType = T;
+  DynamicMatrix() = default;
+  DynamicMatrix(size_t m, size_t n) : t_(n, m) {}
+  DynamicMatrix(size_t m, size_t n, const T& init) : t_(n, m, init) {}
+  DynamicMatrix(size_t m, size_t n, const T* host_column_major) : t_(n, m, host_column_major) {}
+  DynamicMatrix(const DMatDMatMultExpr<T>& e) { *this = e; }
+
+  size_t rows() const { return t_.columns(); }
+  size_t columns() const { return t_.rows(); }
+  size_t spacing() const { return t_.columns(); }  // leading dimension: the column length
+  T* data() { return t_.data(); }
+  const T* data() const { return t_.data(); }
+  void resize(size_t m, size_t n, bool preserve = true) { t_.resize(n, m, preserve); }
+  void upload(const T* host_column_major) { t_.upload(host_column_major); }
+  std::vector<T> toVector() const { return t_.toVector(); }  // column-major order
+  T operator()(size_t i, size_t j) const { return t_(j, i); }
+  size_t nonZeros() const { return t_.nonZeros(); }
+  bool isAliased(const void* p) const { return p == this || t_.isAliased(p); }
+  // the row-major n x m transpose that is this matrix's storage
+  const DynamicMatrix<T>& storage() const { return t_; }
+  DynamicMatrix<T>& storage() { return t_; }
+
+  // C = L * R stored as C^T = R^T * L^T: the same GEMM with the operands exchanged and each
+  // operand's transposition flipped
+  DynamicMatrix& operator=(const DMatDMatMultExpr<T>& e) {
+    t_ = DMatDMatMultExpr<T>{e.rhs, e.lhs, e.scale, !e.tb, !e.ta};
+    return *this;
+  }
+  DynamicMatrix& operator+=(const DMatDMatMultExpr<T>& e) {
+    t_ += DMatDMatMultExpr<T>{e.rhs, e.lhs, e.scale, !e.tb, !e.ta};
+    return *this;
+  }
+  DynamicMatrix& operator-=(const DMatDMatMultExpr<T>& e) {
+    t_ -= DMatDMatMultExpr<T>{e.rhs, e.lhs, e.scale, !e.tb, !e.ta};
+    return *this;
+  }
+
+ private:
+  DynamicMatrix<T> t_;
+};
+
+// products with column-major dense operands (DMatTDMatMultExpr, TDMatDMatMultExpr,
+// TDMatTDMatMultExpr, TDMatDVecMultExpr)
+template <typename T>
+inline DMatDMatMultExpr<T> operator*(const DynamicMatrix<T, columnMajor>& A, const DynamicMatrix<T>& B) {
+  if (A.columns() != B.rows()) throw std::invalid_argument("Matrix sizes do not match");
+  return {A.storage(), B, T(1), true, false};
+}
+template <typename T>
+inline DMatDMatMultExpr<T> operator*(const DynamicMatrix<T>& A, const DynamicMatrix<T, columnMajor>& B) {
+  if (A.columns() != B.rows()) throw std::invalid_argument("Matrix sizes do not match");
+  return {A, B.storage(), T(1), false, true};
+}
+template <typename T>
+inline DMatDMatMultExpr<T> operator*(const DynamicMatrix<T, columnMajor>& A,
+                                     const DynamicMatrix<T, columnMajor>& B) {
+  if (A.columns() != B.rows()) throw std::invalid_argument("Matrix sizes do not match");
+  return {A.storage(), B.storage(), T(1), true, true};
+}
+template <typename T>
+inline DMatDVecMultExpr<T> operator*(const DynamicMatrix<T, columnMajor>& A, const DynamicVector<T>& x) {
+  if (A.columns() != x.size()) throw std::invalid_argument("Matrix and vector sizes do not match");
+  return {A.storage(), x, T(1), true};
+}
+// trans() of a column-major matrix is the row-major matrix in its storage
+template <typename T>
+inline const DynamicMatrix<T>& trans(const DynamicMatrix<T, columnMajor>& A) {
+  return A.storage();
+}
+
+// solve / det / inv for a column-major system matrix: A's storage is A^T, whose row-major
+// transpose is A (one device transpose); det and inv use that det(A^T) = det(A) and
+// inv(A^T) = inv(A)^T, so they run on the storage directly (the factorisation is then of A
+// itself, as the reference's getrf on column-major storage)
+inline DynamicMatrix<double> rowMajorCopy(const DynamicMatrix<double, columnMajor>& A) {
+  DynamicMatrix<double> R(A.rows(), A.columns());
+  if (A.rows() * A.columns())
+    detail::check(bzg_dtranspose(context(), A.columns(), A.rows(), A.data(), A.rows(), R.data(), A.columns()),
+                  "transpose");
+  return R;
+}
+inline void solve(const DynamicMatrix<double, columnMajor>& A, DynamicVector<double>& x,
+                  const DynamicVector<double>& b) {
+  solve(rowMajorCopy(A), x, b);
+}
+inline double det(const DynamicMatrix<double, columnMajor>& A) {
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
+  return det(A.storage());
+}
+inline void invert(DynamicMatrix<double, columnMajor>& A) {
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
+  invert(A.storage());
+}
+inline DynamicMatrix<double, columnMajor> inv(const DynamicMatrix<double, columnMajor>& A) {
+  DynamicMatrix<double, columnMajor> R(A.rows(), A.columns());
+  R.storage() = A.storage();
+  invert(R);
+  return R;
+}
+
+// Column-major sparse matrix: the build API runs by columns (append(i, j, v) with rows
+// strictly increasing inside column j, then finalize(j)); the storage is the row-major
+// CompressedMatrix of the transpose, and S * x is the reference's column-major product.
+template <typename T>
+class CompressedMatrix<T, columnMajor> {
+ public:
+  using ElementType = T;
+  CompressedMatrix() = default;
+  CompressedMatrix(size_t m, size_t n) : t_(n, m) {}
+  CompressedMatrix(size_t m, size_t n, size_t nonzeros) : t_(n, m, nonzeros) {}
+  size_t rows() const { return t_.columns(); }
+  size_t columns() const { return t_.rows(); }
+  size_t nonZeros() const { return t_.nonZeros(); }
+  size_t nonZeros(size_t j) const { return t_.nonZeros(j); }  // of column j
+  void reserve(size_t nonzeros) { t_.reserve(nonzeros); }
+  void resize(size_t m, size_t n) { t_.resize(n, m); }
+  void append(size_t i, size_t j, const T& value, bool check = false) {
+    if (i >= rows() || j >= columns()) throw std::out_of_range("Invalid compressed matrix access index");
+    t_.append(j, i, value, check);
+  }
+  void finalize(size_t j) {
+    if (j >= columns()) throw std::out_of_range("Invalid column access index");
+    t_.finalize(j);
+  }
+  void set(size_t i, size_t j, const T& value) { t_.set(j, i, value); }
+  void insert(size_t i, size_t j, const T& value) { t_.insert(j, i, value); }
+  T operator()(size_t i, size_t j) const { return t_(j, i); }
+  const CompressedMatrix<T>& storage() const { return t_; }
+
+ private:
+  CompressedMatrix<T> t_;
+};
+
+template <typename T>
+inline TSMatDVecMultExpr<T> operator*(const CompressedMatrix<T, columnMajor>& S, const DynamicVector<T>& x) {
+  if (S.columns() != x.size()) throw std::invalid_argument("Matrix and vector sizes do not match");
+  return {S.storage(), x};
 }
 
 }  // namespace gpu

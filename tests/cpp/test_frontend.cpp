@@ -393,6 +393,130 @@ int main() {
     }
     check(ok, "y -= A * x");
   }
+  // column-major containers: products over all storage-order combinations against a host
+  // product (DMatTDMatMultExpr / TDMatDMatMultExpr / TDMatTDMatMultExpr, TDMatDVecMultExpr),
+  // aliasing, solve / det / inv, and a column-built sparse matrix bit-exact to the reference's
+  // column-order loop (TSMatDVecMultExpr.h:374-383)
+  {
+    const size_t m = 45, k = 38, n = 29;
+    std::vector<double> a(m * k), b(k * n);  // row-major host copies
+    for (auto& v : a) v = U(rng);
+    for (auto& v : b) v = U(rng);
+    auto colmajor = [](const std::vector<double>& r, size_t rows, size_t cols) {
+      std::vector<double> c(rows * cols);
+      for (size_t i = 0; i < rows; ++i)
+        for (size_t j = 0; j < cols; ++j) c[j * rows + i] = r[i * cols + j];
+      return c;
+    };
+    std::vector<double> want(m * n, 0.0);
+    for (size_t i = 0; i < m; ++i)
+      for (size_t j = 0; j < n; ++j)
+        for (size_t q = 0; q < k; ++q) want[i * n + j] += a[i * k + q] * b[q * n + j];
+    DynamicMatrix<double> Ar(m, k, a.data()), Br(k, n, b.data());
+    DynamicMatrix<double, columnMajor> Ac(m, k, colmajor(a, m, k).data()), Bc(k, n, colmajor(b, k, n).data());
+    bool ok = Ac.rows() == m && Ac.columns() == k && Ac(3, 5) == a[3 * k + 5];
+    auto close_rm = [&](const std::vector<double>& g) {
+      bool r = g.size() == m * n;
+      for (size_t i = 0; i < m * n && r; ++i) r = relaxed(g[i], want[i], 1e-12);
+      return r;
+    };
+    auto close_cm = [&](const std::vector<double>& g) {
+      bool r = g.size() == m * n;
+      for (size_t i = 0; i < m && r; ++i)
+        for (size_t j = 0; j < n && r; ++j) r = relaxed(g[j * m + i], want[i * n + j], 1e-12);
+      return r;
+    };
+    DynamicMatrix<double> C1;
+    C1 = Ac * Br; ok = ok && close_rm(C1.toVector());
+    C1 = Ar * Bc; ok = ok && close_rm(C1.toVector());
+    C1 = Ac * Bc; ok = ok && close_rm(C1.toVector());
+    DynamicMatrix<double, columnMajor> C2;
+    C2 = Ar * Br; ok = ok && C2.rows() == m && C2.columns() == n && close_cm(C2.toVector());
+    C2 = Ac * Bc; ok = ok && close_cm(C2.toVector());
+    C2 = Ac * Br; ok = ok && close_cm(C2.toVector());
+    // aliasing: B = A * B with column-major B (square A)
+    std::vector<double> sq(k * k);
+    for (auto& v : sq) v = U(rng);
+    DynamicMatrix<double, columnMajor> Sq(k, k, colmajor(sq, k, k).data()), Bc2(k, n, colmajor(b, k, n).data());
+    Bc2 = Sq * Bc2;
+    {
+      auto g = Bc2.toVector();
+      for (size_t i = 0; i < k && ok; ++i)
+        for (size_t j = 0; j < n && ok; ++j) {
+          double s = 0.0;
+          for (size_t q = 0; q < k; ++q) s += sq[i * k + q] * b[q * n + j];
+          ok = relaxed(g[j * k + i], s, 1e-12);
+        }
+    }
+    // y = A * x, column-major A
+    std::vector<double> x(k);
+    for (auto& v : x) v = U(rng);
+    DynamicVector<double> X(k, x.data()), Y;
+    Y = Ac * X;
+    {
+      auto g = Y.toVector();
+      for (size_t i = 0; i < m && ok; ++i) {
+        double s = 0.0;
+        for (size_t q = 0; q < k; ++q) s += a[i * k + q] * x[q];
+        ok = relaxed(g[i], s, 1e-12);
+      }
+    }
+    // solve / det / inv with a column-major system matrix
+    std::vector<double> sa(k * k);
+    for (size_t i = 0; i < k; ++i)
+      for (size_t j = 0; j < k; ++j) sa[i * k + j] = U(rng) + (i == j ? 5.0 : 0.0);
+    DynamicMatrix<double> Sr(k, k, sa.data());
+    DynamicMatrix<double, columnMajor> Sc(k, k, colmajor(sa, k, k).data());
+    DynamicVector<double> xs, xr;
+    solve(Sc, xs, X);
+    solve(Sr, xr, X);
+    {
+      auto g1 = xs.toVector(), g2 = xr.toVector();
+      for (size_t i = 0; i < k && ok; ++i) ok = relaxed(g1[i], g2[i], 1e-12);
+    }
+    ok = ok && relaxed(det(Sc), det(Sr), 1e-12);
+    {
+      auto Ic = inv(Sc);
+      auto Ir = inv(Sr);
+      auto g1 = Ic.toVector(), g2 = Ir.toVector();
+      for (size_t i = 0; i < k && ok; ++i)
+        for (size_t j = 0; j < k && ok; ++j) ok = relaxed(g1[j * k + i], g2[i * k + j], 1e-12);
+    }
+    check(ok, "column-major DynamicMatrix products, aliasing, solve / det / inv");
+  }
+  {
+    const size_t m = 300, n = 260;
+    CompressedMatrix<double, columnMajor> S(m, n);
+    std::vector<std::vector<std::pair<size_t, double>>> cols(n);
+    std::uniform_int_distribution<int> len(0, 8), row(0, (int)m - 1);
+    for (size_t j = 0; j < n; ++j) {
+      std::vector<size_t> rs;
+      for (int q = len(rng); q > 0; --q) rs.push_back((size_t)row(rng));
+      std::sort(rs.begin(), rs.end());
+      rs.erase(std::unique(rs.begin(), rs.end()), rs.end());
+      for (size_t i : rs) {
+        const double v = U(rng);
+        S.append(i, j, v);
+        cols[j].push_back({i, v});
+      }
+      S.finalize(j);
+    }
+    std::vector<double> x(n);
+    for (auto& v : x) v = U(rng);
+    DynamicVector<double> X(n, x.data()), Y;
+    Y = S * X;
+    std::vector<double> want(m, 0.0);
+    for (size_t j = 0; j < n; ++j)
+      for (auto& e : cols[j]) {
+        const double prod = e.second * x[j];
+        want[e.first] = want[e.first] + prod;
+      }
+    auto g = Y.toVector();
+    bool ok = Y.size() == m && S.rows() == m && S.columns() == n;
+    for (size_t i = 0; i < m && ok; ++i) ok = std::memcmp(&g[i], &want[i], sizeof(double)) == 0;
+    if (!cols[7].empty()) ok = ok && S(cols[7][0].first, 7) == cols[7][0].second;
+    check(ok, "column-major CompressedMatrix: S * x bit-identical to the reference loop");
+  }
   // y = trans(A) * x (TDMatDVecMultExpr)
   {
     const size_t m = 300, n = 77;
