@@ -37,7 +37,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "FP64-accurate GEMM TFLOP/s vs native FP64; batched-solve p50/p99 at 1e-12"
 SEED = 128753984
-PEAK = {"fp64": 78.6, "bf16": 2500.0, "int8": 5000.0, "hbm": 8000.0}
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "int8": 5000.0, "hbm": 8000.0}
 
 
 def log(*a):
@@ -265,6 +265,20 @@ class Bench:
             out[key] = rec
         out["speedup_mixed_vs_fp64"] = out["mixed_fp32_lu_fp64_ir"]["speedup_vs_fp64"]
         out["fp64_lu"]["max_backward_error"] = self._backward_errors(n, r, a_off, b_off, dA0, dB0, X64)
+        # both roofline fractions: getrf + getrs flops against the published FP64 (FP32 for the
+        # mixed solver's factorisation) peak, and the algorithmic HBM bytes against 8 TB/s.
+        # FP64 LU: A read and its factors written (16 n^2 B) plus B read and X written
+        # (16 n nrhs B); mixed: A read in FP64, its FP32 copy written and read back (8 + 4 + 4
+        # n^2 B), plus B read and X written.
+        n2 = n.astype(np.float64) ** 2
+        nr = n.astype(np.float64) * r
+        bytes_fp64 = float(np.sum(16 * n2 + 16 * nr))
+        bytes_mixed = float(np.sum(16 * n2 + 16 * nr))
+        for key, peak, by in (("fp64_lu", PEAK["fp64"], bytes_fp64), ("mixed_fp32_lu_fp64_ir", PEAK["fp32"], bytes_mixed),
+                              ("mixed_lapack_eps", PEAK["fp32"], bytes_mixed)):
+            sec = out[key]["p50_ms"] / 1e3
+            out[key]["roofline"] = {"compute_frac": flops / sec / 1e12 / peak, "compute_peak_tflops": peak,
+                                    "hbm_frac": by / sec / 1e9 / PEAK["hbm"], "bytes": by}
         return out
 
     def _lat_stats(self, lat, count, flops):
