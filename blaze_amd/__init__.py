@@ -52,6 +52,8 @@ SIGNATURES = {
     "bzg_csr_plan_create": (i32, [vp, i64, i64, vp, C.POINTER(vp)]),
     "bzg_csr_plan_set_sum_start": (i32, [vp, i32]),
     "bzg_csc_to_csr_host": (i32, [i64, i64, vp, vp, vp, i32, vp, vp, vp]),
+    "bzg_mm_read_csr": (i32, [C.c_char_p, vp]),
+    "bzg_csr_host_free": (i32, [vp]),
     "bzg_csr_plan_destroy": (i32, [vp]),
     "bzg_csr_plan_tune": (i32, [vp, i32, i32, i32]),
     "bzg_csr_plan_describe": (i32, [vp, C.c_char_p, C.c_size_t]),
@@ -171,3 +173,24 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class CsrHost(C.Structure):
+    _fields_ = [("n_rows", i64), ("n_cols", i64), ("nnz", i64), ("rowptr", C.POINTER(i32)),
+                ("colidx", C.POINTER(i32)), ("val", C.POINTER(f64))]
+
+
+def read_matrix_market(path):
+    """(rowptr, colidx, val, (n_rows, n_cols)) of a Matrix Market coordinate file, read by the
+    library's bzg_mm_read_csr (host arrays copied into numpy, then freed)."""
+    import numpy as np
+    h = CsrHost()
+    check("bzg_mm_read_csr", lib().bzg_mm_read_csr(str(path).encode(), C.byref(h)))
+    shape = (int(h.n_rows), int(h.n_cols))
+    try:
+        rp = np.ctypeslib.as_array(h.rowptr, (h.n_rows + 1,)).copy()
+        ci = np.ctypeslib.as_array(h.colidx, (max(h.nnz, 1),))[:h.nnz].copy()
+        va = np.ctypeslib.as_array(h.val, (max(h.nnz, 1),))[:h.nnz].copy()
+    finally:
+        lib().bzg_csr_host_free(C.byref(h))  # clears h
+    return rp, ci, va, shape

@@ -37,7 +37,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["context.hip", "gemm.hip", "gemv.hip", "spmv.hip", "cg.hip", "lu.hip", "ozaki.hip",
-               "server.hip"]
+               "server.hip", "mmio.hip"]
 
 
 def _run(cmd: list[str]) -> None:

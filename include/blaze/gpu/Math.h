@@ -718,6 +718,21 @@ class CompressedMatrix {  // row-major (the column-major form is the specialisat
   detail::DeviceBuffer<T> d_tval_;
 };
 
+// S = the matrix of a Matrix Market coordinate file (bzg_mm_read_csr: general / symmetric /
+// skew-symmetric, real / integer / pattern, duplicates summed); std::invalid_argument or
+// std::runtime_error with the reader's message on failure
+inline void readMatrixMarket(const std::string& path, CompressedMatrix<double>& S) {
+  bzg_csr_host h{};
+  detail::check(bzg_mm_read_csr(path.c_str(), &h), "readMatrixMarket");
+  try {
+    S.assignCSR((size_t)h.n_rows, (size_t)h.n_cols, h.rowptr, h.colidx, h.val);
+  } catch (...) {
+    bzg_csr_host_free(&h);
+    throw;
+  }
+  bzg_csr_host_free(&h);
+}
+
 template <typename T>
 void DynamicVector<T>::assign(const TSMatDVecMultExpr<T>& e) {
   const size_t n = e.mat.columns();

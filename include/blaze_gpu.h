@@ -202,6 +202,24 @@ bzg_status bzg_csr_plan_set_sum_start(bzg_csr_plan* plan, int32_t positive_zero)
 bzg_status bzg_csc_to_csr_host(int64_t n_rows, int64_t n_cols, const int32_t* colptr, const int32_t* rowidx,
                                const void* val, int32_t elem_bytes, int32_t* rowptr, int32_t* colidx,
                                void* val_out);
+/* ---- Matrix Market input (host) ------------------------------------------------------------
+ * No reference function (the reference points to an external Matrix Market project for its
+ * sparse inputs).  Reads a "coordinate" file -- real / double / integer / pattern values (pattern
+ * entries are 1.0), general / symmetric / skew-symmetric (expanded to both triangles) -- into a
+ * host CSR: rows in order, columns ascending, duplicate entries summed in file order.  The
+ * arrays are malloc'ed by the library; bzg_csr_host_free releases them.  Errors: unreadable
+ * file, bad header or entry, index out of range, fewer entries than declared
+ * (BZG_ERR_INVALID_ARGUMENT); dense "array" files, complex / hermitian fields, sizes >= 2^31
+ * (BZG_ERR_UNSUPPORTED). */
+typedef struct {
+  int64_t n_rows, n_cols, nnz;
+  int32_t* rowptr; /* n_rows + 1 */
+  int32_t* colidx; /* nnz */
+  double* val;     /* nnz */
+} bzg_csr_host;
+bzg_status bzg_mm_read_csr(const char* path, bzg_csr_host* out);
+bzg_status bzg_csr_host_free(bzg_csr_host* matrix);
+
 bzg_status bzg_csr_plan_destroy(bzg_csr_plan* plan);
 /* Overrides the vector-CSR shape chosen from the histogram: lanes per row in {1,2,4,8,16,32},
  * rows per lane group in {1,2}, nonzeros loaded per lane per batch in {1,2,4,8}; used by the

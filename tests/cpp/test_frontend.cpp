@@ -517,6 +517,25 @@ int main() {
     if (!cols[7].empty()) ok = ok && S(cols[7][0].first, 7) == cols[7][0].second;
     check(ok, "column-major CompressedMatrix: S * x bit-identical to the reference loop");
   }
+  // readMatrixMarket: a symmetric coordinate file expanded to both triangles, then S * x
+  {
+    const char* path = "/tmp/bzg_frontend_test.mtx";
+    FILE* f = std::fopen(path, "w");
+    bool ok = f != nullptr;
+    if (f) {
+      std::fputs("%%MatrixMarket matrix coordinate real symmetric\n% comment\n3 3 4\n1 1 2\n2 1 -1\n3 2 -1\n3 3 2\n", f);
+      std::fclose(f);
+      CompressedMatrix<double> S;
+      readMatrixMarket(path, S);
+      DynamicVector<double> X(3, 1.0), Y;
+      Y = S * X;
+      ok = S.rows() == 3 && S.columns() == 3 && S.nonZeros() == 6 && S(0, 1) == -1.0 && S(1, 0) == -1.0 &&
+           Y.toVector() == std::vector<double>{1.0, -2.0, 1.0};
+      std::remove(path);
+      ok = ok && throws<std::invalid_argument>([&] { CompressedMatrix<double> Z; readMatrixMarket(path, Z); }, "");
+    }
+    check(ok, "readMatrixMarket");
+  }
   // y = trans(A) * x (TDMatDVecMultExpr)
   {
     const size_t m = 300, n = 77;
