@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void csr_vector_kernel(int64_t n_rows, const i
                                                          const int32_t* __restrict__ colidx,
                                                          const T* __restrict__ val,
                                                          const T* __restrict__ x,
-                                                         T* __restrict__ y) {
+                                                         T* __restrict__ y, T s0) {
   const int lig = threadIdx.x % G;
   const int group_base = (threadIdx.x % 64) - lig;  // first lane of this group in the wave
   const int64_t g = (xcd_block() * blockDim.x + threadIdx.x) / G;
@@ -115,10 +115,11 @@ __global__ __launch_bounds__(256) void csr_vector_kernel(int64_t n_rows, const i
     }
     batches = max(batches, (len[r] + BATCH - 1) / BATCH);
   }
-  // Starting from -0 reproduces sp = p0 exactly (-0 + p == p for every p, also p = +-0).
+  // s0 = -0 reproduces sp = p0 exactly (-0 + p == p for every p, also p = +-0); s0 = +0 is the
+  // reset-then-accumulate of the column-major product.
   T s[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) s[r] = neg_zero<T>();
+  for (int r = 0; r < R; ++r) s[r] = s0;
   for (int32_t b = 0; b < batches; ++b) {
     T v[R][CAP];
     int32_t c[R][CAP];
@@ -174,7 +175,8 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n_blocks, const 
                                                         const int32_t* __restrict__ rowptr,
                                                         const int32_t* __restrict__ colidx,
                                                         const T* __restrict__ val,
-                                                        const T* __restrict__ x, T* __restrict__ y) {
+                                                        const T* __restrict__ x, T* __restrict__ y,
+                                                        T s0) {
   __shared__ T prod[BNNZ];
   constexpr int PER = BNNZ / 256;
   const int tid = threadIdx.x;
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n_blocks, const 
   for (int h = 0; h < 2; ++h) {
     const int t = tid + 256 * h;
     if (t < nrow) {
-      T s = neg_zero<T>();
+      T s = s0;
       for (int32_t k = rs[h]; k < re[h]; ++k) s = add_rn(s, prod[k]);
       y[cur.x + t] = (re[h] == rs[h]) ? T(0) : s;
     }
@@ -275,14 +277,6 @@ __global__ __launch_bounds__(256) void csr_rows_kernel(int64_t n_rows, const int
   }
 }
 
-// y += +0 on every entry: turns the -0 that a row of only -0 products leaves into +0 and keeps
-// every other value, so the sums equal those of an accumulation that starts from +0.
-template <typename T>
-__global__ __launch_bounds__(256) void plus_zero_kernel(int64_t n, T* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) y[i] = y[i] + T(0);
-}
-
 // y = 0 for a matrix without nonzeros (x may then be null, so no SpMV kernel runs).
 template <typename T>
 __global__ __launch_bounds__(256) void zero_rows_kernel(int64_t n, T* __restrict__ y) {
@@ -304,23 +298,26 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
     BZG_LAUNCH_CHECK();
     return BZG_OK;
   }
+  // start of the sequential row sums: -0 (SMatDVecMultExpr: the first product itself) or +0
+  // (a column-major product: y reset, then accumulated)
+  const T s0 = plan->plus_zero ? T(0) : T(-T(0));
   if (plan->n_short > 0 && plan->mode == 0) {
     const int64_t blocks = plan->n_blocks;  // one workgroup per row block
     BZG_REQUIRE(blocks < (1LL << 31), BZG_ERR_UNSUPPORTED, "csrmv: too many row blocks");
     if (plan->block_nnz == 4096) {
       if (plan->block_nt)
         hipLaunchKernelGGL((csr_block_kernel<T, 4096, true>), dim3((unsigned)blocks), dim3(256), 0,
-                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y, s0);
       else
         hipLaunchKernelGGL((csr_block_kernel<T, 4096, false>), dim3((unsigned)blocks), dim3(256), 0,
-                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y, s0);
     } else {
       if (plan->block_nt)
         hipLaunchKernelGGL((csr_block_kernel<T, 2048, true>), dim3((unsigned)blocks), dim3(256), 0,
-                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y, s0);
       else
         hipLaunchKernelGGL((csr_block_kernel<T, 2048, false>), dim3((unsigned)blocks), dim3(256), 0,
-                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y);
+                           ctx->stream, plan->n_blocks, plan->blocks, rowptr, colidx, val, x, y, s0);
     }
     BZG_LAUNCH_CHECK();
   } else if (plan->n_short > 0) {
@@ -334,7 +331,7 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
 #define BZG_VEC(GG, RR, CC)                                                                    \
   if (G == GG && R == RR && CP == CC) {                                                        \
     hipLaunchKernelGGL((csr_vector_kernel<T, GG, RR, CC>), grid, block, 0, ctx->stream,         \
-                       plan->n_short, plan->short_rows, rowptr, colidx, val, x, y);             \
+                       plan->n_short, plan->short_rows, rowptr, colidx, val, x, y, s0);         \
     launched = true;                                                                           \
   }
 #define BZG_VEC_C(GG, RR) BZG_VEC(GG, RR, 1) BZG_VEC(GG, RR, 2) BZG_VEC(GG, RR, 4) BZG_VEC(GG, RR, 8)
@@ -360,11 +357,6 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
     const int64_t blocks = (plan->n_huge + 7) / 8 * 8;
     hipLaunchKernelGGL((csr_rows_kernel<T, 256, 8>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
                        plan->n_huge, plan->huge_rows, rowptr, colidx, val, x, y);
-    BZG_LAUNCH_CHECK();
-  }
-  if (plan->plus_zero && !std::is_integral<T>::value) {
-    hipLaunchKernelGGL(plus_zero_kernel<T>, dim3((unsigned)((plan->n_rows + 255) / 256)), dim3(256), 0, ctx->stream,
-                       plan->n_rows, y);
     BZG_LAUNCH_CHECK();
   }
   return BZG_OK;

@@ -191,7 +191,8 @@ bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
 /* positive_zero != 0: every y entry is the sum started from +0 (column-major products,
  * TSMatDVecMultExpr's reset-then-accumulate, blaze/math/expressions/TSMatDVecMultExpr.h:329-383)
  * instead of from the first product (SMatDVecMultExpr, SVecDVecInnerExpr.h:127-135).  The two
- * differ only in the sign of an all-zero sum: one extra pass adds +0 to y. */
+ * differ only in the sign of an all-zero sum; the kernels take the start value, so the cost is
+ * the same. */
 bzg_status bzg_csr_plan_set_sum_start(bzg_csr_plan* plan, int32_t positive_zero);
 /* Host CSC -> CSR pack (a column-major CompressedMatrix, CompressedMatrix.h:3118, into the
  * layout the device products read): colptr (n_cols + 1), rowidx, val (elem_bytes 4 or 8) in;
