@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -43,9 +45,29 @@ bool next_data_line(FILE* f, std::string& line) {
 
 }  // namespace
 
+namespace {
+bzg_status mm_read_csr(const char* path, bzg_csr_host* out);
+}  // namespace
+
 extern "C" {
 
 bzg_status bzg_mm_read_csr(const char* path, bzg_csr_host* out) {
+  // no C++ exception leaves the C ABI: allocation failures become a status
+  try {
+    return mm_read_csr(path, out);
+  } catch (const std::bad_alloc&) {
+    bzg::set_error("mm_read_csr: out of host memory");
+    return BZG_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    bzg::set_error(std::string("mm_read_csr: ") + e.what());
+    return BZG_ERR_INVALID_ARGUMENT;
+  }
+}
+
+}  // extern "C"
+
+namespace {
+bzg_status mm_read_csr(const char* path, bzg_csr_host* out) {
   BZG_REQUIRE(path && out, BZG_ERR_INVALID_ARGUMENT, "mm_read_csr: null argument");
   *out = bzg_csr_host{};
   FileCloser fc{std::fopen(path, "r")};
@@ -77,7 +99,8 @@ bzg_status bzg_mm_read_csr(const char* path, bzg_csr_host* out) {
     double v;
   };
   std::vector<E> e;
-  e.reserve((size_t)entries * (sym == "general" ? 1 : 2));
+  // the declared count is not trusted for the reservation (a short file fails below)
+  e.reserve((size_t)std::min<long long>(entries, 1LL << 24) * (sym == "general" ? 1 : 2));
   for (long long q = 0; q < entries; ++q) {
     BZG_REQUIRE(next_data_line(fc.f, line), BZG_ERR_INVALID_ARGUMENT, "mm_read_csr: fewer entries than declared");
     long long i = 0, j = 0;
@@ -130,6 +153,9 @@ bzg_status bzg_mm_read_csr(const char* path, bzg_csr_host* out) {
   out->val = val;
   return BZG_OK;
 }
+}  // namespace
+
+extern "C" {
 
 bzg_status bzg_csr_host_free(bzg_csr_host* m) {
   BZG_REQUIRE(m != nullptr, BZG_ERR_INVALID_ARGUMENT, "csr_host_free: null argument");

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <new>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -561,7 +562,13 @@ bzg_status bzg_csc_to_csr_host(int64_t n_rows, int64_t n_cols, const int32_t* co
   BZG_REQUIRE(nnz >= 0 && (nnz == 0 || (rowidx && val && colidx && val_out)), BZG_ERR_INVALID_ARGUMENT,
               "csc_to_csr_host: null array");
   // counting sort by row; columns are visited in order, so every row lists its columns ascending
-  std::vector<int64_t> cnt((size_t)n_rows + 1, 0);
+  std::vector<int64_t> cnt;
+  try {
+    cnt.assign((size_t)n_rows + 1, 0);
+  } catch (const std::bad_alloc&) {
+    bzg::set_error("csc_to_csr_host: out of host memory");
+    return BZG_ERR_OUT_OF_MEMORY;
+  }
   for (int64_t c = 0; c < n_cols; ++c) {
     BZG_REQUIRE(colptr[c + 1] >= colptr[c], BZG_ERR_INVALID_ARGUMENT, "csc_to_csr_host: colptr decreasing");
     for (int64_t e = colptr[c]; e < colptr[c + 1]; ++e) {
