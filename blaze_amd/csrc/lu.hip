@@ -23,6 +23,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <new>
 #include <vector>
 
 namespace bzg {
@@ -949,7 +950,7 @@ __global__ __launch_bounds__(256) void inv_scatter_kernel(int64_t count, int col
 
 extern "C" {
 
-bzg_status bzg_dinv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+static bzg_status dinv_batched_impl(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
                             const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info, const int32_t* order) {
   BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dinv_batched: null context");
   BZG_REQUIRE(count >= 0 && count < (1LL << 31), BZG_ERR_INVALID_ARGUMENT, "dinv_batched: bad count");
@@ -1004,6 +1005,17 @@ bzg_status bzg_dinv_batched(bzg_context* ctx, int64_t count, const int32_t* n, c
   }
   BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host copy of b_off is released on return
   return BZG_OK;
+}
+
+bzg_status bzg_dinv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
+                            const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info, const int32_t* order) {
+  // no C++ exception leaves the C ABI: a host allocation failure becomes a status
+  try {
+    return dinv_batched_impl(ctx, count, n, a_off, p_off, A, ipiv, info, order);
+  } catch (const std::bad_alloc&) {
+    bzg::set_error("dinv_batched: out of host memory");
+    return BZG_ERR_OUT_OF_MEMORY;
+  }
 }
 
 bzg_status bzg_ddet_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,

@@ -403,7 +403,7 @@ static bzg_status build_row_blocks(bzg_context* ctx, bzg_csr_plan* p, const int3
 
 extern "C" {
 
-bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
+static bzg_status csr_plan_create_impl(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
                                const int32_t* rowptr_host, bzg_csr_plan** out) {
   BZG_REQUIRE(ctx && out, BZG_ERR_INVALID_ARGUMENT, "csr_plan_create: null argument");
   BZG_REQUIRE(n_rows >= 0 && n_cols >= 0, BZG_ERR_INVALID_ARGUMENT, "csr_plan_create: negative size");
@@ -509,6 +509,17 @@ bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
                 (long long)n_huge, (long long)huge_thr);
   *out = p;
   return BZG_OK;
+}
+
+bzg_status bzg_csr_plan_create(bzg_context* ctx, int64_t n_rows, int64_t n_cols,
+                               const int32_t* rowptr_host, bzg_csr_plan** out) {
+  // no C++ exception leaves the C ABI: a host allocation failure becomes a status
+  try {
+    return csr_plan_create_impl(ctx, n_rows, n_cols, rowptr_host, out);
+  } catch (const std::bad_alloc&) {
+    bzg::set_error("csr_plan_create: out of host memory");
+    return BZG_ERR_OUT_OF_MEMORY;
+  }
 }
 
 bzg_status bzg_csr_plan_tune(bzg_csr_plan* p, int32_t lanes, int32_t rows_per_group, int32_t cap) {
