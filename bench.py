@@ -265,6 +265,38 @@ class Bench:
             out[key] = rec
         out["speedup_mixed_vs_fp64"] = out["mixed_fp32_lu_fp64_ir"]["speedup_vs_fp64"]
         out["fp64_lu"]["max_backward_error"] = self._backward_errors(n, r, a_off, b_off, dA0, dB0, X64)
+        # det() on every system (batched getrf + detNxN's rule) and inv() on the first 10^4
+        # (getrf, then getrs against identity columns, eight per pass); A restored before each call
+        p_off = np.zeros(count, np.int64); p_off[1:] = np.cumsum(n.astype(np.int64))[:-1]
+        dp = self.dev(p_off)
+        dpiv = torch.empty(int(n.sum()), dtype=torch.int32, device="cuda")
+        ddet = torch.empty(count, dtype=torch.float64, device="cuda")
+        dA = torch.empty_like(dA0)
+        getrf_flops = float(np.sum((2.0 / 3.0) * n.astype(np.float64) ** 3 - 0.5 * n.astype(np.float64) ** 2
+                                   + (5.0 / 6.0) * n))
+        for key, cnt, fn in (("det_batched", count, "bzg_ddet_batched"), ("inv_batched_1e4", 10_000, "bzg_dinv_batched")):
+            lat = []
+            for it in range(warmup + max(2, steps // 2)):
+                dA.copy_(dA0)
+                ev[0].record()
+                if fn == "bzg_ddet_batched":
+                    self.ctx.call(fn, cnt, vp(dn), vp(da), vp(dp), vp(dA), vp(dpiv), vp(dinfo), vp(ddet), vp(order))
+                else:
+                    self.ctx.call(fn, cnt, vp(dn), vp(da), vp(dp), vp(dA), vp(dpiv), vp(dinfo), C.c_void_p(0))
+                ev[1].record()
+                self.sync()
+                if it >= warmup:
+                    lat.append(ev[0].elapsed_time(ev[1]) / 1e3)
+            nn = n[:cnt].astype(np.float64)
+            fl = getrf_flops if cnt == count else float(np.sum((2.0 / 3.0) * nn ** 3 - 0.5 * nn ** 2 + (5.0 / 6.0) * nn
+                                                               + (4.0 / 3.0) * nn ** 3))
+            rec = {"systems": cnt, "p50_ms": float(np.median(lat) * 1e3), "gflops": fl / float(np.median(lat)) / 1e9,
+                   "flops": ("getrf" if cnt == count else "getrf + getri-equivalent (4/3 n^3)"),
+                   "info_nonzero": int((dinfo[:cnt] != 0).sum().item())}
+            if cnt == count:
+                rec["nonfinite_det"] = int((~torch.isfinite(ddet)).sum().item())
+            out[key] = rec
+        del dA
         # both roofline fractions: getrf + getrs flops against the published FP64 (FP32 for the
         # mixed solver's factorisation) peak, and the algorithmic HBM bytes against 8 TB/s.
         # FP64 LU: A read and its factors written (16 n^2 B) plus B read and X written
@@ -333,9 +365,19 @@ class Bench:
         x = self.uniform(n, 12)
         out["zipf1e6_s1.5_fp64"] = self._spmv_run(rp, ci, va, x, n, "bzg_dcsrmv", steps)
         out["zipf1e6_s1.5_fp64"]["mean_row"] = nnz / n
+        # y = trans(S) * x (TSMatDVecMultExpr) on the same matrix: the host pack gives the CSR of
+        # trans(S) (rows of Poisson length ~34, columns ascending), its plan sums from +0
+        trp = np.zeros(n + 1, np.int32); tci = np.zeros(nnz, np.int32); tva = np.zeros(nnz)
+        t0 = time.perf_counter()
+        bz.check("pack", bz.lib().bzg_csc_to_csr_host(n, n, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                                                      va.ctypes.data_as(C.c_void_p), 8, trp.ctypes.data_as(C.c_void_p),
+                                                      tci.ctypes.data_as(C.c_void_p), tva.ctypes.data_as(C.c_void_p)))
+        pack_s = time.perf_counter() - t0
+        out["zipf1e6_s1.5_trans_fp64"] = self._spmv_run(trp, tci, tva, x, n, "bzg_dcsrmv", steps, plus_zero=True)
+        out["zipf1e6_s1.5_trans_fp64"]["host_pack_s"] = pack_s
         return out
 
-    def _spmv_run(self, rp, ci, va, x, ncols, fn, steps):
+    def _spmv_run(self, rp, ci, va, x, ncols, fn, steps, plus_zero=False):
         torch, bz = self.torch, self.bz
         n = len(rp) - 1
         nnz = int(rp[-1])
@@ -348,6 +390,8 @@ class Bench:
         plan = C.c_void_p()
         bz.check("plan", bz.lib().bzg_csr_plan_create(self.ctx.handle, n, ncols, rp.ctypes.data_as(C.c_void_p),
                                                       C.byref(plan)))
+        if plus_zero:
+            bz.check("sum start", bz.lib().bzg_csr_plan_set_sum_start(plan, 1))
         it = [0]
 
         def one():
