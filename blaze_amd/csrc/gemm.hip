@@ -364,7 +364,11 @@ __global__ void scale_matrix_kernel(int M, int N, T beta, T* C, long ldc) {
   *p = (beta == T(0)) ? T(0) : beta * *p;
 }
 
-// INT32 GEMM on VALU: 64x64 tile, 256 threads, 4x4 micro-tile per thread.
+// INT32 GEMM on VALU: 64x64 tile, 256 threads, 4x4 micro-tile per thread.  TA / TB: the
+// operand is stored transposed (A as K x M, B as N x K); the staging loop then walks the tile so
+// that neighbouring threads read neighbouring addresses.  Wrap-around integer arithmetic is
+// exact, so every form gives the same bits as the product of transposed copies.
+template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void igemm_kernel(int M, int N, int K, uint32_t alpha,
                                                     const int32_t* __restrict__ A, long lda,
                                                     const int32_t* __restrict__ B, long ldb,
@@ -380,10 +384,14 @@ __global__ __launch_bounds__(256) void igemm_kernel(int M, int N, int K, uint32_
   uint32_t acc[4][4] = {};
   for (int k0 = 0; k0 < K; k0 += BK) {
     for (int e = tid; e < T * BK; e += 256) {
-      const int r = e / BK, c = e % BK;  // A element (m0+r, k0+c)
-      As[c][r] = (m0 + r < M && k0 + c < K) ? (uint32_t)A[(long)(m0 + r) * lda + k0 + c] : 0u;
-      const int kr = e / T, nc = e % T;  // B element (k0+kr, n0+nc)
-      Bs[kr][nc] = (k0 + kr < K && n0 + nc < N) ? (uint32_t)B[(long)(k0 + kr) * ldb + n0 + nc] : 0u;
+      // A element (m0+r, k0+c): stored at [m][k], or at [k][m] when TA
+      const int r = TA ? e % T : e / BK, c = TA ? e / T : e % BK;
+      const bool ain = m0 + r < M && k0 + c < K;
+      As[c][r] = ain ? (uint32_t)(TA ? A[(long)(k0 + c) * lda + m0 + r] : A[(long)(m0 + r) * lda + k0 + c]) : 0u;
+      // B element (k0+kr, n0+nc): stored at [k][n], or at [n][k] when TB
+      const int kr = TB ? e % BK : e / T, nc = TB ? e / BK : e % T;
+      const bool bin = k0 + kr < K && n0 + nc < N;
+      Bs[kr][nc] = bin ? (uint32_t)(TB ? B[(long)(n0 + nc) * ldb + k0 + kr] : B[(long)(k0 + kr) * ldb + n0 + nc]) : 0u;
     }
     __syncthreads();
 #pragma unroll
@@ -639,19 +647,32 @@ bzg_status bzg_sgemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_
   return bzg::gemm_dispatch<float>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, transa != 0, transb != 0);
 }
 
-bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,
-                     const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
-                     int32_t beta, int32_t* C, int64_t ldc) {
-  bzg_status st = bzg::check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        int32_t alpha, const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
+                        int32_t beta, int32_t* C, int64_t ldc) {
+  const bool ta = transa != 0, tb = transb != 0;
+  bzg_status st = bzg::check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc, ta, tb);
   if (st != BZG_OK) return st;
   if (m == 0 || n == 0) return BZG_OK;
   const long tiles = ((m + 63) / 64) * ((n + 63) / 64);
   BZG_REQUIRE(tiles < (1L << 31), BZG_ERR_UNSUPPORTED, "igemm: too many tiles");
-  hipLaunchKernelGGL(bzg::igemm_kernel, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, (int)m,
-                     (int)n, (int)k, (uint32_t)alpha, A, (long)lda, B, (long)ldb, (uint32_t)beta, C,
-                     (long)ldc);
+  const dim3 grid((unsigned)tiles), block(256);
+#define BZG_IGEMM(TA_, TB_)                                                                                  \
+  hipLaunchKernelGGL((bzg::igemm_kernel<TA_, TB_>), grid, block, 0, ctx->stream, (int)m, (int)n, (int)k,   \
+                     (uint32_t)alpha, A, (long)lda, B, (long)ldb, (uint32_t)beta, C, (long)ldc)
+  if (ta && tb) BZG_IGEMM(true, true);
+  else if (ta) BZG_IGEMM(true, false);
+  else if (tb) BZG_IGEMM(false, true);
+  else BZG_IGEMM(false, false);
+#undef BZG_IGEMM
   BZG_LAUNCH_CHECK();
   return BZG_OK;
+}
+
+bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,
+                     const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
+                     int32_t beta, int32_t* C, int64_t ldc) {
+  return bzg_igemm_ex(ctx, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 }  // extern "C"

@@ -367,6 +367,10 @@ bzg_status bzg_sgemv_t(bzg_context* ctx, int64_t m, int64_t n, float alpha, cons
                        const float* x, float beta, float* y) {
   return bzg::gemv_t_dispatch<float>(ctx, m, n, alpha, A, lda, x, beta, y);
 }
+bzg_status bzg_igemv_t(bzg_context* ctx, int64_t m, int64_t n, int32_t alpha, const int32_t* A, int64_t lda,
+                       const int32_t* x, int32_t beta, int32_t* y) {
+  return bzg::gemv_t_dispatch<int32_t>(ctx, m, n, alpha, A, lda, x, beta, y);
+}
 
 bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,
                      int64_t lda, const double* x, double beta, double* y) {

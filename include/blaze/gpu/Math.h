@@ -141,8 +141,7 @@ inline bzg_status gemm(int64_t m, int64_t n, int64_t k, float al, const float* A
 inline bzg_status gemm(int64_t m, int64_t n, int64_t k, int al, const int* A, int64_t lda,
                        const int* B, int64_t ldb, int be, int* C, int64_t ldc, bool ta = false,
                        bool tb = false) {
-  if (ta || tb) throw std::invalid_argument("trans(): int matrix products are row-major only in this build");
-  return bzg_igemm(context(), m, n, k, al, A, lda, B, ldb, be, C, ldc);
+  return bzg_igemm_ex(context(), ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
 }
 // m x n = the STORED shape of A; ta: y = A^T x (y has n entries) instead of A x
 inline bzg_status gemv(int64_t m, int64_t n, double al, const double* A, int64_t lda,
@@ -157,7 +156,7 @@ inline bzg_status gemv(int64_t m, int64_t n, float al, const float* A, int64_t l
 }
 inline bzg_status gemv(int64_t m, int64_t n, int al, const int* A, int64_t lda, const int* x,
                        int be, int* y, bool ta = false) {
-  if (ta) throw std::invalid_argument("trans(): int matrix products are row-major only in this build");
+  if (ta) return bzg_igemv_t(context(), m, n, al, A, lda, x, be, y);
   return bzg_igemv(context(), m, n, al, A, lda, x, be, y);
 }
 inline bzg_status csrmv(const bzg_csr_plan* p, const int32_t* rp, const int32_t* ci,

@@ -14,7 +14,7 @@
  * Reference interfaces replaced (paths relative to the Blaze source tree):
  *   bzg_dgemm / bzg_sgemm / bzg_igemm  <- DMatDMatMultExpr assign (blaze/math/expressions/
  *        DMatDMatMultExpr.h:516-569) and its BLAS seam cblas_dgemm (blaze/math/blas/cblas/gemm.h:161)
- *   bzg_dgemm_ex / bzg_sgemm_ex        <- the same products over column-major (transposed) operands
+ *   bzg_dgemm_ex / bzg_sgemm_ex / bzg_igemm_ex <- the same products over column-major (transposed) operands
  *        (DMatTDMatMultExpr.h, TDMatDMatMultExpr.h, TDMatTDMatMultExpr.h; CBLAS transposes)
  *   bzg_dgemv / bzg_sgemv / bzg_igemv  <- DMatDVecMultExpr assign (DMatDVecMultExpr.h:408-458)
  *        and cblas_dgemv (blaze/math/blas/cblas/gemv.h:153)
@@ -107,6 +107,11 @@ bzg_status bzg_sgemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, float al
 bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,
                      const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
                      int32_t beta, int32_t* C, int64_t ldc);
+/* bzg_igemm over transposed operands (transa / transb as bzg_dgemm_ex): INT32 wrap-around
+ * products are exact, so every form equals the product of transposed copies bit for bit. */
+bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        int32_t alpha, const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
+                        int32_t beta, int32_t* C, int64_t ldc);
 /* C = alpha * op(A) * op(B) + beta * C with op(A) = A or A^T (transa != 0: A is stored k x m,
  * lda >= m) and op(B) = B or B^T (transb != 0: B is stored n x k, ldb >= k).  Replaces the
  * products over the other storage orders (DMatTDMatMultExpr.h, TDMatDMatMultExpr.h,
@@ -167,6 +172,8 @@ bzg_status bzg_dgemv_t(bzg_context* ctx, int64_t m, int64_t n, double alpha, con
                        const double* x, double beta, double* y);
 bzg_status bzg_sgemv_t(bzg_context* ctx, int64_t m, int64_t n, float alpha, const float* A, int64_t lda,
                        const float* x, float beta, float* y);
+bzg_status bzg_igemv_t(bzg_context* ctx, int64_t m, int64_t n, int32_t alpha, const int32_t* A, int64_t lda,
+                       const int32_t* x, int32_t beta, int32_t* y);
 bzg_status bzg_dgemv(bzg_context* ctx, int64_t m, int64_t n, double alpha, const double* A,
                      int64_t lda, const double* x, double beta, double* y);
 bzg_status bzg_sgemv(bzg_context* ctx, int64_t m, int64_t n, float alpha, const float* A,

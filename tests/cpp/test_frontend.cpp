@@ -536,6 +536,55 @@ int main() {
     }
     check(ok, "readMatrixMarket");
   }
+  // smatdvecmult/AliasingTest.cpp:69-300 fixtures (sA3x4, sB3x3, da4, db3): products by hand,
+  // aliased assignment x = S * x equal to the non-aliased result, row- and column-major S
+  {
+    CompressedMatrix<double> sA(3, 4), sB(3, 3);
+    CompressedMatrix<double, columnMajor> tsA(3, 4), tsB(3, 3);
+    const double a[3][4] = {{-1, 0, -2, 0}, {0, 2, -3, 1}, {0, 1, 2, 2}};
+    const double b[3][3] = {{-1, 0, 0}, {1, -2, 2}, {0, 0, -3}};
+    for (size_t i = 0; i < 3; ++i) {
+      for (size_t j = 0; j < 4; ++j)
+        if (a[i][j] != 0) { sA.set(i, j, a[i][j]); tsA.set(i, j, a[i][j]); }
+      for (size_t j = 0; j < 3; ++j)
+        if (b[i][j] != 0) { sB.set(i, j, b[i][j]); tsB.set(i, j, b[i][j]); }
+    }
+    const std::vector<double> da4 = {-1, 0, -3, 2}, db3 = {1, 2, 3};
+    const std::vector<double> want_a = {7, 11, -2}, want_b = {-1, 3, -9};
+    DynamicVector<double> x4(4, da4.data()), r;
+    r = sA * x4;
+    bool ok = r.toVector() == want_a;
+    r = tsA * x4;
+    ok = ok && r.toVector() == want_a;
+    DynamicVector<double> x3(3, db3.data());
+    x3 = sB * x3;  // aliased
+    ok = ok && x3.toVector() == want_b;
+    DynamicVector<double> y3(3, db3.data());
+    y3 = tsB * y3;  // aliased, column-major
+    ok = ok && y3.toVector() == want_b;
+    check(ok, "smatdvecmult AliasingTest fixtures (row- and column-major, aliased)");
+  }
+  // ClassTest2.cpp:1244-1273 with column-major int operands (its row-major/column-major forms):
+  // (1 0 2; 0 3 0; 4 0 5) * (0 2 0 0; 1 3 0 4; 0 0 0 5) = (0 2 0 10; 3 9 0 12; 0 8 0 25)
+  {
+    const std::vector<int> b_cm = {0, 1, 0, 2, 3, 0, 0, 0, 0, 0, 4, 5};  // 3 x 4, by columns
+    const std::vector<int> a_cm = {1, 0, 4, 0, 3, 0, 2, 0, 5};            // 3 x 3, by columns
+    const std::vector<int> a_rm = {1, 0, 2, 0, 3, 0, 4, 0, 5};
+    const std::vector<int> b_rm = {0, 2, 0, 0, 1, 3, 0, 4, 0, 0, 0, 5};
+    const std::vector<int> want = {0, 2, 0, 10, 3, 9, 0, 12, 0, 8, 0, 25};
+    DynamicMatrix<int> Ar(3, 3, a_rm.data()), Br(3, 4, b_rm.data()), C;
+    DynamicMatrix<int, columnMajor> Ac(3, 3, a_cm.data()), Bc(3, 4, b_cm.data());
+    C = Ar * Bc;
+    bool ok = C.toVector() == want;
+    C = Ac * Br;
+    ok = ok && C.toVector() == want;
+    C = Ac * Bc;
+    ok = ok && C.toVector() == want;
+    DynamicVector<int> x(4, 1), y;
+    y = Bc * x;  // row sums of B
+    ok = ok && y.toVector() == std::vector<int>{2, 8, 5};
+    check(ok, "ClassTest2 known answer with column-major int operands");
+  }
   // y = trans(A) * x (TDMatDVecMultExpr)
   {
     const size_t m = 300, n = 77;

@@ -204,3 +204,40 @@ def test_transposed_alpha_beta_and_argument_checks():
     import blaze_amd as bz
     with pytest.raises(bz.BlazeGpuError):   # transposed A needs lda >= m
         ctx().call("bzg_dgemm_ex", 1, 0, m, n, k, 1.0, p(dA), m - 1, p(dB), n, 0.0, p(dC), n)
+
+
+def test_int32_transposed_operands_exact():
+    # bzg_igemm_ex: all four storage-order forms, full-range int32 (wrap-around products),
+    # alpha / beta: bit-identical to the oracle's exact product of the logical operands
+    rng = np.random.default_rng(34)
+    for (m, n, k, pad) in [(1, 1, 1, 0), (7, 5, 3, 1), (67, 45, 33, 2), (130, 70, 200, 0)]:
+        A = rng.integers(-2 ** 31, 2 ** 31, (m, k), dtype=np.int64).astype(np.int32)
+        B = rng.integers(-2 ** 31, 2 ** 31, (k, n), dtype=np.int64).astype(np.int32)
+        C0 = rng.integers(-2 ** 31, 2 ** 31, (m, n), dtype=np.int64).astype(np.int32)
+        AB = oracle.gemm(A, B)
+        want = (3 * AB.astype(np.int64) - 2 * C0.astype(np.int64)).astype(np.uint64).astype(np.uint32).view(np.int32)
+        for ta in (0, 1):
+            for tb in (0, 1):
+                As = np.zeros((k, m + pad), np.int32) if ta else np.zeros((m, k + pad), np.int32)
+                Bs = np.zeros((n, k + pad), np.int32) if tb else np.zeros((k, n + pad), np.int32)
+                if ta: As[:, :m] = A.T
+                else: As[:, :k] = A
+                if tb: Bs[:, :k] = B.T
+                else: Bs[:, :n] = B
+                dA, dB, dC = dev(As), dev(Bs), dev(C0)
+                ctx().call("bzg_igemm_ex", ta, tb, m, n, k, 3, p(dA), As.shape[1], p(dB), Bs.shape[1], -2, p(dC), n)
+                assert np.array_equal(host(dC), want), (m, n, k, ta, tb)
+
+
+def test_int32_transposed_gemv_exact():
+    # bzg_igemv_t: y = A^T x over a row-major A, wrap-around exact against the oracle's gemv
+    # of the transposed copy
+    rng = np.random.default_rng(35)
+    for (m, n, pad) in [(1, 1, 0), (3, 7, 1), (500, 300, 2), (20000, 64, 0), (64, 20000, 0)]:
+        A = np.zeros((m, n + pad), np.int32)
+        A[:, :n] = rng.integers(-2 ** 31, 2 ** 31, (m, n), dtype=np.int64).astype(np.int32)
+        x = rng.integers(-2 ** 31, 2 ** 31, m, dtype=np.int64).astype(np.int32)
+        want = oracle.gemv(np.ascontiguousarray(A[:, :n].T), x)
+        dA, dx, dy = dev(A), dev(x), full((n,), np.int32, 5)  # held: the pointers must stay valid
+        ctx().call("bzg_igemv_t", m, n, 1, p(dA), n + pad, p(dx), 0, p(dy))
+        assert np.array_equal(host(dy), want), (m, n)

@@ -282,5 +282,6 @@ def test_gemv_transposed(dt):
         assert np.all(np.abs(host(dy3) - w3) <= (m + 4) * u * (0.5 * absv + 2 * np.abs(y0)) + 1e-300)
     # m = 0: y = beta y
     dy = dev(np.ones(5, dt))
-    ctx().call(fn, 0, 5, 1.0, p(dev(np.zeros(5, dt))), 5, p(dev(np.zeros(1, dt))), 3.0, p(dy))
+    dA0, dx0 = dev(np.zeros(5, dt)), dev(np.zeros(1, dt))
+    ctx().call(fn, 0, 5, 1.0, p(dA0), 5, p(dx0), 3.0, p(dy))
     assert np.array_equal(host(dy), np.full(5, 3.0, dt))
