@@ -1113,6 +1113,49 @@ inline DynamicMatrix<double, columnMajor> inv(const DynamicMatrix<double, column
   return R;
 }
 
+// lu(A, L, U, P) for a column-major A (LU.h:98-140, 222-330 with SO = columnMajor): the
+// factorisation of A itself (getrf on column-major storage), A = P * L * U with L unit lower,
+// U upper and P(permut[i], i) = 1.  The device factors a row-major copy of A, which has the same
+// pivots and factors.  Square A only.
+inline void lu(const DynamicMatrix<double, columnMajor>& A, DynamicMatrix<double, columnMajor>& L,
+               DynamicMatrix<double, columnMajor>& U, DynamicMatrix<double, columnMajor>& P) {
+  const size_t n = A.rows();
+  if (A.rows() != A.columns()) throw std::invalid_argument("lu: this build supports square matrices");
+  if (n > 512) throw std::invalid_argument("lu: this build supports n <= 512");
+  L.resize(n, n, false);
+  U.resize(n, n, false);
+  P.resize(n, n, false);
+  if (n == 0) return;
+  DynamicMatrix<double> R = rowMajorCopy(A);
+  const int32_t nn = (int32_t)n;
+  const int64_t zero = 0;
+  detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
+  detail::DeviceBuffer<int64_t> doff;
+  dn.allocate(1); dpiv.allocate(n); dinfo.allocate(1); doff.allocate(1);
+  detail::h2d(dn.ptr, &nn, 1);
+  detail::h2d(doff.ptr, &zero, 1);
+  detail::check(bzg_dgetrf_batched(context(), 1, dn.ptr, doff.ptr, doff.ptr, R.data(), dpiv.ptr, dinfo.ptr, nullptr),
+                "lu: getrf");
+  std::vector<double> f = R.toVector(), l(n * n, 0.0), u(n * n, 0.0), pm(n * n, 0.0);  // l, u, pm by columns
+  std::vector<int32_t> ipiv(n);
+  detail::d2h(ipiv.data(), dpiv.ptr, n);
+  for (size_t j = 0; j < n; ++j) {
+    l[j * n + j] = 1.0;
+    for (size_t i = j + 1; i < n; ++i) l[j * n + i] = f[i * n + j];
+    for (size_t i = 0; i <= j; ++i) u[j * n + i] = f[i * n + j];
+  }
+  std::vector<size_t> permut(n);
+  for (size_t i = 0; i < n; ++i) permut[i] = i;
+  for (size_t i = 0; i < n; ++i) {
+    const size_t q = (size_t)(ipiv[i] - 1);
+    if (q != i) std::swap(permut[q], permut[i]);
+  }
+  for (size_t i = 0; i < n; ++i) pm[i * n + permut[i]] = 1.0;  // P(permut[i], i), column i
+  L.upload(l.data());
+  U.upload(u.data());
+  P.upload(pm.data());
+}
+
 // Column-major sparse matrix: the build API runs by columns (append(i, j, v) with rows
 // strictly increasing inside column j, then finalize(j)); the storage is the row-major
 // CompressedMatrix of the transpose, and S * x is the reference's column-major product.

@@ -585,6 +585,38 @@ int main() {
     ok = ok && y.toVector() == std::vector<int>{2, 8, 5};
     check(ok, "ClassTest2 known answer with column-major int operands");
   }
+  // lu() with a column-major A (lu/DenseTest.h: P * L * U == A for column-major storage)
+  {
+    const size_t n = 41;
+    std::vector<double> a(n * n);  // by columns
+    for (auto& v : a) v = U(rng);
+    DynamicMatrix<double, columnMajor> A(n, n, a.data()), L, Um, P;
+    lu(A, L, Um, P);
+    auto l = L.toVector(), u = Um.toVector(), pm = P.toVector();
+    bool ok = L.rows() == n && Um.columns() == n && P.rows() == n;
+    for (size_t j = 0; j < n && ok; ++j)
+      for (size_t i = 0; i < n && ok; ++i) {
+        if (i < j) ok = l[j * n + i] == 0.0;                    // L unit lower
+        if (i == j) ok = ok && l[j * n + i] == 1.0;
+        if (i > j) ok = ok && u[j * n + i] == 0.0;               // U upper
+      }
+    for (size_t i = 0; i < n && ok; ++i) {                       // P a permutation
+      double rs = 0.0;
+      for (size_t j = 0; j < n; ++j) rs += pm[j * n + i];
+      ok = rs == 1.0;
+    }
+    for (size_t i = 0; i < n && ok; ++i)                         // P * L * U == A
+      for (size_t j = 0; j < n && ok; ++j) {
+        double s = 0.0;
+        for (size_t q = 0; q < n; ++q) {
+          double lu_qj = 0.0;
+          for (size_t r = 0; r < n; ++r) lu_qj += l[r * n + q] * u[j * n + r];
+          s += pm[q * n + i] * lu_qj;
+        }
+        ok = relaxed(s, a[j * n + i], 1e-12);
+      }
+    check(ok, "lu() column-major: A == P * L * U");
+  }
   // y = trans(A) * x (TDMatDVecMultExpr)
   {
     const size_t m = 300, n = 77;
