@@ -615,6 +615,33 @@ def gemm_leg(b, rec, args):
         Ag = b.graded((ms_, k), 30 + sigma, sigma); Bg = b.graded((k, ms_), 40 + sigma, sigma)
         errs[f"sigma{sigma}"] = b.sigma_case(Ag, Bg, variants)
     rec[f"rel_frob_err_by_sigma_{ms_}x{ms_}x{k}"] = errs
+    # the plain products in the other element types (same size, HIP events on the library's
+    # stream): FP32 on MFMA, INT32 on the VALU (no INT32 MFMA; wrap-around exact)
+    torch = b.torch
+    other = {}
+    for name, fn32, dt, one in (("fp32", "bzg_sgemm", torch.float32, 1.0), ("int32", "bzg_igemm", torch.int32, 1)):
+        if dt == torch.int32:
+            dA = torch.randint(-100, 100, (m, k), dtype=dt, device="cuda")
+            dB = torch.randint(-100, 100, (k, n), dtype=dt, device="cuda")
+        else:
+            dA = torch.from_numpy(A.astype(np.float32)).cuda(); dB = torch.from_numpy(B.astype(np.float32)).cuda()
+        dC = torch.empty(m, n, dtype=dt, device="cuda")
+        zero = 0 if dt == torch.int32 else 0.0
+        call = lambda: b.ctx.call(fn32, m, n, k, one, vp(dA), k, vp(dB), n, zero, vp(dC), n)
+        call(); b.sync()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 3
+        ev[0].record()
+        for _ in range(reps):
+            call()
+        ev[1].record(); b.sync()
+        ms = ev[0].elapsed_time(ev[1]) / reps
+        peak = PEAK["fp32"] if name == "fp32" else None
+        other[name] = {"ms": ms, "tflops": 2.0 * m * n * k / ms / 1e9,
+                       "frac_peak": (2.0 * m * n * k / ms / 1e9 / peak) if peak else None}
+        del dA, dB, dC
+    rec["plain_gemm_other_types"] = other
+    log(f"[gemm] other element types: {json.dumps(other)}")
     if m == 8192:
         mr, nr_, kr = 16384, 1024, 16384
         Ar = b.uniform((mr, kr), 3); Br = b.uniform((kr, nr_), 4)
