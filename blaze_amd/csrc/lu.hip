@@ -907,42 +907,53 @@ __global__ __launch_bounds__(256) void det_from_lu_kernel(int64_t count, const i
   det[i] = d;
 }
 
-// Identity columns [col0, col0 + nr) of each system, nr = min(8, n_i - col0), as the row-major
-// n_i x nr right-hand side at B + b_off[i]; nrhs[i] = nr, or 0 for a system that is done, empty,
-// out of range or singular.  On the first chunk an empty system gets info 0 (dgetri's n = 0).
-__global__ __launch_bounds__(256) void inv_rhs_kernel(int64_t count, int col0, const int32_t* __restrict__ n,
-                                                      int32_t* __restrict__ info, const int64_t* __restrict__ b_off,
-                                                      double* __restrict__ B, int32_t* __restrict__ nrhs) {
-  const int64_t i = blockIdx.x;
-  if (i >= count) return;
-  const int ni = n[i];
-  if (col0 == 0 && ni == 0 && threadIdx.x == 0) info[i] = 0;
-  const bool live = ni > col0 && ni <= NMAX && info[i] == 0;
-  const int nr = live ? min(8, ni - col0) : 0;
-  if (threadIdx.x == 0) nrhs[i] = nr;
-  double* b = B + b_off[i];
+// The batched inverse solves A_i X = I one eight-column chunk at a time, every (system, chunk)
+// pair a separate "virtual system" of one getrs launch (same factors, pivots and offsets as its
+// system, its own right-hand-side block).  inv_rhs_kernel writes chunk v's identity columns
+// [col[v], col[v] + nr) as the row-major n x nr block at B + b_off[v], nr = min(8, n - col[v]),
+// and sets nrhs[v] = nr, or 0 when its system is singular (info[sys[v]] != 0).
+__global__ __launch_bounds__(256) void inv_rhs_kernel(int64_t nv, const int32_t* __restrict__ n,
+                                                      const int32_t* __restrict__ col,
+                                                      const int32_t* __restrict__ sys,
+                                                      const int32_t* __restrict__ info,
+                                                      const int64_t* __restrict__ b_off, double* __restrict__ B,
+                                                      int32_t* __restrict__ nrhs) {
+  const int64_t v = blockIdx.x;
+  if (v >= nv) return;
+  const int ni = n[v], c0 = col[v];
+  const int nr = (info[sys[v]] == 0) ? min(8, ni - c0) : 0;
+  if (threadIdx.x == 0) nrhs[v] = nr;
+  double* b = B + b_off[v];
   for (int e = threadIdx.x; e < ni * nr; e += 256) {
     const int r = e / nr, q = e - r * nr;
-    b[e] = (r == col0 + q) ? 1.0 : 0.0;
+    b[e] = (r == c0 + q) ? 1.0 : 0.0;
   }
 }
 
-// A_i[:, col0 + q] = X_i[:, q] for the chunk's solutions (row-major n_i x nrhs[i] at B + b_off[i]).
-__global__ __launch_bounds__(256) void inv_scatter_kernel(int64_t count, int col0, const int32_t* __restrict__ n,
+// info[i] = 0 for an empty system (n[i] = 0), whose factorisation getrf reports out of range.
+__global__ __launch_bounds__(256) void empty_info_kernel(int64_t count, const int32_t* __restrict__ n,
+                                                         int32_t* __restrict__ info) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < count && n[i] == 0) info[i] = 0;
+}
+
+// A_i[:, col[v] + q] = X_v[:, q]: the chunk's solution written over the factors.
+__global__ __launch_bounds__(256) void inv_scatter_kernel(int64_t nv, const int32_t* __restrict__ n,
+                                                          const int32_t* __restrict__ col,
                                                           const int32_t* __restrict__ nrhs,
                                                           const int64_t* __restrict__ a_off,
                                                           const int64_t* __restrict__ b_off,
                                                           const double* __restrict__ B, double* __restrict__ A) {
-  const int64_t i = blockIdx.x;
-  if (i >= count) return;
-  const int nr = nrhs[i];
+  const int64_t v = blockIdx.x;
+  if (v >= nv) return;
+  const int nr = nrhs[v];
   if (nr == 0) return;
-  const int ni = n[i];
-  const double* b = B + b_off[i];
-  double* a = A + a_off[i];
+  const int ni = n[v], c0 = col[v];
+  const double* b = B + b_off[v];
+  double* a = A + a_off[v];
   for (int e = threadIdx.x; e < ni * nr; e += 256) {
     const int r = e / nr, q = e - r * nr;
-    a[(int64_t)r * ni + col0 + q] = b[e];
+    a[(int64_t)r * ni + c0 + q] = b[e];
   }
 }
 }  // namespace
@@ -962,47 +973,81 @@ static bzg_status dinv_batched_impl(bzg_context* ctx, int64_t count, const int32
   BZG_HIP_CHECK(hipMemcpyAsync(hn.data(), n, (size_t)count * 4, hipMemcpyDeviceToHost, ctx->stream));
   BZG_HIP_CHECK(hipMemcpyAsync(ha.data(), a_off, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
   BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  int maxn = 0;
-  int64_t extent = 0, bsz = 0;
-  std::vector<int64_t> hb((size_t)count, 0);
+  int64_t extent = 0;
   for (int64_t i = 0; i < count; ++i) {
-    hb[(size_t)i] = bsz;
     if (hn[(size_t)i] >= 1 && hn[(size_t)i] <= bzg::NMAX) {
       BZG_REQUIRE(ha[(size_t)i] >= 0, BZG_ERR_INVALID_ARGUMENT, "dinv_batched: negative offset");
-      maxn = std::max(maxn, hn[(size_t)i]);
       extent = std::max<int64_t>(extent, ha[(size_t)i] + (int64_t)hn[(size_t)i] * hn[(size_t)i]);
-      bsz += (int64_t)hn[(size_t)i] * 8;
     }
   }
-  // A_i := P L U in place (dgetrf)
+  // A_i := P L U in place (dgetrf); an empty system then gets info 0 (dgetri's n = 0)
   bzg_status st = bzg::lu_batched(ctx, 1, count, n, nullptr, a_off, nullptr, p_off, A, nullptr, ipiv, info, order);
   if (st) return st;
-  // scratch (slot 9): a copy of the factors, the right-hand-side chunks, b_off, nrhs, getrs info
-  const size_t lu_b = (size_t)extent * 8, rhs_b = (size_t)std::max<int64_t>(bsz, 1) * 8;
-  const size_t off_b = (size_t)count * 8, cnt_b = ((size_t)count * 4 + 7) / 8 * 8;
-  void* w = nullptr;
-  st = bzg::workspace(ctx, 9, lu_b + rhs_b + off_b + 2 * cnt_b, &w);
-  if (st) return st;
-  double* LUw = static_cast<double*>(w);
-  double* Bw = reinterpret_cast<double*>(static_cast<char*>(w) + lu_b);
-  int64_t* bo = reinterpret_cast<int64_t*>(static_cast<char*>(w) + lu_b + rhs_b);
-  int32_t* nr = reinterpret_cast<int32_t*>(static_cast<char*>(w) + lu_b + rhs_b + off_b);
-  int32_t* info2 = reinterpret_cast<int32_t*>(static_cast<char*>(w) + lu_b + rhs_b + off_b + cnt_b);
-  BZG_HIP_CHECK(hipMemcpyAsync(bo, hb.data(), off_b, hipMemcpyHostToDevice, ctx->stream));
-  if (extent > 0) BZG_HIP_CHECK(hipMemcpyAsync(LUw, A, lu_b, hipMemcpyDeviceToDevice, ctx->stream));
-  // X = A^-1 I, eight columns per dgetrs pass, written over the factors in A
-  const int chunks = std::max(1, (maxn + 7) / 8);
-  for (int c = 0; c < chunks; ++c) {
-    hipLaunchKernelGGL(bzg::inv_rhs_kernel, dim3((unsigned)count), dim3(256), 0, ctx->stream, count, 8 * c, n, info,
-                       (const int64_t*)bo, Bw, nr);
-    BZG_LAUNCH_CHECK();
-    if (maxn == 0) break;
-    st = bzg::lu_batched(ctx, 2, count, n, nr, a_off, bo, p_off, LUw, Bw, ipiv, info2, order);
-    if (st) return st;
-    hipLaunchKernelGGL(bzg::inv_scatter_kernel, dim3((unsigned)count), dim3(256), 0, ctx->stream, count, 8 * c, n,
-                       (const int32_t*)nr, a_off, (const int64_t*)bo, (const double*)Bw, A);
-    BZG_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bzg::empty_info_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, count,
+                     n, info);
+  BZG_LAUNCH_CHECK();
+  // virtual systems: one per (system, eight-column chunk) of every system in range
+  std::vector<int32_t> vn, vcol, vsys;
+  std::vector<int64_t> va, vp, vb;
+  int64_t bsz_v = 0;
+  std::vector<int64_t> hp((size_t)count);
+  BZG_HIP_CHECK(hipMemcpyAsync(hp.data(), p_off, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
+  BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < count; ++i) {
+    const int ni = hn[(size_t)i];
+    if (ni < 1 || ni > bzg::NMAX) continue;
+    for (int c0 = 0; c0 < ni; c0 += 8) {
+      vn.push_back(ni);
+      vcol.push_back(c0);
+      vsys.push_back((int32_t)i);
+      va.push_back(ha[(size_t)i]);
+      vp.push_back(hp[(size_t)i]);
+      vb.push_back(bsz_v);
+      bsz_v += (int64_t)ni * std::min(8, ni - c0);
+    }
   }
+  const int64_t nv = (int64_t)vn.size();
+  if (nv == 0) return BZG_OK;
+  BZG_REQUIRE(nv < (1LL << 31), BZG_ERR_UNSUPPORTED, "dinv_batched: too many column chunks");
+  // scratch (slot 9): a copy of the factors, the right-hand-side blocks, the virtual systems'
+  // n / col / sys / nrhs / getrs info (int32) and a_off / p_off / b_off (int64), their order
+  const size_t lu_b = (size_t)extent * 8, rhs_b = (size_t)bsz_v * 8;
+  const size_t i32_b = ((size_t)nv * 4 + 7) / 8 * 8, i64_b = (size_t)nv * 8;
+  void* w = nullptr;
+  st = bzg::workspace(ctx, 9, lu_b + rhs_b + 6 * i32_b + 3 * i64_b, &w);
+  if (st) return st;
+  char* cur = static_cast<char*>(w);
+  auto take = [&](size_t bytes) { char* q = cur; cur += bytes; return q; };
+  double* LUw = reinterpret_cast<double*>(take(lu_b));
+  double* Bw = reinterpret_cast<double*>(take(rhs_b));
+  int32_t* dn_v = reinterpret_cast<int32_t*>(take(i32_b));
+  int32_t* dcol = reinterpret_cast<int32_t*>(take(i32_b));
+  int32_t* dsys = reinterpret_cast<int32_t*>(take(i32_b));
+  int32_t* dnr = reinterpret_cast<int32_t*>(take(i32_b));
+  int32_t* dinfo2 = reinterpret_cast<int32_t*>(take(i32_b));
+  int32_t* dord = reinterpret_cast<int32_t*>(take(i32_b));
+  int64_t* da_v = reinterpret_cast<int64_t*>(take(i64_b));
+  int64_t* dp_v = reinterpret_cast<int64_t*>(take(i64_b));
+  int64_t* db_v = reinterpret_cast<int64_t*>(take(i64_b));
+  BZG_HIP_CHECK(hipMemcpyAsync(dn_v, vn.data(), (size_t)nv * 4, hipMemcpyHostToDevice, ctx->stream));
+  BZG_HIP_CHECK(hipMemcpyAsync(dcol, vcol.data(), (size_t)nv * 4, hipMemcpyHostToDevice, ctx->stream));
+  BZG_HIP_CHECK(hipMemcpyAsync(dsys, vsys.data(), (size_t)nv * 4, hipMemcpyHostToDevice, ctx->stream));
+  BZG_HIP_CHECK(hipMemcpyAsync(da_v, va.data(), i64_b, hipMemcpyHostToDevice, ctx->stream));
+  BZG_HIP_CHECK(hipMemcpyAsync(dp_v, vp.data(), i64_b, hipMemcpyHostToDevice, ctx->stream));
+  BZG_HIP_CHECK(hipMemcpyAsync(db_v, vb.data(), i64_b, hipMemcpyHostToDevice, ctx->stream));
+  st = bzg_batched_order(ctx, nv, vn.data(), dord);  // longest systems' chunks first
+  if (st) return st;
+  if (extent > 0) BZG_HIP_CHECK(hipMemcpyAsync(LUw, A, lu_b, hipMemcpyDeviceToDevice, ctx->stream));
+  // X = A^-1 I: every chunk of every system in one dgetrs launch over the copied factors, then
+  // written over the factors in A
+  hipLaunchKernelGGL(bzg::inv_rhs_kernel, dim3((unsigned)nv), dim3(256), 0, ctx->stream, nv, dn_v, dcol, dsys,
+                     (const int32_t*)info, (const int64_t*)db_v, Bw, dnr);
+  BZG_LAUNCH_CHECK();
+  st = bzg::lu_batched(ctx, 2, nv, dn_v, dnr, da_v, db_v, dp_v, LUw, Bw, ipiv, dinfo2, dord);
+  if (st) return st;
+  hipLaunchKernelGGL(bzg::inv_scatter_kernel, dim3((unsigned)nv), dim3(256), 0, ctx->stream, nv, dn_v, dcol,
+                     (const int32_t*)dnr, (const int64_t*)da_v, (const int64_t*)db_v, (const double*)Bw, A);
+  BZG_LAUNCH_CHECK();
   BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host copy of b_off is released on return
   return BZG_OK;
 }

@@ -301,11 +301,12 @@ bzg_status bzg_ddet_batched(bzg_context* ctx, int64_t count, const int32_t* n, c
                             const int32_t* order);
 /* Inverses in place (replaces invert()/inv() by LU, blaze/math/dense/Inversion.h:3337-3347:
  * getrf then getri): A_i := A_i^-1, computed as the factorisation of bzg_dgetrf_batched
- * (ipiv, info as there) followed by dgetrs solves against the identity, eight columns per pass.
+ * (ipiv, info as there) followed by dgetrs solves against the identity: every eight-column
+ * chunk of every system is one system of a single batched dgetrs launch (longest first).
  * A system with info[i] > 0 (singular; the reference throws "Inversion of singular matrix
- * failed") keeps its LU factors in A_i; n[i] = 0 gives info 0.  Reads n and a_off back to the
- * host (one synchronisation) and needs scratch of one copy of the factors plus 8 n_i doubles
- * per system (cached in the context). */
+ * failed") keeps its LU factors in A_i; n[i] = 0 gives info 0.  Reads n, a_off and p_off back
+ * to the host (synchronisations) and needs scratch of two copies of the factors' size (the
+ * factors and the right-hand sides) plus 44 bytes per chunk (cached in the context). */
 bzg_status bzg_dinv_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,
                             const int64_t* p_off, double* A, int32_t* ipiv, int32_t* info,
                             const int32_t* order);
