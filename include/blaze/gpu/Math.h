@@ -6,6 +6,8 @@
 //   blaze::gpu::CompressedMatrix<double> S(m, n); ... S.append(i, j, v); S.finalize(i);
 //                                                               y = S * x;        // CSR SpMV
 //   solve(A, x, b);  x = solve(A, b);  solve(A, X, B);                           // LU solve
+//   y = trans(A) * x;  C = trans(A) * B;  y = trans(S) * x;    lu / det / inv / invert
+//   blaze::gpu::DynamicMatrix<double, columnMajor>, CompressedMatrix<double, columnMajor>
 // Semantics follow the reference: operator* throws std::invalid_argument("Matrix sizes do not
 // match") (blaze/math/expressions/DMatDMatMultExpr.h:9646), ("Matrix and vector sizes do not
 // match") (DMatDVecMultExpr.h:6131, SMatDVecMultExpr.h:946); the container assignment
@@ -16,8 +18,9 @@
 // std::runtime_error("Solving LSE with singular system matrix failed") for N <= 6
 // (LSE.h:193-194) or ("Inversion of singular matrix failed") for N >= 7 (lapack/gesv.h:170-174).
 //
-// Storage: row-major only (Blaze's default rowMajor); elements live in HBM and are reached
-// from the host through explicit copies (upload / download / operator() reads).  All work is
+// Storage: row-major containers, and column-major ones stored as their row-major transposes
+// (which is the column-major layout); elements live in HBM and are reached from the host
+// through explicit copies (upload / download / operator() reads).  All work is
 // issued on the per-thread default context's stream and completes before an assignment
 // returns (a download, read or the end of the assignment synchronises).
 //
