@@ -85,3 +85,43 @@ def test_errors(tmp_path, text, code):
 def test_missing_file(tmp_path):
     with pytest.raises(bz.BlazeGpuError):
         bz.read_matrix_market(tmp_path / "absent.mtx")
+
+
+def test_mutated_files_never_crash(tmp_path):
+    # seeded mutations of valid files (truncation, byte garbage, token swaps, huge counts):
+    # each read either returns a well-formed CSR or an error status, never a crash
+    rng = np.random.default_rng(9)
+    seeds = [
+        "%%MatrixMarket matrix coordinate real general\n% c\n4 5 6\n1 1 1.5\n2 3 -2\n4 5 3\n3 3 7\n1 5 2\n4 1 -1\n",
+        "%%MatrixMarket matrix coordinate pattern symmetric\n5 5 4\n2 1\n3 3\n5 2\n4 1\n",
+        "%%MatrixMarket matrix coordinate integer skew-symmetric\n3 3 2\n2 1 4\n3 1 -5\n",
+    ]
+    tokens = ["0", "-1", "99999999999", "1e308", "nan", "x", "%", "\n", " ", "3", "2147483648"]
+    f = tmp_path / "m.mtx"
+    for it in range(300):
+        s = list(seeds[it % len(seeds)])
+        kind = it % 4
+        if kind == 0:      # truncate
+            s = s[:int(rng.integers(0, len(s)))]
+        elif kind == 1:    # random bytes
+            for _ in range(int(rng.integers(1, 6))):
+                s[int(rng.integers(0, len(s)))] = chr(int(rng.integers(32, 127)))
+        elif kind == 2:    # token insertion
+            pos = int(rng.integers(0, len(s)))
+            s[pos:pos] = list(" " + tokens[int(rng.integers(0, len(tokens)))] + " ")
+        else:              # change the declared entry count / sizes
+            text = "".join(s).split("\n")
+            for li, line in enumerate(text):
+                if line and not line.startswith("%"):
+                    text[li] = " ".join(tokens[int(rng.integers(0, len(tokens)))] if rng.random() < 0.5 else w
+                                        for w in line.split())
+                    break
+            s = list("\n".join(text))
+        f.write_text("".join(s))
+        try:
+            rp, ci, va, (m, n) = bz.read_matrix_market(f)
+        except bz.BlazeGpuError:
+            continue
+        assert len(rp) == m + 1 and rp[0] == 0 and np.all(np.diff(rp) >= 0) and rp[-1] == len(ci) == len(va)
+        assert np.all((ci >= 0) & (ci < max(n, 1)))
+        assert _sorted_rows(rp, ci)
