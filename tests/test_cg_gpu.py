@@ -94,3 +94,17 @@ def test_cg_graph_capture_replays_identically():
         torch.cuda.synchronize()
         assert torch.equal(x, eager), rep
     gctx.close()
+
+
+def test_cg_general_plan_path_matches_oracle():
+    # the same loop on a plan retuned to lane groups (vector-CSR SpMV kernels instead of LDS
+    # row blocks) meets the same bar against the oracle
+    import torch
+    import blaze_amd as bz
+    pb = Problem(torch, 48, 91, False)
+    bz.check("tune", bz.lib().bzg_csr_plan_tune(pb.plan.h, 4, 1, 4))
+    assert "vector-csr" in pb.plan.describe()
+    x, rr = pb.run(torch, 40)
+    xo, rro = oracle.cg(pb.rp, pb.ci, pb.va, pb.b, pb.x0, 40)
+    assert np.abs(x.cpu().numpy() - xo).max() <= 1e-9 * max(np.abs(xo).max(), 1e-300)
+    assert abs(rr - rro) <= 1e-6 * rro
