@@ -481,8 +481,15 @@ TuneKey tune_key(int type, int64_t m, int64_t n, int64_t k) {
   return TuneKey{type, ceil_log2(m), ceil_log2(n), ceil_log2(k)};
 }
 
+// Shapes: 0 = 32x32, 1 = 64x64, 2 = 128x128 tiles; 3 = 128x256 (64x128 per wavefront, FP32
+// only: its accumulators need 128 registers, 256 in FP64) -- 123.8 against 116.1 TFLOP/s at
+// 8192^3, 112.0 against 102.7 at 4096^3.
+template <typename T>
+constexpr int kShapes = sizeof(T) == 4 ? 4 : 3;
+
 template <typename T>
 int heuristic_shape(int64_t m, int64_t n) {
+  if (sizeof(T) == 4 && ((m + 127) / 128) * ((n + 255) / 256) >= 240) return 3;
   const long t128 = ((m + 127) / 128) * ((n + 127) / 128);
   const long t64 = ((m + 63) / 64) * ((n + 63) / 64);
   if (t128 >= 240) return 2;
@@ -493,6 +500,10 @@ int heuristic_shape(int64_t m, int64_t n) {
 template <typename T, bool TA = false, bool TB = false>
 bzg_status launch_shape_t(bzg_context* ctx, int shape, int64_t m, int64_t n, int64_t k, T alpha, const T* A,
                           int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc) {
+  if constexpr (sizeof(T) == 4) {
+    if (shape == 3)
+      return launch_tile<T, 128, 256, 64, 128, TA, TB>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  }
   if (shape == 2)
     return launch_tile<T, 128, 128, 64, 64, TA, TB>(ctx, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   if (shape == 1)
@@ -557,7 +568,7 @@ bzg_status gemm_autotune(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int3
   float best_ms = 0.f;
   const bool prof = ctx->profile;
   ctx->profile = false;
-  for (int shape = 0; shape < 3 && !st; ++shape) {
+  for (int shape = 0; shape < kShapes<T> && !st; ++shape) {
     st = launch_shape<T>(ctx, shape, m, n, k, T(1), A, k, B, n, T(0), C, n);  // warm-up
     if (st) break;
     const int reps = 3;

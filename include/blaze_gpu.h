@@ -148,11 +148,12 @@ bzg_status bzg_dgemm_grouped(bzg_context* ctx, int64_t count, const bzg_dgemm_pr
                              int32_t* status);
 
 /* Tile-shape autotuner (no reference counterpart; the reference's blocking is fixed at
- * compile time, blaze/math/dense/MMM.h + Blocking.h:58-61).  Times the three MFMA tile shapes
- * (32 x 32, 64 x 64, 128 x 128) on an m x n x k product (fp32 != 0: float, else double) and
- * records the fastest for the size class (ceil(log2) of m, n, k); later bzg_dgemm / bzg_sgemm
- * calls of that class use it.  All shapes give bit-identical results.  Writes the chosen
- * shape (0, 1, 2) and its time in ms (either may be NULL).  Synchronises the stream. */
+ * compile time, blaze/math/dense/MMM.h + Blocking.h:58-61).  Times the MFMA tile shapes
+ * (0: 32 x 32, 1: 64 x 64, 2: 128 x 128, and for float 3: 128 x 256) on an m x n x k product
+ * (fp32 != 0: float, else double) and records the fastest for the size class (ceil(log2) of
+ * m, n, k); later bzg_dgemm / bzg_sgemm calls of that class use it.  All shapes give
+ * bit-identical results.  Writes the chosen shape and its time in ms (either may be NULL).
+ * Synchronises the stream. */
 bzg_status bzg_gemm_autotune(bzg_context* ctx, int32_t fp32, int64_t m, int64_t n, int64_t k,
                              int32_t* shape, float* ms);
 /* Forgets every tuned size class (back to the built-in heuristic). */

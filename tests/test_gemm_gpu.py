@@ -241,3 +241,26 @@ def test_int32_transposed_gemv_exact():
         dA, dx, dy = dev(A), dev(x), full((n,), np.int32, 5)  # held: the pointers must stay valid
         ctx().call("bzg_igemv_t", m, n, 1, p(dA), n + pad, p(dx), 0, p(dy))
         assert np.array_equal(host(dy), want), (m, n)
+
+
+def test_fp32_large_tile_bit_identical_to_small_tiles():
+    # a 4096 x 4096 FP32 product runs the 128 x 256 tile; rows 0..99 of it must equal, bit for
+    # bit, a 100-row product of the same operands (small tiles), and all four storage-order
+    # forms of the large product must be identical (every tile sums in the same k order)
+    rng = np.random.default_rng(36)
+    m, n, k = 4096, 4096, 200
+    A = rng.uniform(-1, 1, (m, k)).astype(np.float32)
+    B = rng.uniform(-1, 1, (k, n)).astype(np.float32)
+    big = gpu_gemm(A, B)
+    small = gpu_gemm(A[:100].copy(), B)
+    assert np.array_equal(big[:100], small)
+    for ta in (0, 1):
+        for tb in (0, 1):
+            if not ta and not tb:
+                continue
+            dA = dev(np.ascontiguousarray(A.T) if ta else A)
+            dB = dev(np.ascontiguousarray(B.T) if tb else B)
+            dC = full((m, n), np.float32, 0.0)
+            ctx().call("bzg_sgemm_ex", ta, tb, m, n, k, 1.0, p(dA), m if ta else k, p(dB), k if tb else n, 0.0,
+                       p(dC), n)
+            assert np.array_equal(host(dC), big), (ta, tb)
