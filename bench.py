@@ -663,8 +663,8 @@ def main():
     ap.add_argument("--variant", default="auto", choices=["auto", "bf16", "int8"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
+                                   "(sharding across GPUs is not implemented)"}))
         sys.exit(2)
     legs = set(args.legs.split(","))
     b = Bench(args)

@@ -844,7 +844,7 @@ inline void solve_into(const DynamicMatrix<double>& A, double* B_dev, size_t nrh
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
   if (n == 0) return;
   if (n > 512 || nrhs > 8)
-    throw std::invalid_argument("solve: this build supports n <= 512 and up to 8 right-hand sides");
+    throw std::invalid_argument("solve: this library supports n <= 512 and up to 8 right-hand sides");
   DeviceBuffer<double> Acopy(n * n);
   d2d(Acopy.ptr, A.data(), n * n);
   const int32_t nn = (int32_t)n, rr = (int32_t)nrhs;
@@ -900,8 +900,8 @@ inline void lu(const DynamicMatrix<double>& A, DynamicMatrix<double>& L, Dynamic
                DynamicMatrix<double>& P) {
   const size_t n = A.rows();
   if (A.rows() != A.columns())
-    throw std::invalid_argument("lu: this build supports square matrices");
-  if (n > 512) throw std::invalid_argument("lu: this build supports n <= 512");
+    throw std::invalid_argument("lu: this library supports square matrices");
+  if (n > 512) throw std::invalid_argument("lu: this library supports n <= 512");
   L.resize(n, n, false);
   U.resize(n, n, false);
   P.resize(n, n, false);
@@ -947,7 +947,7 @@ inline double det(const DynamicMatrix<double>& A) {
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
   const size_t n = A.rows();
   if (n == 0) return 0.0;
-  if (n > 512) throw std::invalid_argument("det: this build supports n <= 512");
+  if (n > 512) throw std::invalid_argument("det: this library supports n <= 512");
   detail::DeviceBuffer<double> T(n * n);
   detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "det: transpose");
   const int32_t nn = (int32_t)n;
@@ -974,7 +974,7 @@ inline void invert(DynamicMatrix<double>& A) {
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
   const size_t n = A.rows();
   if (n == 0) return;
-  if (n > 512) throw std::invalid_argument("inv: this build supports n <= 512");
+  if (n > 512) throw std::invalid_argument("inv: this library supports n <= 512");
   detail::DeviceBuffer<double> T(n * n);
   detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "inv: transpose");
   const int32_t nn = (int32_t)n;
@@ -1123,8 +1123,8 @@ inline DynamicMatrix<double, columnMajor> inv(const DynamicMatrix<double, column
 inline void lu(const DynamicMatrix<double, columnMajor>& A, DynamicMatrix<double, columnMajor>& L,
                DynamicMatrix<double, columnMajor>& U, DynamicMatrix<double, columnMajor>& P) {
   const size_t n = A.rows();
-  if (A.rows() != A.columns()) throw std::invalid_argument("lu: this build supports square matrices");
-  if (n > 512) throw std::invalid_argument("lu: this build supports n <= 512");
+  if (A.rows() != A.columns()) throw std::invalid_argument("lu: this library supports square matrices");
+  if (n > 512) throw std::invalid_argument("lu: this library supports n <= 512");
   L.resize(n, n, false);
   U.resize(n, n, false);
   P.resize(n, n, false);
