@@ -233,6 +233,16 @@ def ref_gemv(A: np.ndarray, x: np.ndarray) -> np.ndarray:
     return y
 
 
+def ref_tgemv(A: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """y = trans(A) * x by the reference's own TDMatDVecMultExpr (compiled reference)."""
+    m, n = A.shape
+    fn = {np.dtype(np.float64): "ref_dtgemv", np.dtype(np.float32): "ref_stgemv",
+          np.dtype(np.int32): "ref_itgemv"}[A.dtype]
+    y = np.empty(n, A.dtype)
+    getattr(ref(), fn)(i64(m), i64(n), _p(A), _p(x), _p(y))
+    return y
+
+
 def ref_tcsrmv(rowptr, col, val, x, n_cols: int) -> np.ndarray:
     """trans(S) * x by the reference's own TSMatDVecMultExpr (compiled reference)."""
     n = len(rowptr) - 1

@@ -64,6 +64,16 @@ void csrmv(int64_t n_rows, int64_t n_cols, const int32_t* rowptr, const int32_t*
   yv = S * xv;
   for (int64_t i = 0; i < n_rows; ++i) y[i] = yv[i];
 }
+// y = trans(A) * x for a row-major A: the reference's TDMatDVecMultExpr
+template <typename T>
+void tgemv(int64_t m, int64_t n, const T* A, const T* x, T* y) {
+  blaze::DynamicMatrix<T, rowMajor> a;
+  to_blaze(a, m, n, A);
+  blaze::DynamicVector<T, columnVector> xv(m), yv;
+  for (int64_t i = 0; i < m; ++i) xv[i] = x[i];
+  yv = trans(a) * xv;
+  for (int64_t j = 0; j < n; ++j) y[j] = yv[j];
+}
 // y = trans(S) * x for a row-major S: the reference's TSMatDVecMultExpr (a column-major operand)
 template <typename T>
 void tcsrmv(int64_t n_rows, int64_t n_cols, const int32_t* rowptr, const int32_t* col, const T* val,
@@ -96,6 +106,9 @@ void ref_igemv(int64_t m, int64_t n, const int32_t* A, const int32_t* x, int32_t
 void ref_dcsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const double* v, const double* x, double* y) { csrmv(r, c, rp, ci, v, x, y); }
 void ref_scsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const float* v, const float* x, float* y) { csrmv(r, c, rp, ci, v, x, y); }
 void ref_icsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const int32_t* v, const int32_t* x, int32_t* y) { csrmv(r, c, rp, ci, v, x, y); }
+void ref_dtgemv(int64_t m, int64_t n, const double* A, const double* x, double* y) { tgemv(m, n, A, x, y); }
+void ref_stgemv(int64_t m, int64_t n, const float* A, const float* x, float* y) { tgemv(m, n, A, x, y); }
+void ref_itgemv(int64_t m, int64_t n, const int32_t* A, const int32_t* x, int32_t* y) { tgemv(m, n, A, x, y); }
 void ref_dtcsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const double* v, const double* x, double* y) { tcsrmv(r, c, rp, ci, v, x, y); }
 void ref_stcsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const float* v, const float* x, float* y) { tcsrmv(r, c, rp, ci, v, x, y); }
 

@@ -72,6 +72,15 @@ def gemm_case(seed, idx, dtype):
     return (uniform(seed, 2 * idx, m * k).reshape(m, k), uniform(seed, 2 * idx + 1, k * n).reshape(k, n))
 
 
+def gemv_case(seed, idx, m, n, dtype):
+    """Operands of GEMV case idx: A (m x n), x (n) for A * x and xt (m) for trans(A) * x;
+    int32 in [-10, 10] or uniform[-1, 1) (float: the double values rounded)."""
+    if dtype == np.int32:
+        return (ints(seed, 3 * idx, m * n).reshape(m, n), ints(seed, 3 * idx + 1, n), ints(seed, 3 * idx + 2, m))
+    A = uniform(seed, 3 * idx, m * n).reshape(m, n).astype(dtype)
+    return A, uniform(seed, 3 * idx + 1, n).astype(dtype), uniform(seed, 3 * idx + 2, m).astype(dtype)
+
+
 def solve_small_case(n, t):
     """System t (0..3) of size n of the closed-form fixture; t == 3 is exactly singular
     (a zero last row, so every term of a determinant expansion is an exact zero)."""

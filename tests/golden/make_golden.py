@@ -26,7 +26,7 @@ sys.path.insert(0, str(HERE))
 
 import oracle  # noqa: E402
 
-from inputs import (GEMM_SHAPES, gemm_case, ints, laplace3d, solve_small_case, uniform,  # noqa: E402
+from inputs import (GEMM_SHAPES, gemm_case, gemv_case, ints, laplace3d, solve_small_case, uniform,  # noqa: E402
                     uniform_mt, zipf)
 
 GEMM_SEED = 0x0B1A2E
@@ -72,6 +72,31 @@ def spmv_fixtures():
     np.savez_compressed(HERE / "spmv_zipf1e4.npz", n=np.int64(n), s=np.float64(s),
                         kmax=np.int64(kmax), seed=np.uint64(seed), x_stream=np.int64(99),
                         nnz=np.int64(rp[-1]), y=y, y_f32=y32)
+
+
+GEMV_SHAPES = [(m, n) for m in range(7) for n in range(7)] + [(67, 127), (127, 67), (64, 128), (128, 64), (300, 1000)]
+
+
+def gemv_fixtures():
+    # y = A * x and y = trans(A) * x by the reference (DMatDVecMultExpr / TDMatDVecMultExpr) over
+    # the blazetest dmatdvecmult grid (MDaVDa.cpp: {0..6}^2, 67x127, 127x67, 64x128, 128x64) plus
+    # 300 x 1000, for double, float and int32; and the blazetest GEMM grid in float
+    seed = 0x6E3F
+    out = {"seed": np.uint64(seed), "shapes": np.array(GEMV_SHAPES, np.int64)}
+    for tag, dt in (("d", np.float64), ("s", np.float32), ("i", np.int32)):
+        ys, ts = [], []
+        for idx, (m, n) in enumerate(GEMV_SHAPES):
+            A, x, xt = gemv_case(seed, idx, m, n, dt)
+            ys.append(oracle.ref_gemv(A, x) if m else np.zeros(0, dt))
+            ts.append(oracle.ref_tgemv(A, xt) if n else np.zeros(0, dt))
+        out[f"y_{tag}"] = np.concatenate(ys)
+        out[f"yt_{tag}"] = np.concatenate(ts)
+    cs = []
+    for idx, (m, n, k) in enumerate(GEMM_SHAPES):
+        A, B = gemm_case(GEMM_SEED, idx, np.float64)
+        cs.append(oracle.ref_gemm(A.astype(np.float32), B.astype(np.float32)).ravel())
+    out["gemm_c_float"] = np.concatenate(cs)
+    np.savez_compressed(HERE / "gemv_blazetest.npz", **out)
 
 
 def spmv_trans_fixtures():
@@ -125,6 +150,7 @@ def main() -> None:
     gemm_fixtures()
     spmv_fixtures()
     spmv_trans_fixtures()
+    gemv_fixtures()
     solve_fixtures()
     for f in sorted(HERE.glob("*.npz")):
         print(f"{f.name}: {f.stat().st_size} bytes")

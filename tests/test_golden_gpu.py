@@ -116,3 +116,46 @@ def test_lu_factors_vs_lapack():
         assert info == 0
         assert np.array_equal(ip - 1, g[f"piv{n}"]), n
         assert np.abs(LU - g[f"LU{n}"]).max() <= 1e-12 * n, n
+
+
+def test_gemv_transposed_and_sgemm_vs_reference():
+    # the reference's A * x (DMatDVecMultExpr) and trans(A) * x (TDMatDVecMultExpr) over the
+    # dmatdvecmult grid, and its float GEMM grid: int32 bit-exact, floating point within
+    # 2 gamma of the reference (different summation orders)
+    from inputs import gemv_case
+    from tests.gpu_util import dev, full, host, p
+    from tests.test_gemv_spmv_gpu import gpu_gemv
+    g = load("gemv_blazetest.npz")
+    seed = int(g["seed"])
+    fn_t = {np.dtype(np.float64): "bzg_dgemv_t", np.dtype(np.float32): "bzg_sgemv_t", np.dtype(np.int32): "bzg_igemv_t"}
+    for tag, dt, u in (("i", np.int32, 0.0), ("d", np.float64, 2.0 ** -53), ("s", np.float32, 2.0 ** -24)):
+        oy = ot = 0
+        for idx, (m, n) in enumerate(g["shapes"]):
+            m, n = int(m), int(n)
+            A, x, xt = gemv_case(seed, idx, m, n, dt)
+            want, want_t = g[f"y_{tag}"][oy:oy + m], g[f"yt_{tag}"][ot:ot + n]
+            oy += m; ot += n
+            got = gpu_gemv(A, x)
+            got_t = np.zeros(0, dt)
+            if n:
+                dA = dev(A) if A.size else full((1,), dt, 0)
+                dx = dev(xt) if m else full((1,), dt, 0)
+                dy = full((n,), dt, 7)
+                ctx().call(fn_t[np.dtype(dt)], m, n, 1, p(dA), n, p(dx), 0, p(dy))
+                got_t = host(dy)
+            if dt == np.int32:
+                assert np.array_equal(got, want) and np.array_equal(got_t, want_t), (m, n)
+            else:
+                Ad = np.abs(A).astype(np.float64)
+                b = 2 * (n + 1) * u * (Ad @ np.abs(x).astype(np.float64)) + 1e-300
+                bt = 2 * (m + 1) * u * (Ad.T @ np.abs(xt).astype(np.float64)) + 1e-300
+                assert np.all(np.abs(got.astype(np.float64) - want) <= b), (tag, m, n)
+                assert np.all(np.abs(got_t.astype(np.float64) - want_t) <= bt), (tag, m, n)
+    gemm_seed = int(load("gemm_blazetest.npz")["seed"])
+    off = 0
+    for idx, (m, n, k) in enumerate(GEMM_SHAPES):
+        A, B = gemm_case(gemm_seed, idx, np.float64)
+        A, B = A.astype(np.float32), B.astype(np.float32)
+        want = g["gemm_c_float"][off:off + m * n].reshape(m, n); off += m * n
+        bound = 2 * (k + 1) * 2.0 ** -24 * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)) + 1e-300
+        assert np.all(np.abs(gpu_gemm(A, B).astype(np.float64) - want) <= bound), (m, n, k)

@@ -148,6 +148,40 @@ def test_gemv_against_reference_and_exact():
         assert np.all(np.abs(oracle.gemv(Ad, xd) - oracle.ref_gemv(Ad, xd)) <= bound)
 
 
+def test_gemv_golden_fixture():
+    # the reference's A * x and trans(A) * x over the dmatdvecmult grid: int32 bit-exact,
+    # double / float within 2 gamma_n of the oracle's (differently ordered) sums; the
+    # reference's float GEMM grid likewise within 2 gamma_k
+    from inputs import gemv_case
+    g = load("gemv_blazetest.npz")
+    seed = int(g["seed"])
+    for tag, dt, u in (("i", np.int32, 0.0), ("d", np.float64, 2.0 ** -53), ("s", np.float32, 2.0 ** -24)):
+        oy = ot = 0
+        for idx, (m, n) in enumerate(g["shapes"]):
+            m, n = int(m), int(n)
+            A, x, xt = gemv_case(seed, idx, m, n, dt)
+            want, want_t = g[f"y_{tag}"][oy:oy + m], g[f"yt_{tag}"][ot:ot + n]
+            oy += m; ot += n
+            got = oracle.gemv(A, x) if m else np.zeros(0, dt)
+            got_t = oracle.gemv(np.ascontiguousarray(A.T), xt) if n else np.zeros(0, dt)
+            if dt == np.int32:
+                assert np.array_equal(got, want) and np.array_equal(got_t, want_t), (m, n)
+            else:
+                Ad = np.abs(A).astype(np.float64)
+                b = 2 * (n + 1) * u * (Ad @ np.abs(x).astype(np.float64)) + 1e-300
+                bt = 2 * (m + 1) * u * (Ad.T @ np.abs(xt).astype(np.float64)) + 1e-300
+                assert np.all(np.abs(got.astype(np.float64) - want) <= b), (tag, m, n)
+                assert np.all(np.abs(got_t.astype(np.float64) - want_t) <= bt), (tag, m, n)
+    off = 0
+    gemm_seed = int(load("gemm_blazetest.npz")["seed"])
+    for idx, (m, n, k) in enumerate(GEMM_SHAPES):
+        A, B = gemm_case(gemm_seed, idx, np.float64)
+        A, B = A.astype(np.float32), B.astype(np.float32)
+        want = g["gemm_c_float"][off:off + m * n].reshape(m, n); off += m * n
+        bound = 2 * (k + 1) * 2.0 ** -24 * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)) + 1e-300
+        assert np.all(np.abs(oracle.gemm(A, B).astype(np.float64) - want) <= bound), (m, n, k)
+
+
 # ---- SpMV ------------------------------------------------------------------------------
 def test_spmv_laplace_golden_bit_exact():
     g = load("spmv_laplace16.npz")
