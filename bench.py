@@ -616,13 +616,14 @@ def gemm_leg(b, rec, args):
         errs[f"sigma{sigma}"] = b.sigma_case(Ag, Bg, variants)
     rec[f"rel_frob_err_by_sigma_{ms_}x{ms_}x{k}"] = errs
     # the plain products in the other element types (same size, HIP events on the library's
-    # stream): FP32 on MFMA, INT32 on the VALU (no INT32 MFMA; wrap-around exact)
+    # stream): FP32 on MFMA; INT32 (wrap-around exact) as 10 INT8 digit products on the INT8
+    # MFMA, where bzg_igemm sends products of this size
     torch = b.torch
     other = {}
     for name, fn32, dt, one in (("fp32", "bzg_sgemm", torch.float32, 1.0), ("int32", "bzg_igemm", torch.int32, 1)):
         if dt == torch.int32:
-            dA = torch.randint(-100, 100, (m, k), dtype=dt, device="cuda")
-            dB = torch.randint(-100, 100, (k, n), dtype=dt, device="cuda")
+            dA = torch.randint(-2 ** 31, 2 ** 31, (m, k), dtype=torch.int64, device="cuda").to(dt)
+            dB = torch.randint(-2 ** 31, 2 ** 31, (k, n), dtype=torch.int64, device="cuda").to(dt)
         else:
             dA = torch.from_numpy(A.astype(np.float32)).cuda(); dB = torch.from_numpy(B.astype(np.float32)).cuda()
         dC = torch.empty(m, n, dtype=dt, device="cuda")
@@ -639,6 +640,8 @@ def gemm_leg(b, rec, args):
         peak = PEAK["fp32"] if name == "fp32" else None
         other[name] = {"ms": ms, "tflops": 2.0 * m * n * k / ms / 1e9,
                        "frac_peak": (2.0 * m * n * k / ms / 1e9 / peak) if peak else None}
+        if name == "int32":  # the INT8 work the 10 digit products issue, against the INT8 peak
+            other[name]["int8_mfma_frac_peak"] = 10 * 2.0 * m * n * k / ms / 1e9 / PEAK["int8"]
         del dA, dB, dC
     rec["plain_gemm_other_types"] = other
     log(f"[gemm] other element types: {json.dumps(other)}")

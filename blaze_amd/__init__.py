@@ -41,6 +41,7 @@ SIGNATURES = {
     "bzg_gemm_tuning_clear": (i32, []),
     "bzg_sgemm": (i32, [vp, i64, i64, i64, f32, vp, i64, vp, i64, f32, vp, i64]),
     "bzg_igemm_ex": (i32, [vp, i32, i32, i64, i64, i64, i32, vp, i64, vp, i64, i32, vp, i64]),
+    "bzg_igemm_digits": (i32, [vp, i32, i32, i64, i64, i64, i32, vp, i64, vp, i64, i32, vp, i64]),
     "bzg_igemv_t": (i32, [vp, i64, i64, i32, vp, i64, vp, i32, vp]),
     "bzg_igemm": (i32, [vp, i64, i64, i64, i32, vp, i64, vp, i64, i32, vp, i64]),
     "bzg_dtranspose": (i32, [vp, i64, i64, vp, i64, vp, i64]),

@@ -48,7 +48,7 @@ struct bzg_context {
   hipStream_t stream = nullptr;
   int num_cus = 256;
   // Cached scratch for kernels that need it (split GEMM slices, solver staging).
-  bzg::Workspace ws[12];
+  bzg::Workspace ws[13];
   // Optional HIP-event timing of named kernels (bzg_profile_enable / bzg_profile_read).
   // plan of the last FP64-accurate GEMM on this context (bzg_fp64acc_last_plan)
   int fp64acc_moduli = 0, fp64acc_bits = 0;
@@ -83,6 +83,13 @@ struct ProfScope {
 
 // Returns a cached device buffer of at least `bytes` (slot-indexed); grows on demand.
 bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out);
+
+// INT32 GEMM C = alpha op(A) op(B) + beta C (mod 2^32) from balanced INT8 digits on the INT8
+// MFMA (ozaki.hip); the caller has checked the arguments; needs 1 <= k <= kIgemmDigitsMaxK.
+constexpr int64_t kIgemmDigitsMaxK = 32767;
+bzg_status igemm_digits_impl(bzg_context* ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k, int32_t alpha,
+                             const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb, int32_t beta,
+                             int32_t* C, int64_t ldc);
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));

@@ -665,6 +665,18 @@ bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_
   bzg_status st = bzg::check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc, ta, tb);
   if (st != BZG_OK) return st;
   if (m == 0 || n == 0) return BZG_OK;
+  {
+    // the INT8 matrix cores where they measured faster than the vector ALUs (MI355X: from
+    // 256 x 256 x 4096 up, 2.7x at 1024^3, 4.5x at 8192^3; even at 1024 x 1024 x 256): k >= 512
+    // and at least one 256 x 256 tile of output, or k >= 256 and 2048 x 2048; also k within the
+    // exact range and a digit workspace of at most 8 GiB
+    const int64_t mp = (m + 255) / 256 * 256, np = (n + 255) / 256 * 256, kp = (k + 127) / 128 * 128;
+    const double ws_bytes = 16.0 * mp * np + 4.0 * (mp + np) * kp;
+    const double mn = (double)m * (double)n;
+    if (((k >= 512 && mn >= 65536.0) || (k >= 256 && mn >= 4194304.0)) && k <= bzg::kIgemmDigitsMaxK &&
+        ws_bytes <= 8.0 * (1L << 30))
+      return bzg::igemm_digits_impl(ctx, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  }
   const long tiles = ((m + 63) / 64) * ((n + 63) / 64);
   BZG_REQUIRE(tiles < (1L << 31), BZG_ERR_UNSUPPORTED, "igemm: too many tiles");
   const dim3 grid((unsigned)tiles), block(256);
@@ -678,6 +690,17 @@ bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_
 #undef BZG_IGEMM
   BZG_LAUNCH_CHECK();
   return BZG_OK;
+}
+
+bzg_status bzg_igemm_digits(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                            int32_t alpha, const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
+                            int32_t beta, int32_t* C, int64_t ldc) {
+  const bool ta = transa != 0, tb = transb != 0;
+  bzg_status st = bzg::check_gemm_args(ctx, m, n, k, A, lda, B, ldb, C, ldc, ta, tb);
+  if (st != BZG_OK) return st;
+  if (m == 0 || n == 0) return BZG_OK;
+  if (k == 0) return bzg_igemm_ex(ctx, transa, transb, m, n, 0, alpha, A, lda, B, ldb, beta, C, ldc);
+  return bzg::igemm_digits_impl(ctx, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t alpha,

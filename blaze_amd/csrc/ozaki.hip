@@ -526,8 +526,14 @@ __device__ __forceinline__ int swz(int r, int c) { return r * RBYTES + ((c ^ ((r
 // while the 4 x FN MFMAs of k-step ks run; the staged registers of st+1 are written into the
 // other LDS buffer during the last k-step (that buffer was last read before the previous
 // barrier); one barrier closes the stage.
+//
+// MODE 2 (INT32 GEMM from balanced base-256 digits): Ares / Bres are single arrays whose rows
+// hold the 4 digit planes side by side along K (A: d0 d1 d2 d3, B^T: d3 d2 d1 d0, kp each), and
+// z = 3 - c runs digit class c (the pairs i + j = c) as one product over (c + 1) kp: A's segments
+// 0..c against B's segments 3-c..3.  The class sums are written as raw INT32 planes.
 template <int MODE>
-using ResOut = typename std::conditional<MODE == 0, int16_t, int8_t>::type;
+using ResOut = typename std::conditional<MODE == 0, int16_t,
+                                         typename std::conditional<MODE == 1, int8_t, int32_t>::type>::type;
 
 template <int MODE, int FN>
 __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, int kp,
@@ -546,12 +552,15 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
   char* As = smem;                   // [2][RT rows][RBYTES]
   char* Bs = smem + 2 * RSTAGE;      // [2][BN rows][RBYTES]
 
-  const int l = blockIdx.z;
+  const int l = (MODE == 2) ? 3 - (int)blockIdx.z : (int)blockIdx.z;  // MODE 2: the longest class first
   const long a_plane = plane_elems((long)mp * kp * ESZ, 1), b_plane = plane_elems((long)np * kp * ESZ, 1);
-  const char* Ag = Ares + l * a_plane;
-  const char* Bg = Bres + l * b_plane;
-  const float mf = (float)cc->m[l];
-  const float minvf = 1.0f / mf;
+  const char* Ag = (MODE == 2) ? Ares : Ares + l * a_plane;
+  const char* Bg = (MODE == 2) ? Bres + (long)(3 - l) * kp : Bres + l * b_plane;
+  float mf = 0.0f, minvf = 0.0f;
+  if constexpr (MODE == 0) {
+    mf = (float)cc->m[l];
+    minvf = 1.0f / mf;
+  }
 
   const int tiles_n = np / BN;
   const int tiles_m = mp / RT;
@@ -570,6 +579,7 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave >> 1) * 128, wn0 = (wave & 1) * (32 * FN);
   const long kbytes = (long)kp * ESZ;
+  const long rowb = (MODE == 2) ? 4 * kbytes : kbytes;  // bytes per operand row
 
   using acc_t = typename std::conditional<MODE == 0, f16v, v16i>::type;
   acc_t acc[4][FN];
@@ -585,12 +595,12 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int q = tid + 256 * e, r = q >> 3, c = q & 7;
-      ra[e] = *reinterpret_cast<const v4i*>(Ag + (row0 + r) * kbytes + kb + c * 16);
+      ra[e] = *reinterpret_cast<const v4i*>(Ag + (row0 + r) * rowb + kb + c * 16);
     }
 #pragma unroll
     for (int e = 0; e < NBCH; ++e) {
       const int q = tid + 256 * e, r = q >> 3, c = q & 7;
-      rb[e] = *reinterpret_cast<const v4i*>(Bg + (col0 + r) * kbytes + kb + c * 16);
+      rb[e] = *reinterpret_cast<const v4i*>(Bg + (col0 + r) * rowb + kb + c * 16);
     }
   };
   auto sstore = [&](int buf) {
@@ -645,7 +655,7 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
     }
   };
 
-  const int nst = (int)(kbytes / RBYTES);
+  const int nst = (int)((MODE == 2 ? (l + 1) * kbytes : kbytes) / RBYTES);
   // K elements per stage: 64 (bf16) / 128 (i8); bf16 reduces every 4 stages (256 K)
   gload(0);
   sstore(0);
@@ -665,7 +675,7 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
     // the next stage, and the first 4 + FN MFMAs of k-steps 0..NKS-2 by one LDS fragment read
     // of the following k-step; the LDS stores of the next stage ride on the last k-step.
     constexpr int NM = 4 * FN;
-    if constexpr (MODE == 1)  // (the BF16 tile measured faster with the compiler's own order)
+    if constexpr (MODE != 0)  // (the BF16 tile measured faster with the compiler's own order)
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
@@ -686,6 +696,21 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
     kstep(cur);
   }
 
+  if constexpr (MODE == 2) {
+    // class sums as they are (|sum| <= 4 k 2^14 < 2^31 for k < 2^15)
+    int32_t* Cg = Cres + l * plane_elems((long)mp * np, sizeof(int32_t));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const long gr = row0 + wm0 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const long gc = col0 + wn0 + j * 32 + (lane & 31);
+          Cg[gr * np + gc] = acc[i][j][r];
+        }
+    return;
+  } else {
   // epilogue: symmetric residue mod m
   const long cplane = plane_elems((long)mp * np, sizeof(ResOut<MODE>));
   ResOut<MODE>* Cg = Cres + l * cplane;
@@ -714,6 +739,7 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
         const long gc = col0 + wn0 + j * 32 + (lane & 31);
         Cg[gr * np + gc] = (ResOut<MODE>)v;
       }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -899,6 +925,137 @@ bzg_status plan_fp64acc(int64_t k, int32_t moduli, int mode, Fp64AccPlan& pl, co
   return BZG_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// device: INT32 GEMM from INT8 digits (wrap-around, exact modulo 2^32)
+// ------------------------------------------------------------------------------------------
+// Balanced base-256 digits: x = d0 + 2^8 d1 + 2^16 d2 + 2^24 d3 (mod 2^32), d_i in [-128, 127].
+// Each step subtracts the digit, which leaves a multiple of 256 and makes the shift exact; the
+// last digit only counts modulo 2^8.  The product then is
+//   A B = sum_{i+j <= 3} 2^{8(i+j)} A_i B_j (mod 2^32)
+// (the pairs with i + j >= 4 vanish modulo 2^32): 10 INT8 products in 4 classes c = i + j.
+__device__ __forceinline__ void digits4(int x, uint32_t& b0, uint32_t& b1, uint32_t& b2, uint32_t& b3) {
+  long v = x;
+  const long d0 = (long)(int8_t)(uint8_t)(v & 0xff);
+  v = (v - d0) >> 8;
+  const long d1 = (long)(int8_t)(uint8_t)(v & 0xff);
+  v = (v - d1) >> 8;
+  const long d2 = (long)(int8_t)(uint8_t)(v & 0xff);
+  v = (v - d2) >> 8;
+  b0 = (uint32_t)(uint8_t)d0;
+  b1 = (uint32_t)(uint8_t)d1;
+  b2 = (uint32_t)(uint8_t)d2;
+  b3 = (uint32_t)(v & 0xff);
+}
+
+// 16 values -> 4 digit segments of 16 bytes at o + seg(q) kp (seg(q) = q, or 3 - q with REV).
+template <bool REV>
+__device__ __forceinline__ void store_digits16(const int* v, char* __restrict__ o, long kp) {
+  uint32_t w[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w[q][t] = 0u;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    uint32_t b0, b1, b2, b3;
+    digits4(v[u], b0, b1, b2, b3);
+    const int sh = 8 * (u & 3), t = u >> 2;
+    w[0][t] |= b0 << sh;
+    w[1][t] |= b1 << sh;
+    w[2][t] |= b2 << sh;
+    w[3][t] |= b3 << sh;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    *reinterpret_cast<uint4*>(o + (long)(REV ? 3 - q : q) * kp) = make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
+}
+
+// Digit rows from a source whose rows run along K: row i of out (rp rows of 4 kp bytes) from
+// src[i * ld + kk] (rows x k; zeros in the padding).  16 consecutive kk per thread; grid x covers
+// kp in 4096-element chunks, y strides over the rows.
+template <bool REV>
+__global__ __launch_bounds__(256) void digits_rows_kernel(int rows, int k, const int* __restrict__ src, long ld,
+                                                          int rp, int kp, char* __restrict__ out) {
+  const int k0 = (blockIdx.x * 256 + threadIdx.x) * 16;
+  if (k0 >= kp) return;
+  for (int i = blockIdx.y; i < rp; i += gridDim.y) {
+    int v[16];
+    const int* s = src + (long)i * ld + k0;
+    if (i < rows && k0 + 16 <= k && ((uintptr_t)s & 15) == 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int4 t4 = reinterpret_cast<const int4*>(s)[u];
+        v[4 * u] = t4.x;
+        v[4 * u + 1] = t4.y;
+        v[4 * u + 2] = t4.z;
+        v[4 * u + 3] = t4.w;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = (i < rows && k0 + u < k) ? s[u] : 0;
+    }
+    store_digits16<REV>(v, out + (long)i * 4 * kp + k0, kp);
+  }
+}
+
+// Digit rows from a source whose columns run along K: row j of out from src[kk * ld + j]
+// (k x rows).  A workgroup transposes a 128 (kk) x 64 (j) tile through LDS (rows padded to 65
+// words); grid x = rp / 64, y = kp / 128.
+template <bool REV>
+__global__ __launch_bounds__(256) void digits_cols_kernel(int rows, int k, const int* __restrict__ src, long ld,
+                                                          int rp, int kp, char* __restrict__ out) {
+  __shared__ int tile[128][65];
+  const int kb = blockIdx.y * 128, jb = blockIdx.x * 64;
+  const int tid = threadIdx.x, c = tid & 63;
+#pragma unroll 8
+  for (int p = 0; p < 32; ++p) {
+    const int r = (tid >> 6) + 4 * p, kk = kb + r, j = jb + c;
+    tile[r][c] = (kk < k && j < rows) ? src[(long)kk * ld + j] : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int t = tid + 256 * e, jr = t >> 3, g = t & 7;
+    const int j = jb + jr, kk0 = kb + 16 * g;
+    if (j >= rp || kk0 >= kp) continue;
+    int v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = tile[16 * g + u][jr];
+    store_digits16<REV>(v, out + (long)j * 4 * kp + kk0, kp);
+  }
+}
+
+// C = alpha (P0 + 2^8 P1 + 2^16 P2 + 2^24 P3) + beta C (mod 2^32) from the class planes P_c
+// (mp x np, cplane apart), Horner from the top class.  beta == 0 never reads C.  4 consecutive
+// columns per thread; grid x covers n in 1024-column chunks, y strides over rows.
+__global__ __launch_bounds__(256) void digits_combine_kernel(int m, int n, int np, long cplane, uint32_t alpha,
+                                                             const int* __restrict__ P, uint32_t beta,
+                                                             int* __restrict__ C, long ldc) {
+  const int j0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (j0 >= n) return;
+  for (int i = blockIdx.y; i < m; i += gridDim.y) {
+    const long o = (long)i * np + j0;
+    const int4 p0 = *reinterpret_cast<const int4*>(P + o);
+    const int4 p1 = *reinterpret_cast<const int4*>(P + cplane + o);
+    const int4 p2 = *reinterpret_cast<const int4*>(P + 2 * cplane + o);
+    const int4 p3 = *reinterpret_cast<const int4*>(P + 3 * cplane + o);
+    const int a0[4] = {p0.x, p0.y, p0.z, p0.w}, a1[4] = {p1.x, p1.y, p1.z, p1.w};
+    const int a2[4] = {p2.x, p2.y, p2.z, p2.w}, a3[4] = {p3.x, p3.y, p3.z, p3.w};
+    int* crow = C + (long)i * ldc + j0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (j0 + u >= n) break;
+      uint32_t s = (uint32_t)a3[u];
+      s = (s << 8) + (uint32_t)a2[u];
+      s = (s << 8) + (uint32_t)a1[u];
+      s = (s << 8) + (uint32_t)a0[u];
+      uint32_t r = alpha * s;
+      if (beta != 0u) r += beta * (uint32_t)crow[u];
+      crow[u] = (int)r;
+    }
+  }
+}
+
 }  // namespace
 
 bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k, const double* A,
@@ -1064,6 +1221,61 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   }
 #undef BZG_CRT_ALL
 #undef BZG_CRT
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+
+bzg_status igemm_digits_impl(bzg_context* ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k, int32_t alpha,
+                             const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb, int32_t beta,
+                             int32_t* C, int64_t ldc) {
+  // arguments checked by the caller; m, n >= 1, 1 <= k <= kIgemmDigitsMaxK
+  BZG_REQUIRE(k >= 1 && k <= kIgemmDigitsMaxK, BZG_ERR_UNSUPPORTED,
+              "igemm (INT8 digits): k must be 1..32767 (class sums exact in INT32)");
+  const long mp = (m + RT - 1) / RT * RT, np = (n + RT - 1) / RT * RT;
+  const long kp = (k + RBYTES - 1) / RBYTES * RBYTES;
+  const size_t a_bytes = ((size_t)mp * 4 * kp + 255) / 256 * 256;
+  const size_t b_bytes = ((size_t)np * 4 * kp + 255) / 256 * 256;
+  const long cplane = plane_elems(mp * np, sizeof(int32_t));
+  const size_t c_bytes = (size_t)4 * cplane * sizeof(int32_t);
+  void* ws = nullptr;
+  bzg_status st = workspace(ctx, 12, a_bytes + b_bytes + c_bytes, &ws);
+  if (st) return st;
+  char* Ad = static_cast<char*>(ws);
+  char* Bd = Ad + a_bytes;
+  int32_t* P = reinterpret_cast<int32_t*>(Bd + b_bytes);
+  // digit rows of op(A) (m x k) and of op(B)^T (n x k)
+  const dim3 grows_a((unsigned)((kp + 4095) / 4096), (unsigned)std::min<long>(mp, 16384));
+  const dim3 grows_b((unsigned)((kp + 4095) / 4096), (unsigned)std::min<long>(np, 16384));
+  if (!ta)
+    hipLaunchKernelGGL(digits_rows_kernel<false>, grows_a, dim3(256), 0, ctx->stream, (int)m, (int)k, A, (long)lda,
+                       (int)mp, (int)kp, Ad);
+  else
+    hipLaunchKernelGGL(digits_cols_kernel<false>, dim3((unsigned)(mp / 64), (unsigned)(kp / 128)), dim3(256), 0,
+                       ctx->stream, (int)m, (int)k, A, (long)lda, (int)mp, (int)kp, Ad);
+  BZG_LAUNCH_CHECK();
+  if (!tb)
+    hipLaunchKernelGGL(digits_cols_kernel<true>, dim3((unsigned)(np / 64), (unsigned)(kp / 128)), dim3(256), 0,
+                       ctx->stream, (int)n, (int)k, B, (long)ldb, (int)np, (int)kp, Bd);
+  else
+    hipLaunchKernelGGL(digits_rows_kernel<true>, grows_b, dim3(256), 0, ctx->stream, (int)n, (int)k, B, (long)ldb,
+                       (int)np, (int)kp, Bd);
+  BZG_LAUNCH_CHECK();
+  const size_t lds = 4 * RSTAGE;
+  static bool attr_set = false;
+  if (!attr_set) {
+    BZG_HIP_CHECK(hipFuncSetAttribute((const void*)residue_gemm_kernel<2, 4>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  {
+    ProfScope prof_gemm(ctx, "igemm_digits_gemm");
+    hipLaunchKernelGGL((residue_gemm_kernel<2, 4>), dim3((unsigned)((mp / RT) * (np / RT)), 1, 4), dim3(256), lds,
+                       ctx->stream, (int)mp, (int)np, (int)kp, Ad, Bd, P, nullptr);
+    BZG_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(digits_combine_kernel, dim3((unsigned)((n + 1023) / 1024), (unsigned)std::min<long>(m, 16384)),
+                     dim3(256), 0, ctx->stream, (int)m, (int)n, (int)np, cplane, (uint32_t)alpha, P,
+                     (uint32_t)beta, C, (long)ldc);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }

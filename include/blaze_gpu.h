@@ -108,10 +108,23 @@ bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t 
                      const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
                      int32_t beta, int32_t* C, int64_t ldc);
 /* bzg_igemm over transposed operands (transa / transb as bzg_dgemm_ex): INT32 wrap-around
- * products are exact, so every form equals the product of transposed copies bit for bit. */
+ * products are exact, so every form equals the product of transposed copies bit for bit.
+ * bzg_igemm and bzg_igemm_ex run products with k <= 32767 and either k >= 512 and m * n >=
+ * 256 * 256, or k >= 256 and m * n >= 2048 * 2048 (digit workspace <= 8 GiB), as
+ * bzg_igemm_digits; the rest on the vector ALUs. */
 bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
                         int32_t alpha, const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
                         int32_t beta, int32_t* C, int64_t ldc);
+/* The same product (same arguments and result bits as bzg_igemm_ex) on the INT8 matrix cores:
+ * each operand is split into 4 balanced base-256 digits, the 10 digit-pair products with
+ * i + j <= 3 run as 4 INT8 GEMMs with exact INT32 sums, and C = alpha * sum_c 2^(8c) P_c +
+ * beta * C is formed modulo 2^32.  Needs k <= 32767 (BZG_ERR_UNSUPPORTED above: the class
+ * sums must stay below 2^31); k == 0 sets C = beta * C.  Uses 16 bytes of workspace per
+ * element of the 256-padded C plus 4 bytes per element of each padded operand.  Replaces
+ * the same Blaze integer products as bzg_igemm_ex. */
+bzg_status bzg_igemm_digits(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                            int32_t alpha, const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
+                            int32_t beta, int32_t* C, int64_t ldc);
 /* C = alpha * op(A) * op(B) + beta * C with op(A) = A or A^T (transa != 0: A is stored k x m,
  * lda >= m) and op(B) = B or B^T (transb != 0: B is stored n x k, ldb >= k).  Replaces the
  * products over the other storage orders (DMatTDMatMultExpr.h, TDMatDMatMultExpr.h,
