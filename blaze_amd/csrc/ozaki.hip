@@ -567,8 +567,9 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
   const int nt = tiles_m * tiles_n;
   int bid = blockIdx.x;
   if (nt >= 64 && nt % 8 == 0) bid = (bid % 8) * (nt / 8) + bid / 8;  // XCD-contiguous
-  // walk tiles in column-major groups of 4 tile rows for L2 reuse
-  const int G = 4;
+  // walk tiles in column-major groups of 8 tile rows for L2 reuse (8192^3, 14 INT8 residue
+  // products: 7.94 ms per call against 8.04 with groups of 4, 8.36 with 2, 8.10 with 16)
+  const int G = 8;
   const int group = bid / (G * tiles_n);
   const int first_m = group * G;
   const int gsize = min(tiles_m - first_m, G);
