@@ -109,8 +109,8 @@ bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t 
                      int32_t beta, int32_t* C, int64_t ldc);
 /* bzg_igemm over transposed operands (transa / transb as bzg_dgemm_ex): INT32 wrap-around
  * products are exact, so every form equals the product of transposed copies bit for bit.
- * bzg_igemm and bzg_igemm_ex run products with k <= 32767 and either k >= 512 and m * n >=
- * 256 * 256, or k >= 256 and m * n >= 2048 * 2048 (digit workspace <= 8 GiB), as
+ * bzg_igemm and bzg_igemm_ex run products with k <= 32767, m, n >= 128 and either k >= 512
+ * and m * n >= 256 * 256, or k >= 256 and m * n >= 2048 * 2048 (digit workspace <= 8 GiB), as
  * bzg_igemm_digits; the rest on the vector ALUs. */
 bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
                         int32_t alpha, const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
