@@ -120,3 +120,26 @@ def test_routed_products_exact():
     dC2 = full((m, n), np.int32, -7)
     ctx().call("bzg_igemm_digits", 0, 0, m, n, k, 1, p(dA), k, p(dB), n, 0, p(dC2), n)
     assert np.array_equal(host(dC2), want)
+
+
+def test_digits_random_shapes_exact():
+    # seeded sweep over ragged shapes (k crossing the 128-byte stage and 256-row tile edges),
+    # storage orders, leading-dimension padding and alpha / beta, bit-exact against the oracle
+    rng = np.random.default_rng(74)
+    for case in range(24):
+        m, n = (int(v) for v in rng.integers(1, 700, 2))
+        k = int(rng.integers(1, 3000))
+        ta, tb, pad = int(rng.integers(0, 2)), int(rng.integers(0, 2)), int(rng.integers(0, 5))
+        alpha, beta = (int(v) for v in rng.integers(-2 ** 31, 2 ** 31, 2, dtype=np.int64))
+        if case % 4 == 0:
+            beta = 0
+        A = rng.integers(-2 ** 31, 2 ** 31, (m, k), dtype=np.int64).astype(np.int32)
+        B = rng.integers(-2 ** 31, 2 ** 31, (k, n), dtype=np.int64).astype(np.int32)
+        C0 = rng.integers(-2 ** 31, 2 ** 31, (m, n), dtype=np.int64).astype(np.int32)
+        AB = oracle.gemm(A, B).astype(np.int64)
+        want = _wrap((alpha * AB) % 2 ** 32 + (beta * C0.astype(np.int64)) % 2 ** 32)
+        As, Bs = _stored(A, B, ta, tb, pad)
+        dA, dB, dC = dev(As), dev(Bs), dev(C0)
+        ctx().call("bzg_igemm_digits", ta, tb, m, n, k, alpha, p(dA), As.shape[1], p(dB), Bs.shape[1], beta,
+                   p(dC), n)
+        assert np.array_equal(host(dC), want), (case, m, n, k, ta, tb, pad, alpha, beta)
