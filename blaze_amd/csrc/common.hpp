@@ -85,8 +85,10 @@ struct ProfScope {
 bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out);
 
 // INT32 GEMM C = alpha op(A) op(B) + beta C (mod 2^32) from balanced INT8 digits on the INT8
-// MFMA (ozaki.hip); the caller has checked the arguments; needs 1 <= k <= kIgemmDigitsMaxK.
+// MFMA (ozaki.hip); the caller has checked the arguments and m, n, k >= 1.  One pass covers
+// k <= kIgemmDigitsMaxK; longer K runs in chunks of at most kIgemmDigitsChunk.
 constexpr int64_t kIgemmDigitsMaxK = 32767;
+constexpr int64_t kIgemmDigitsChunk = 32640;  // 255 x 128
 bzg_status igemm_digits_impl(bzg_context* ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k, int32_t alpha,
                              const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb, int32_t beta,
                              int32_t* C, int64_t ldc);

@@ -670,13 +670,14 @@ bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_
     // the INT8 matrix cores where they measured faster than the vector ALUs (MI355X: from
     // 256 x 256 x 4096 up, 2.7x at 1024^3, 4.5x at 8192^3; even at 1024 x 1024 x 256): k >= 512
     // and at least one 256 x 256 tile of output, or k >= 256 and 2048 x 2048; m and n >= 128
-    // (the digit GEMM pads both to 256: skinny products stay on the VALU); k within the exact
-    // range and a digit workspace of at most 8 GiB
-    const int64_t mp = (m + 255) / 256 * 256, np = (n + 255) / 256 * 256, kp = (k + 127) / 128 * 128;
+    // (the digit GEMM pads both to 256: skinny products stay on the VALU); a digit workspace
+    // of at most 8 GiB (one K chunk of the digits and the 4 class planes)
+    const int64_t mp = (m + 255) / 256 * 256, np = (n + 255) / 256 * 256;
+    const int64_t kp = std::min<int64_t>((k + 127) / 128 * 128, bzg::kIgemmDigitsChunk);
     const double ws_bytes = 16.0 * mp * np + 4.0 * (mp + np) * kp;
     const double mn = (double)m * (double)n;
     if (((k >= 512 && mn >= 65536.0) || (k >= 256 && mn >= 4194304.0)) && std::min(m, n) >= 128 &&
-        k <= bzg::kIgemmDigitsMaxK && ws_bytes <= 8.0 * (1L << 30))
+        ws_bytes <= 8.0 * (1L << 30))
       return bzg::igemm_digits_impl(ctx, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   }
   const long tiles = ((m + 63) / 64) * ((n + 63) / 64);

@@ -1226,12 +1226,12 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
   return BZG_OK;
 }
 
-bzg_status igemm_digits_impl(bzg_context* ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k, int32_t alpha,
-                             const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb, int32_t beta,
-                             int32_t* C, int64_t ldc) {
-  // arguments checked by the caller; m, n >= 1, 1 <= k <= kIgemmDigitsMaxK
-  BZG_REQUIRE(k >= 1 && k <= kIgemmDigitsMaxK, BZG_ERR_UNSUPPORTED,
-              "igemm (INT8 digits): k must be 1..32767 (class sums exact in INT32)");
+namespace {
+// One launch sequence: 1 <= k <= kIgemmDigitsMaxK (the class sums stay below 2^31).
+bzg_status igemm_digits_chunk(bzg_context* ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k, int32_t alpha,
+                              const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb, int32_t beta,
+                              int32_t* C, int64_t ldc) {
+  BZG_REQUIRE(k >= 1 && k <= kIgemmDigitsMaxK, BZG_ERR_UNSUPPORTED, "igemm (INT8 digits): chunk k out of range");
   const long mp = (m + RT - 1) / RT * RT, np = (n + RT - 1) / RT * RT;
   const long kp = (k + RBYTES - 1) / RBYTES * RBYTES;
   const size_t a_bytes = ((size_t)mp * 4 * kp + 255) / 256 * 256;
@@ -1278,6 +1278,26 @@ bzg_status igemm_digits_impl(bzg_context* ctx, bool ta, bool tb, int64_t m, int6
                      dim3(256), 0, ctx->stream, (int)m, (int)n, (int)np, cplane, (uint32_t)alpha, P,
                      (uint32_t)beta, C, (long)ldc);
   BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+}  // namespace
+
+bzg_status igemm_digits_impl(bzg_context* ctx, bool ta, bool tb, int64_t m, int64_t n, int64_t k, int32_t alpha,
+                             const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb, int32_t beta,
+                             int32_t* C, int64_t ldc) {
+  // arguments checked by the caller; m, n, k >= 1.  Longer K runs in chunks of at most
+  // kIgemmDigitsChunk (a multiple of 128, so only the last chunk is padded), the later ones
+  // accumulated with beta = 1: wrap-around sums are exact in any grouping.
+  const int64_t nch = (k + kIgemmDigitsChunk - 1) / kIgemmDigitsChunk;
+  const int64_t kc = ((k + nch - 1) / nch + 127) / 128 * 128;
+  for (int64_t k0 = 0; k0 < k; k0 += kc) {
+    const int64_t kk = std::min(kc, k - k0);
+    const int32_t* Ak = ta ? A + k0 * lda : A + k0;
+    const int32_t* Bk = tb ? B + k0 : B + k0 * ldb;
+    const bzg_status st = igemm_digits_chunk(ctx, ta, tb, m, n, kk, alpha, Ak, lda, Bk, ldb,
+                                             k0 == 0 ? beta : 1, C, ldc);
+    if (st) return st;
+  }
   return BZG_OK;
 }
 

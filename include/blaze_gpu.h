@@ -109,7 +109,7 @@ bzg_status bzg_igemm(bzg_context* ctx, int64_t m, int64_t n, int64_t k, int32_t 
                      int32_t beta, int32_t* C, int64_t ldc);
 /* bzg_igemm over transposed operands (transa / transb as bzg_dgemm_ex): INT32 wrap-around
  * products are exact, so every form equals the product of transposed copies bit for bit.
- * bzg_igemm and bzg_igemm_ex run products with k <= 32767, m, n >= 128 and either k >= 512
+ * bzg_igemm and bzg_igemm_ex run products with m, n >= 128 and either k >= 512
  * and m * n >= 256 * 256, or k >= 256 and m * n >= 2048 * 2048 (digit workspace <= 8 GiB), as
  * bzg_igemm_digits; the rest on the vector ALUs. */
 bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
@@ -118,10 +118,11 @@ bzg_status bzg_igemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_
 /* The same product (same arguments and result bits as bzg_igemm_ex) on the INT8 matrix cores:
  * each operand is split into 4 balanced base-256 digits, the 10 digit-pair products with
  * i + j <= 3 run as 4 INT8 GEMMs with exact INT32 sums, and C = alpha * sum_c 2^(8c) P_c +
- * beta * C is formed modulo 2^32.  Needs k <= 32767 (BZG_ERR_UNSUPPORTED above: the class
- * sums must stay below 2^31); k == 0 sets C = beta * C.  Uses 16 bytes of workspace per
- * element of the 256-padded C plus 4 bytes per element of each padded operand.  Replaces
- * the same Blaze integer products as bzg_igemm_ex. */
+ * beta * C is formed modulo 2^32.  One pass covers k <= 32767 (the class sums stay below
+ * 2^31); longer K runs in chunks of at most 32640, accumulated with beta = 1 (exact: wrap-
+ * around sums do not depend on the grouping).  k == 0 sets C = beta * C.  Uses 16 bytes of
+ * workspace per element of the 256-padded C plus 4 bytes per element of each padded operand
+ * chunk.  Replaces the same Blaze integer products as bzg_igemm_ex. */
 bzg_status bzg_igemm_digits(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
                             int32_t alpha, const int32_t* A, int64_t lda, const int32_t* B, int64_t ldb,
                             int32_t beta, int32_t* C, int64_t ldc);

@@ -77,19 +77,31 @@ def test_digits_exact_at_largest_k():
     assert np.array_equal(host(dC2), want)
 
 
-def test_digits_rejects_k_beyond_exact_range_and_k_zero_scales():
-    m, n, k = 8, 8, 32768
-    A = np.ones((m, k), np.int32)
-    B = np.ones((k, n), np.int32)
-    dA, dB, dC = dev(A), dev(B), full((m, n), np.int32, 5)
-    with pytest.raises(bz.BlazeGpuError):
-        ctx().call("bzg_igemm_digits", 0, 0, m, n, k, 1, p(dA), k, p(dB), n, 0, p(dC), n)
-    # bzg_igemm keeps such k on the vector ALUs
-    ctx().call("bzg_igemm", m, n, k, 1, p(dA), k, p(dB), n, 0, p(dC), n)
-    assert np.array_equal(host(dC), np.full((m, n), k, np.int32))
+def test_digits_long_k_in_chunks_and_k_zero_scales():
+    # k beyond one pass (32767) runs in K chunks accumulated with beta = 1: 70001 = 3 chunks,
+    # full-range values, alpha / beta, both storage orders of each operand
+    rng = np.random.default_rng(75)
+    m, n, k = 40, 72, 70001
+    A = rng.integers(-2 ** 31, 2 ** 31, (m, k), dtype=np.int64).astype(np.int32)
+    B = rng.integers(-2 ** 31, 2 ** 31, (k, n), dtype=np.int64).astype(np.int32)
+    C0 = rng.integers(-2 ** 31, 2 ** 31, (m, n), dtype=np.int64).astype(np.int32)
+    want = _wrap((-3 * oracle.gemm(A, B).astype(np.int64)) % 2 ** 32 + (5 * C0.astype(np.int64)) % 2 ** 32)
+    for ta, tb in ((0, 0), (1, 1)):
+        As, Bs = _stored(A, B, ta, tb, 1)
+        dA, dB, dC = dev(As), dev(Bs), dev(C0)
+        ctx().call("bzg_igemm_digits", ta, tb, m, n, k, -3, p(dA), As.shape[1], p(dB), Bs.shape[1], 5, p(dC), n)
+        assert np.array_equal(host(dC), want), (ta, tb)
+    # routed: 256 x 256 x 33000 (two chunks) through bzg_igemm
+    m2, n2, k2 = 256, 256, 33000
+    A2 = rng.integers(-2 ** 31, 2 ** 31, (m2, k2), dtype=np.int64).astype(np.int32)
+    B2 = rng.integers(-2 ** 31, 2 ** 31, (k2, n2), dtype=np.int64).astype(np.int32)
+    dA2, dB2, dC2 = dev(A2), dev(B2), full((m2, n2), np.int32, -7)
+    ctx().call("bzg_igemm", m2, n2, k2, 1, p(dA2), k2, p(dB2), n2, 0, p(dC2), n2)
+    assert np.array_equal(host(dC2), oracle.gemm(A2, B2))
     # k == 0: C = beta * C
-    ctx().call("bzg_igemm_digits", 0, 0, m, n, 0, 1, p(dA), 1, p(dB), n, 3, p(dC), n)
-    assert np.array_equal(host(dC), np.full((m, n), 3 * k, np.int32))
+    dC3 = full((m, n), np.int32, 5)
+    ctx().call("bzg_igemm_digits", 0, 0, m, n, 0, 1, p(dA), 1, p(dB), n, 3, p(dC3), n)
+    assert np.array_equal(host(dC3), np.full((m, n), 15, np.int32))
 
 
 def test_routed_products_exact():
