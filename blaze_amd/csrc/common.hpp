@@ -56,6 +56,9 @@ struct bzg_context {
   double* host_stats = nullptr;       // pinned readback of the split GEMM's operand statistics
   unsigned fp64acc_cc_gen = 0;       // workspace generation holding the split GEMM's CRT constants
   int fp64acc_cc_key = -1;            // ... and for which (slice type, moduli count); -1: none
+  // the residue GEMM's 128 KB dynamic-LDS limit has been raised on this context's device, per
+  // mode (0 BF16 slices, 1 INT8 slices, 2 INT32 digits)
+  bool residue_lds_attr[3] = {false, false, false};
   bool profile = false;
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
 };

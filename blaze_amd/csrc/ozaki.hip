@@ -1179,15 +1179,14 @@ bzg_status dgemm_fp64acc_impl(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
 #undef BZG_RES
   BZG_LAUNCH_CHECK();
   const size_t lds = 4 * RSTAGE;
-  static bool attr_set[2] = {false, false};
-  if (!attr_set[mode]) {
+  if (!ctx->residue_lds_attr[mode]) {
     if (mode == 0)
       BZG_HIP_CHECK(hipFuncSetAttribute((const void*)residue_gemm_kernel<0, 2>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     else
       BZG_HIP_CHECK(hipFuncSetAttribute((const void*)residue_gemm_kernel<1, 4>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set[mode] = true;
+    ctx->residue_lds_attr[mode] = true;
   }
   {
   ProfScope prof_gemm(ctx, "fp64acc_gemm");
@@ -1262,11 +1261,10 @@ bzg_status igemm_digits_chunk(bzg_context* ctx, bool ta, bool tb, int64_t m, int
                        (int)np, (int)kp, Bd);
   BZG_LAUNCH_CHECK();
   const size_t lds = 4 * RSTAGE;
-  static bool attr_set = false;
-  if (!attr_set) {
+  if (!ctx->residue_lds_attr[2]) {
     BZG_HIP_CHECK(hipFuncSetAttribute((const void*)residue_gemm_kernel<2, 4>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
+    ctx->residue_lds_attr[2] = true;
   }
   {
     ProfScope prof_gemm(ctx, "igemm_digits_gemm");
