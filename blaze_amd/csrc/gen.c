@@ -200,42 +200,114 @@ void bzg_gen_zipf_fill(uint64_t seed, int64_t n, int64_t ncols, const int32_t* r
 }
 
 /* ---- dense solve systems ------------------------------------------------------------------ */
-/* Apply H = I - 2 v v^T / (v^T v) from the left (side 0) or the right (side 1) to the n x n
- * row-major matrix M. */
-static void householder_apply(int n, const double* v, double* M, int side, double* tmp) {
-  double vv = 0;
-  for (int i = 0; i < n; i++) vv += v[i] * v[i];
-  const double f = 2.0 / vv;
-  if (side == 0) { /* M <- M - f v (v^T M) */
-    for (int j = 0; j < n; j++) tmp[j] = 0;
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) tmp[j] += v[i] * M[(int64_t)i * n + j];
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) M[(int64_t)i * n + j] -= f * v[i] * tmp[j];
-  } else { /* M <- M - f (M v) v^T */
-    for (int i = 0; i < n; i++) {
-      double s = 0;
-      for (int j = 0; j < n; j++) s += M[(int64_t)i * n + j] * v[j];
-      tmp[i] = s;
+/* Haar-distributed random orthogonal factors by Stewart's method (G. W. Stewart, "The efficient
+ * generation of random orthogonal matrices with an application to condition estimators",
+ * SIAM J. Numer. Anal. 17(3), 1980): Q = H_0 H_1 ... H_{n-2} D, where H_k is the Householder
+ * reflector of a fresh N(0, I) vector x_k of length n - k (acting on coordinates k..n-1), taking
+ * x_k to -sign(x_k[0]) ||x_k|| e_0, and D = diag(d) with d_k = -sign(x_k[0]) (d_{n-1} = the sign
+ * of one more N(0,1) draw).  This is the distribution of the Q factor of the Householder QR of
+ * an n x n Gaussian matrix with the signs of R's diagonal made positive (fresh Gaussian vectors
+ * stand in for the transformed columns by the rotational invariance of N(0, I)).
+ * The reflector vectors v_k (x_k with sign(x_k[0]) ||x_k|| added to its first entry) are stored
+ * one after another in `vs` (n(n+1)/2 - 1 values), their factors 2 / v_k^T v_k in `f`. */
+static void haar_reflectors(uint64_t seed, uint64_t stream, int n, double* vs, double* f, double* d) {
+  int64_t pos = 0;
+  for (int k = 0; k + 1 < n; k++) {
+    const int len = n - k;
+    double* v = vs + pos;
+    double nrm2 = 0.0;
+    for (int i = 0; i < len; i++) {
+      v[i] = ctr_normal(seed, stream, (uint64_t)pos + (uint64_t)i);
+      nrm2 += v[i] * v[i];
     }
+    const double sg = v[0] < 0.0 ? -1.0 : 1.0;
+    v[0] += sg * sqrt(nrm2);          /* H x = -sg ||x|| e_0 */
+    double vtv = 0.0;
+    for (int i = 0; i < len; i++) vtv += v[i] * v[i];
+    f[k] = vtv > 0.0 ? 2.0 / vtv : 0.0;
+    d[k] = -sg;                       /* R_kk = -sg ||x||: its sign is made positive */
+    pos += len;
+  }
+  if (n >= 1) d[n - 1] = ctr_normal(seed, stream, (uint64_t)pos) < 0.0 ? -1.0 : 1.0;
+}
+
+/* M <- Q M (side 0) or M <- M Q^T (side 1) for Q = H_0 ... H_{n-2} D, M n x n row-major. */
+static void haar_apply(int n, const double* vs, const double* f, const double* d, double* M, int side,
+                       double* tmp) {
+  int64_t end = 0;  /* one past the last reflector vector */
+  for (int k = 0; k + 1 < n; k++) end += n - k;
+  if (side == 0) {
+    /* Q M = H_0 (H_1 (... H_{n-2} (D M))) */
     for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) M[(int64_t)i * n + j] -= f * tmp[i] * v[j];
+      for (int j = 0; j < n; j++) M[(int64_t)i * n + j] *= d[i];
+    int64_t pos = end;
+    for (int k = n - 2; k >= 0; k--) {
+      const int len = n - k;
+      pos -= len;
+      const double* v = vs + pos;
+      for (int j = 0; j < n; j++) tmp[j] = 0.0;
+      for (int i = 0; i < len; i++) {
+        const double vi = v[i];
+        const double* row = M + (int64_t)(k + i) * n;
+        for (int j = 0; j < n; j++) tmp[j] += vi * row[j];
+      }
+      for (int i = 0; i < len; i++) {
+        const double c = f[k] * v[i];
+        double* row = M + (int64_t)(k + i) * n;
+        for (int j = 0; j < n; j++) row[j] -= c * tmp[j];
+      }
+    }
+  } else {
+    /* M Q^T = ((M D) H_{n-2}) ... H_0  (the reflectors are symmetric) */
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) M[(int64_t)i * n + j] *= d[j];
+    int64_t pos = end;
+    for (int k = n - 2; k >= 0; k--) {
+      const int len = n - k;
+      pos -= len;
+      const double* v = vs + pos;
+      for (int i = 0; i < n; i++) {
+        double* row = M + (int64_t)i * n + k;
+        double s = 0.0;
+        for (int j = 0; j < len; j++) s += row[j] * v[j];
+        s *= f[k];
+        for (int j = 0; j < len; j++) row[j] -= s * v[j];
+      }
+    }
   }
 }
 
+/* Q itself (n x n row-major, n <= 512), for the generator's tests. */
+void bzg_gen_haar_orthogonal(uint64_t seed, uint64_t stream, int32_t n, double* Q) {
+  if (n <= 0 || n > 512) return;
+  double* vs = (double*)malloc(sizeof(double) * ((size_t)n * (n + 1) / 2 + 1));
+  double* f = (double*)malloc(sizeof(double) * (size_t)n);
+  double* d = (double*)malloc(sizeof(double) * (size_t)n);
+  double* tmp = (double*)malloc(sizeof(double) * (size_t)n);
+  if (vs && f && d && tmp) {
+    haar_reflectors(seed, stream, n, vs, f, d);
+    memset(Q, 0, sizeof(double) * (size_t)n * (size_t)n);
+    for (int i = 0; i < n; i++) Q[(int64_t)i * n + i] = 1.0;
+    haar_apply(n, vs, f, d, Q, 0, tmp);
+  }
+  free(vs); free(f); free(d); free(tmp);
+}
+
 /* kind 0: A = R + n*I, R uniform[-1,1) (diagonally dominant).
- * kind 1: A = H1 H2 diag(logspace(0, -8, n)) H3 H4 with Householder reflectors H from
- *         Gaussian vectors: orthogonal factors, singular values exactly 10^(-8 t/(n-1)),
- *         so cond_2(A) = 1e8.
+ * kind 1: A = Q1 diag(logspace(0, -8, n)) Q2^T with Q1, Q2 independent Haar-random orthogonal
+ *         matrices (Stewart's method above: the Q factors of Gaussian matrices): singular
+ *         values exactly 10^(-8 t/(n-1)), so cond_2(A) = 1e8.
  * B uniform[-1,1), n x nrhs row-major.  Systems are independent (stream = system index). */
 void bzg_gen_solve_batch(uint64_t seed, int64_t count, const int32_t* n, const int32_t* nrhs,
                          const int8_t* kind, const int64_t* a_off, const int64_t* b_off,
                          double* A, double* B) {
 #pragma omp parallel
   {
-    double* v = (double*)malloc(sizeof(double) * 512);
+    double* vs = (double*)malloc(sizeof(double) * (512 * 513 / 2 + 1));
+    double* f = (double*)malloc(sizeof(double) * 512);
+    double* d = (double*)malloc(sizeof(double) * 512);
     double* tmp = (double*)malloc(sizeof(double) * 512);
-#pragma omp for schedule(dynamic, 16)
+#pragma omp for schedule(dynamic, 1)
     for (int64_t s = 0; s < count; s++) {
       const int ns = n[s];
       double* M = A + a_off[s];
@@ -243,19 +315,21 @@ void bzg_gen_solve_batch(uint64_t seed, int64_t count, const int32_t* n, const i
       if (kind[s] == 0) {
         for (int64_t i = 0; i < (int64_t)ns * ns; i++) M[i] = ctr_pm1(seed, st, (uint64_t)i);
         for (int i = 0; i < ns; i++) M[(int64_t)i * ns + i] += (double)ns;
-      } else {
+      } else if (ns > 0 && ns <= 512) {
         memset(M, 0, sizeof(double) * (size_t)ns * (size_t)ns);
         for (int i = 0; i < ns; i++)
           M[(int64_t)i * ns + i] = ns > 1 ? pow(10.0, -8.0 * (double)i / (double)(ns - 1)) : 1.0;
-        for (int h = 0; h < 4; h++) {
-          for (int i = 0; i < ns; i++) v[i] = ctr_normal(seed, st + 1 + (uint64_t)h, (uint64_t)i);
-          householder_apply(ns, v, M, h < 2 ? 0 : 1, tmp);
-        }
+        haar_reflectors(seed, st + 2, ns, vs, f, d); /* Q2 */
+        haar_apply(ns, vs, f, d, M, 1, tmp);
+        haar_reflectors(seed, st + 1, ns, vs, f, d); /* Q1 */
+        haar_apply(ns, vs, f, d, M, 0, tmp);
       }
       double* R = B + b_off[s];
       for (int64_t i = 0; i < (int64_t)ns * nrhs[s]; i++) R[i] = ctr_pm1(seed, st + 7, (uint64_t)i);
     }
-    free(v);
+    free(vs);
+    free(f);
+    free(d);
     free(tmp);
   }
 }
