@@ -369,9 +369,13 @@ bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_h
  * 2^-P sqrt((sum_i D_i^2 ||B||_F^2 + sum_j E_j^2 ||A||_F^2) / (12 sum_q ||A_:q||^2 ||B_q:||^2))
  * with P the bits per operand the count allows and D_i (E_j) the power of two above row i's
  * (column j's) largest |entry|; the smallest count whose estimate is <= 8e-15 is used (24 if
- * none is).  This reads 5 numbers back to the host (one stream synchronisation), so capture a
- * graph with an explicit count.  Workspace is allocated from the context on first use and
- * cached. */
+ * none is), or <= 3.2e-15 when single entries dominate their rows or columns (k sum_i D_i^2 >
+ * 12 ||A||_F^2 or the same for B), where the realised error scatters more widely around the
+ * estimate.  This reads 5 numbers back to the host (one stream synchronisation), so capture a
+ * graph with an explicit count.  A NaN or an infinity in A or B (seen by that pass) makes the
+ * call the plain FP64 GEMM (bzg_dgemm), so such values propagate exactly as there; with an
+ * explicit count the rows of A and columns of B that hold them are recomputed in FP64 on the
+ * device.  Workspace is allocated from the context on first use and cached. */
 bzg_status bzg_dgemm_fp64acc(bzg_context* ctx, int64_t m, int64_t n, int64_t k,
                              const double* A, int64_t lda, const double* B, int64_t ldb,
                              double* C, int64_t ldc, int32_t moduli);
