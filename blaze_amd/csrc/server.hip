@@ -584,17 +584,35 @@ bzg_status bzg_server_add_csr(bzg_server* s, int64_t n_rows, int64_t n_cols, con
     s->peak = std::max(s->peak, s->used);
   }
   bzg_context* c = s->ctx[0];
-  BZG_HIP_CHECK(hipMalloc(&M.rowptr, (size_t)(n_rows + 1) * 4));
-  BZG_HIP_CHECK(hipMalloc(&M.colidx, std::max<size_t>((size_t)nnz * 4, 4)));
-  BZG_HIP_CHECK(hipMalloc(&M.val, std::max<size_t>((size_t)nnz * 8, 8)));
-  BZG_HIP_CHECK(hipMemcpyAsync(M.rowptr, rowptr, (size_t)(n_rows + 1) * 4, hipMemcpyHostToDevice, c->stream));
-  if (nnz) {
-    BZG_HIP_CHECK(hipMemcpyAsync(M.colidx, colidx, (size_t)nnz * 4, hipMemcpyHostToDevice, c->stream));
-    BZG_HIP_CHECK(hipMemcpyAsync(M.val, val, (size_t)nnz * 8, hipMemcpyHostToDevice, c->stream));
-  }
+  // every failure below frees what was allocated and returns the budget reserved above
+  auto fail = [&](bzg_status st, const std::string& what) {
+    if (M.plan) bzg_csr_plan_destroy(M.plan);
+    if (M.rowptr) hipFree(M.rowptr);
+    if (M.colidx) hipFree(M.colidx);
+    if (M.val) hipFree(M.val);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->used -= M.bytes;
+    }
+    if (!what.empty()) bzg::set_error("server_add_csr: " + what);
+    return st;
+  };
+  hipError_t e = hipMalloc(&M.rowptr, (size_t)(n_rows + 1) * 4);
+  if (e == hipSuccess) e = hipMalloc(&M.colidx, std::max<size_t>((size_t)nnz * 4, 4));
+  if (e == hipSuccess) e = hipMalloc(&M.val, std::max<size_t>((size_t)nnz * 8, 8));
+  if (e != hipSuccess) return fail(BZG_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemcpyAsync(M.rowptr, rowptr, (size_t)(n_rows + 1) * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && nnz) e = hipMemcpyAsync(M.colidx, colidx, (size_t)nnz * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && nnz) e = hipMemcpyAsync(M.val, val, (size_t)nnz * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(BZG_ERR_HIP, std::string("copy: ") + hipGetErrorString(e));
   bzg_status st = bzg_csr_plan_create(c, n_rows, n_cols, rowptr, &M.plan);
-  if (st) return st;
-  BZG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (st) {
+    M.plan = nullptr;
+    return fail(st, "");  // the plan's own message stays in bzg_last_error
+  }
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return fail(BZG_ERR_HIP, std::string("plan: ") + hipGetErrorString(e));
   std::lock_guard<std::mutex> lk(s->mu);
   s->csrs.push_back(M);
   *handle = (int32_t)s->csrs.size() - 1;

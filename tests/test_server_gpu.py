@@ -170,3 +170,27 @@ def test_budget_admission_and_release():
         assert e.value.status == -4
     finally:
         srv.close()
+
+
+def test_add_csr_failure_returns_budget():
+    # a matrix the plan builder rejects (rowptr[0] != 0) frees its device copies and gives its
+    # budget back: the same budget then still holds a valid matrix of almost its whole size
+    import numpy as np
+    import blaze_amd as bz
+    from blaze_amd.serve import Server
+    n = 20000
+    rp = np.arange(n + 1, dtype=np.int32)                 # one entry per row: 4 (n+1) + 12 n bytes
+    ci = np.arange(n, dtype=np.int32); va = np.ones(n)
+    need = 4 * (n + 1) + 12 * n
+    srv = Server(budget_bytes=need + 1024, max_inflight=4, streams=1)
+    try:
+        bad = rp.copy(); bad[0] = 1
+        for _ in range(3):
+            with pytest.raises(bz.BlazeGpuError) as e:
+                srv.add_csr(bad, ci, va, n)
+            assert e.value.status == -1
+            assert srv.stats()["bytes_in_use"] == 0
+        h = srv.add_csr(rp, ci, va, n)
+        assert h == 0 and srv.stats()["bytes_in_use"] == need
+    finally:
+        srv.close()
