@@ -49,6 +49,43 @@ def have_ref() -> bool:
     return REF_SO.exists()
 
 
+REF_LAPACK_SO = HERE / "_ref" / "libblazeref_lapack.so"
+_ref_lapack = None
+
+
+def have_ref_lapack() -> bool:
+    return REF_LAPACK_SO.exists()
+
+
+def ref_lapack() -> C.CDLL:
+    """The reference's solve() (LAPACK-backed for N >= 7) compiled against the image's LAPACK;
+    LAPACK runs sequentially, the systems are spread over OpenMP threads."""
+    global _ref_lapack
+    if _ref_lapack is None:
+        if not REF_LAPACK_SO.exists():
+            raise FileNotFoundError(f"{REF_LAPACK_SO} not built (needs the reference tree and a LAPACK)")
+        import os
+        os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+        os.environ.setdefault("MKL_NUM_THREADS", "1")
+        L = C.CDLL(str(REF_LAPACK_SO))
+        L.ref_time_dsolve_batch.restype = f64
+        L.ref_time_dsolve_batch.argtypes = [i64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(i64)]
+        _ref_lapack = L
+    return _ref_lapack
+
+
+def ref_solve_batch(n, nrhs, a_off, b_off, A, B, threads=0, reps=0):
+    """(X, seconds, fails): the reference's solve(A_i, X_i, B_i) for every system of the pools."""
+    X = np.zeros_like(B)
+    fails = i64(0)
+    arrs = [np.ascontiguousarray(n, np.int32), np.ascontiguousarray(nrhs, np.int32),
+            np.ascontiguousarray(a_off, np.int64), np.ascontiguousarray(b_off, np.int64)]
+    t = ref_lapack().ref_time_dsolve_batch(len(arrs[0]), *[_p(a) for a in arrs], _p(A), _p(B), _p(X),
+                                           int(threads), int(reps), C.byref(fails))
+    return X, t, fails.value
+
+
 def ref() -> C.CDLL:
     global _ref
     if _ref is None:

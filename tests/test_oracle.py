@@ -395,3 +395,24 @@ def test_cg_oracle_converges_to_direct_solve():
     x0 = rng.uniform(-1, 1, n)
     x, rr = oracle.cg(rp, ci, va, np.zeros(n), x0, 120)
     assert np.abs(x).max() <= 1e-10
+
+
+def test_reference_solve_batch_lapack():
+    # the reference's own solve() (closed forms for n <= 6, dgesv_ through the image's LAPACK
+    # above), the CPU baseline of the batched solves: solutions agree with the oracle's gesv
+    if not oracle.have_ref_lapack():
+        pytest.skip("reference solve() with LAPACK not built")
+    rng = np.random.default_rng(5)
+    n = np.array([4, 5, 6, 7, 16, 65, 200], np.int32)
+    r = np.array([1, 3, 2, 8, 1, 4, 5], np.int32)
+    a_off = np.zeros(len(n), np.int64); b_off = np.zeros(len(n), np.int64)
+    a_off[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]; b_off[1:] = np.cumsum(n.astype(np.int64) * r)[:-1]
+    As = [rng.uniform(-1, 1, (k, k)) + k * np.eye(k) for k in n]
+    Bs = [rng.uniform(-1, 1, (k, q)) for k, q in zip(n, r)]
+    A = np.concatenate([a.ravel() for a in As]); B = np.concatenate([b.ravel() for b in Bs])
+    X, t, fails = oracle.ref_solve_batch(n, r, a_off, b_off, A, B, threads=2, reps=1)
+    assert fails == 0 and t > 0
+    for i, (a, b) in enumerate(zip(As, Bs)):
+        x = X[b_off[i]:b_off[i] + b.size].reshape(b.shape)
+        xo, _, _, info = oracle.gesv(a, b)
+        assert info == 0 and np.abs(x - xo).max() <= 1e-13 * max(1.0, np.abs(xo).max())
