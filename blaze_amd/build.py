@@ -36,7 +36,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["context.hip", "gemm.hip", "gemv.hip", "spmv.hip", "cg.hip", "lu.hip", "ozaki.hip",
+HIP_SOURCES = ["context.hip", "gemm.hip", "gemv.hip", "spmv.hip", "cg.hip", "lu.hip", "lu_mixed.hip", "ozaki.hip",
                "server.hip", "mmio.hip"]
 
 
@@ -68,7 +68,7 @@ def build(verbose: bool = True) -> Path:
 
     gen_out = LIB / "libbzg_gen.so"
     gen_tmp = LIB / "libbzg_gen.so.tmp"
-    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-fopenmp", f"-I{INCLUDE}",
+    _run(["gcc", "-O3", "-march=x86-64-v3", "-std=c11", "-fPIC", "-shared", "-fopenmp", f"-I{INCLUDE}",
           str(CSRC / "gen.c"), "-o", str(gen_tmp), "-lm"])
     os.replace(gen_tmp, gen_out)
     if verbose:

@@ -84,6 +84,17 @@ struct ProfScope {
   }
 };
 
+// On-chip mixed-precision batched solver (lu_mixed.hip): stable partition of a batch into
+// order classes, and the launches of the class kernels.
+bzg_status launch_partition(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,
+                            const int32_t* order, int32_t* info, int32_t* iter, int32_t* lists, int cap,
+                            int* counts);
+bzg_status launch_dsgesv_onchip(bzg_context* ctx, const int32_t* lists, int cap, const int* counts,
+                                const int32_t* n, const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
+                                const double* A, const double* B, double* X, int32_t* iter, int32_t* info,
+                                int* counters, int32_t* fb_list, int* fb_count, double tol, int max_iter,
+                                const int* wg_per_cu, int64_t count);
+
 // Returns a cached device buffer of at least `bytes` (slot-indexed); grows on demand.
 bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out);
 

@@ -728,9 +728,9 @@ __global__ __launch_bounds__(NT) void dsgesv_batched_kernel(
   }
 }
 
-// FP64 fallback for the systems listed by the mixed-precision kernel: copy A to the
-// workgroup's FP64 scratch, factor, solve B into X.  One instance (NM = NMAX) serves every
-// order: fallbacks are rare.
+// FP64 fallback for the systems listed by the mixed-precision kernels: copy A to the
+// workgroup's FP64 scratch, factor, solve B into X.  The NM = NSMALL instance takes the listed
+// systems of order <= NSMALL, the NM = NMAX instance the others (each skips the rest).
 template <int NM>
 __global__ __launch_bounds__(NT) void gesv_fallback_kernel(
     const int32_t* __restrict__ nv, const int32_t* __restrict__ rv,
@@ -751,6 +751,7 @@ __global__ __launch_bounds__(NT) void gesv_fallback_kernel(
     if (idx >= count) break;
     const int s = list[idx];
     const int n = nv[s], nrhs = rv[s];
+    if ((NM == NSMALL) != (n <= NSMALL)) continue;
     const double* a = A + a_off[s];
     for (int e = tid; e < n * n; e += NT) W[e] = a[e];
     __syncthreads();
@@ -801,6 +802,9 @@ struct SolverTuning {
   int wg_per_cu_mixed = 8;
 };
 static SolverTuning g_tuning;
+// Workgroups per CU of the on-chip mixed kernels for the order classes 1..32, 33..64, 65..128,
+// 129..256 (at most what registers and LDS let reside; extra workgroups find the list drained).
+static int g_onchip_wg_per_cu[4] = {12, 6, 2, 1};
 
 }  // namespace bzg
 
@@ -837,18 +841,13 @@ bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_h
 namespace bzg {
 namespace {
 // Batched LU entry points (MODE as in gesv_batched_kernel).
-bzg_status lu_batched(bzg_context* ctx, int mode, int64_t count, const int32_t* n, const int32_t* nrhs,
-                      const int64_t* a_off, const int64_t* b_off, const int64_t* p_off, double* A,
-                      double* B, int32_t* ipiv, int32_t* info, const int32_t* order) {
-  // Scratch: slot 0 = counters [0] small-kernel work counter, [1] big count, [2] big-kernel
-  // work counter; slot 3 = list of the systems of order > NSMALL
-  void *cnt = nullptr, *lst = nullptr;
-  bzg_status st = workspace(ctx, 0, 256, &cnt);
-  if (!st) st = workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
-  if (st) return st;
-  int* counters = static_cast<int*>(cnt);
-  int32_t* big = static_cast<int32_t*>(lst);
-  ProfScope prof(ctx, mode == 0 ? "dgesv" : (mode == 1 ? "dgetrf" : "dgetrs"));
+// Scratch of one call: counters[0] small-kernel work counter, [1] big count, [2] big-kernel
+// work counter (at least 8 ints, cleared by a kernel); big = room for `count` system indices
+// (the systems of order > NSMALL).  Allocates nothing.
+bzg_status lu_batched_scratch(bzg_context* ctx, int mode, int64_t count, const int32_t* n, const int32_t* nrhs,
+                              const int64_t* a_off, const int64_t* b_off, const int64_t* p_off, double* A,
+                              double* B, int32_t* ipiv, int32_t* info, const int32_t* order, int* counters,
+                              int32_t* big) {
   hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 8);
   BZG_LAUNCH_CHECK();
   hipLaunchKernelGGL(classify_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -868,6 +867,19 @@ bzg_status lu_batched(bzg_context* ctx, int mode, int64_t count, const int32_t* 
                      (const int32_t*)big, (const int*)(counters + 1), counters + 0, mode);
   BZG_LAUNCH_CHECK();
   return BZG_OK;
+}
+
+// The same with the scratch cached in the context: slot 0 = counters, slot 3 = big list.
+bzg_status lu_batched(bzg_context* ctx, int mode, int64_t count, const int32_t* n, const int32_t* nrhs,
+                      const int64_t* a_off, const int64_t* b_off, const int64_t* p_off, double* A,
+                      double* B, int32_t* ipiv, int32_t* info, const int32_t* order) {
+  void *cnt = nullptr, *lst = nullptr;
+  bzg_status st = workspace(ctx, 0, 256, &cnt);
+  if (!st) st = workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
+  if (st) return st;
+  ProfScope prof(ctx, mode == 0 ? "dgesv" : (mode == 1 ? "dgetrf" : "dgetrs"));
+  return lu_batched_scratch(ctx, mode, count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
+                            static_cast<int*>(cnt), static_cast<int32_t*>(lst));
 }
 
 // Determinants from the factors (one thread per system), detNxN's rule
@@ -1129,47 +1141,51 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
   BZG_REQUIRE(n && nrhs && a_off && b_off && A && B && X && iter && info, BZG_ERR_INVALID_ARGUMENT,
               "dsgesv_batched: null array");
   BZG_REQUIRE(max_iter >= 0 && max_iter <= 1000, BZG_ERR_INVALID_ARGUMENT, "dsgesv_batched: bad max_iter");
-  // persistent grids never larger than the batch; the workspace is sized for the full grid
-  // so that it is not regrown as batch sizes vary
+  // Orders 1..256 run on chip (lu_mixed.hip, four order classes), orders 257..512 in the
+  // streaming kernel below; systems that do not converge fall back to the FP64 LU.
+  // Persistent grids never larger than the batch; the workspaces are sized for the full grids
+  // so that they are not regrown as batch sizes vary.
   const int64_t cap = std::max<int64_t>(count, 1);
-  const int grid32_full = ctx->num_cus * bzg::g_tuning.wg_per_cu_mixed;
   const int grid_big_full = ctx->num_cus * 2;
-  const int grid32 = (int)std::min<int64_t>(grid32_full, cap);
+  const int grid_fb_full = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
   const int grid_big = (int)std::min<int64_t>(grid_big_full, cap);
-  // scratch: slot 0 counters ([0] small work, [1] fallback count, [2] fallback work,
-  // [3] big count, [4] big work); slot 1 FP32 matrices of the small kernel's workgroups;
-  // slot 5 FP32 matrices of the big kernel's; slot 2 FP64 matrices of the fallback kernel;
-  // slot 3 fallback list then big list
-  void *cnt = nullptr, *w32 = nullptr, *w32b = nullptr, *w64 = nullptr, *lst = nullptr;
+  const int grid_fb = (int)std::min<int64_t>(grid_fb_full, cap);
+  // scratch: slot 0 counters ([0..3] on-chip class work counters, [4] fallback count, [5] and
+  // [6] fallback work counters, [7] order-257..512 work counter, [8..12] class sizes);
+  // slot 3 the five class lists (count entries each) and the fallback list; slot 5 FP32
+  // matrices of the order-257..512 kernel's workgroups; slot 2 FP64 matrices of the fallback
+  // kernels' workgroups
+  void *cnt = nullptr, *w32b = nullptr, *w64 = nullptr, *lst = nullptr;
   const size_t nn_small = (size_t)bzg::NSMALL * bzg::NSMALL, nn_big = (size_t)bzg::NMAX * bzg::NMAX;
+  const size_t w64_bytes = std::max((size_t)grid_fb_full * nn_small, (size_t)grid_big_full * nn_big) * sizeof(double);
   bzg_status st = bzg::workspace(ctx, 0, 256, &cnt);
-  if (!st) st = bzg::workspace(ctx, 1, (size_t)grid32_full * nn_small * sizeof(float), &w32);
   if (!st) st = bzg::workspace(ctx, 5, (size_t)grid_big_full * nn_big * sizeof(float), &w32b);
-  if (!st) st = bzg::workspace(ctx, 2, (size_t)grid_big_full * nn_big * sizeof(double), &w64);
-  if (!st) st = bzg::workspace(ctx, 3, (size_t)2 * count * sizeof(int32_t), &lst);
+  if (!st) st = bzg::workspace(ctx, 2, w64_bytes, &w64);
+  if (!st) st = bzg::workspace(ctx, 3, (size_t)6 * count * sizeof(int32_t), &lst);
   if (st) return st;
   int* counters = static_cast<int*>(cnt);
-  int32_t* fb_list = static_cast<int32_t*>(lst);
-  int32_t* big = fb_list + count;
+  int32_t* lists = static_cast<int32_t*>(lst);
+  int32_t* fb_list = lists + 5 * count;
   bzg::ProfScope prof(ctx, "dsgesv");
-  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 8);
+  hipLaunchKernelGGL(bzg::reset_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, counters, 16);
   BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bzg::classify_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
-                     (int)count, n, nrhs, info, iter, big, counters + 3);
-  BZG_LAUNCH_CHECK();
+  st = bzg::launch_partition(ctx, count, n, nrhs, order, info, iter, lists, (int)count, counters + 8);
+  if (st) return st;
   hipLaunchKernelGGL(bzg::dsgesv_batched_kernel<bzg::NMAX>, dim3(grid_big), dim3(bzg::NT), 0, ctx->stream,
                      (int)count, n, nrhs, a_off, b_off, A, B, X, iter, info, order,
-                     (const int32_t*)big, (const int*)(counters + 3), counters + 4,
-                     static_cast<float*>(w32b), fb_list, counters + 1, tol, max_iter);
+                     (const int32_t*)(lists + 4 * count), (const int*)(counters + 12), counters + 7,
+                     static_cast<float*>(w32b), fb_list, counters + 4, tol, max_iter);
   BZG_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bzg::dsgesv_batched_kernel<bzg::NSMALL>, dim3(grid32), dim3(bzg::NT), 0, ctx->stream,
-                     (int)count, n, nrhs, a_off, b_off, A, B, X, iter, info, order,
-                     (const int32_t*)big, (const int*)(counters + 3), counters + 0,
-                     static_cast<float*>(w32), fb_list, counters + 1, tol, max_iter);
+  st = bzg::launch_dsgesv_onchip(ctx, lists, (int)count, counters + 8, n, nrhs, a_off, b_off, A, B, X, iter, info,
+                                 counters, fb_list, counters + 4, tol, max_iter, bzg::g_onchip_wg_per_cu, count);
+  if (st) return st;
+  hipLaunchKernelGGL(bzg::gesv_fallback_kernel<bzg::NSMALL>, dim3(grid_fb), dim3(bzg::NT), 0, ctx->stream, n,
+                     nrhs, a_off, b_off, A, B, X, info, static_cast<const int32_t*>(fb_list), counters + 4,
+                     counters + 5, static_cast<double*>(w64));
   BZG_LAUNCH_CHECK();
   hipLaunchKernelGGL(bzg::gesv_fallback_kernel<bzg::NMAX>, dim3(grid_big), dim3(bzg::NT), 0, ctx->stream, n,
-                     nrhs, a_off, b_off, A, B, X, info, static_cast<const int32_t*>(fb_list), counters + 1,
-                     counters + 2, static_cast<double*>(w64));
+                     nrhs, a_off, b_off, A, B, X, info, static_cast<const int32_t*>(fb_list), counters + 4,
+                     counters + 6, static_cast<double*>(w64));
   BZG_LAUNCH_CHECK();
   return BZG_OK;
 }
