@@ -7,18 +7,25 @@ TFLOP/s counted as 2*M*N*K FP64-equivalent flops per step, inputs resident in HB
 
 Beside it (fields of the same JSON line):
   native_fp64      the plain FP64 MFMA GEMM on the same inputs (the comparator) and the ratio;
-  fp64acc_*        both slice variants, the rectangular 16384 x 1024 x 16384 case, and the
-                   relative Frobenius error per exponent spread sigma in {0, 4, 10} against an
-                   extended-precision (x87 long double) reference on sampled entries;
-  solve            100,000 systems, n ~ U{4..256}, nrhs ~ U{1..8}, 90% A = R + n I, 10% with
-                   cond = 1e8: plain FP64 batched LU vs FP32 LU + FP64 refinement, systems/s,
-                   p50/p99 batch latency, worst backward error (target 1e-12), fallback count;
+  fp64acc_*        both slice variants; errors are relative Frobenius norms over 16 whole rows
+                   of C against an extended-precision (x87 long double) product;
+  fp64acc_8192_by_sigma   the same product for entries u 2^e, e = round(N(0, sigma)), sigma in
+                   {0, 4, 10}: rate of the automatic moduli count, ratio to native, error;
+  rel_frob_err_by_sigma_512x512x8192   full-matrix errors per sigma on a smaller M, N;
+  fp64acc_16384x1024x16384   the rectangular case;
+  plain_gemm_512   the plain FP64 GEMM at 512^3 on mt19937_64 inputs (the reference's
+                   CPU-runnable case), rate and FP64 MFMA fraction;
+  solve            100,000 systems, n ~ U{4..256}, nrhs ~ U{1..8}, 90% A = R + n I, 10%
+                   Q1 diag(logspace(0,-8,n)) Q2^T with Haar-random Q (cond 1e8): plain FP64
+                   batched LU vs FP32 LU + FP64 refinement, systems/s, p50/p99 batch latency,
+                   worst backward error (target 1e-12), fallback count;
   spmv             CSR SpMV GB/s (algorithmic bytes) for the 128^3 7-point Laplacian (fp64,
                    fp32) and the n = 1e6 power-law matrix, rotating copies past the 256 MB MALL;
-  serve            (when built) the continuous-batching server's p50/p99 per request class;
+  gemv, cg, serve  dense GEMV, the CG loop, the continuous-batching server (p50/p99 per class);
   roofline         dominant kernel (the residue-slice GEMM) from HIP events on its stream;
-  cpu_baseline     the reference's own GEMM kernel (Blaze, compiled from its headers into
-                   oracle/_ref) on the host cores, on a bounded 4096^3 sample.
+  cpu_baseline     the reference's own GEMM (Blaze, compiled from its headers into oracle/_ref)
+                   on the host cores at the headline size; every other leg carries its own
+                   `cpu_baseline` (Blaze GEMM 512^3, SpMV, GEMV, solve() with LAPACK).
 """
 from __future__ import annotations
 
@@ -101,31 +108,44 @@ class Bench:
         return float(np.sqrt(np.sum((got.astype(np.longdouble) - ref) ** 2) / np.sum(ref ** 2)))
 
     # -- GEMM legs -------------------------------------------------------------------------
-    def gemm_case(self, m, n, k, A, B, steps, warmup, variants):
+    def gemm_case(self, m, n, k, A, B, steps, warmup, variants, whole_rows=16):
+        """Native FP64 and each split variant: rate, and the relative Frobenius error over
+        `whole_rows` complete rows of C (every column) against an extended-precision product."""
         torch = self.torch
         dA, dB = self.dev(A), self.dev(B)
         dC = torch.empty((m, n), dtype=torch.float64, device="cuda")
         fl = 2.0 * m * n * k
-        rng = np.random.default_rng(7)
-        rows = rng.integers(0, m, 512); cols = rng.integers(0, n, 512)
-        ref = self.ref_entries(A, B, rows, cols)
+        rows = np.sort(np.random.default_rng(7).choice(m, size=min(whole_rows, m), replace=False))
+        ref = self.ref_rows(A, B, rows)
+        drows = torch.from_numpy(rows).to("cuda")
+        err = lambda: self.rel_frob(dC.index_select(0, drows).cpu().numpy(), ref)
+        sample = f"{len(rows)} whole rows of C ({len(rows) * n} entries) against an x87 long-double product"
         out = {}
         t = self.timed(lambda: self.ctx.call("bzg_dgemm", m, n, k, 1.0, vp(dA), k, vp(dB), n, 0.0, vp(dC), n),
                        steps, warmup)
-        Ch = dC.cpu().numpy()
-        out["native_fp64"] = {"tflops": fl / t / 1e12, "ms": t * 1e3,
-                              "rel_frob_err": self.rel_frob(Ch[rows, cols], ref)}
+        out["native_fp64"] = {"tflops": fl / t / 1e12, "ms": t * 1e3, "rel_frob_err": err(),
+                              "err_sample": sample}
         for name, fn in variants:
             t = self.timed(lambda: self.ctx.call(fn, m, n, k, vp(dA), k, vp(dB), n, vp(dC), n, 0),
                            steps, warmup)
-            Ch = dC.cpu().numpy()
             nm, bits, est = C.c_int32(), C.c_int32(), C.c_double()
             self.ctx.call("bzg_fp64acc_last_plan", C.byref(nm), C.byref(bits), C.byref(est))
-            out[name] = {"tflops": fl / t / 1e12, "ms": t * 1e3,
-                         "rel_frob_err": self.rel_frob(Ch[rows, cols], ref),
+            out[name] = {"tflops": fl / t / 1e12, "ms": t * 1e3, "rel_frob_err": err(), "err_sample": sample,
                          "vs_native": (fl / t / 1e12) / out["native_fp64"]["tflops"],
                          "moduli": nm.value, "operand_bits": bits.value, "estimated_err": est.value}
+        del dA, dB, dC
         return out
+
+    @staticmethod
+    def ref_rows(A, B, rows):
+        """Extended-precision (x87 long double, 64-bit significand) rows of A @ B, one thread per
+        block of rows."""
+        from concurrent.futures import ThreadPoolExecutor
+        Bl = B.astype(np.longdouble)
+        parts = [p for p in np.array_split(rows, min(16, len(rows))) if len(p)]
+        with ThreadPoolExecutor(len(parts)) as ex:
+            outs = list(ex.map(lambda r: A[r].astype(np.longdouble) @ Bl, parts))
+        return np.concatenate(outs)
 
     @staticmethod
     def ref_full(A, B):
@@ -205,6 +225,7 @@ class Bench:
         log(f"[solve] generated {count} systems ({na * 8 / 1e9:.1f} GB) in {time.perf_counter() - t0:.1f}s")
         dn, dr, da, db = self.dev(n), self.dev(r), self.dev(a_off), self.dev(b_off)
         dA0, dB0 = self.dev(A), self.dev(B)
+        self.solve_inputs = (n, r, a_off, b_off, A, B) if "cpu" in self.args.legs else None
         del A
         order = torch.empty(count, dtype=torch.int32, device="cuda")
         self.ctx.call("bzg_batched_order", count, n.ctypes.data_as(C.c_void_p), vp(order))
@@ -351,6 +372,7 @@ class Bench:
         g.bzg_gen_laplace3d(C.c_int64(nx), rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
                             va.ctypes.data_as(C.c_void_p))
         x = self.uniform(nr, 11)
+        self.spmv_inputs = {"laplace128_fp64": (rp, ci, va, x, nr)}
         out["laplace128_fp64"] = self._spmv_run(rp, ci, va, x, nr, "bzg_dcsrmv", steps)
         out["laplace128_fp32"] = self._spmv_run(rp, ci, va.astype(np.float32), x.astype(np.float32), nr,
                                                 "bzg_scsrmv", steps)
@@ -363,6 +385,7 @@ class Bench:
         g.bzg_gen_zipf_fill(C.c_uint64(SEED), C.c_int64(n), C.c_int64(n), rp.ctypes.data_as(C.c_void_p),
                             ci.ctypes.data_as(C.c_void_p), va.ctypes.data_as(C.c_void_p))
         x = self.uniform(n, 12)
+        self.spmv_inputs["zipf1e6_s1.5_fp64"] = (rp, ci, va, x, n)
         out["zipf1e6_s1.5_fp64"] = self._spmv_run(rp, ci, va, x, n, "bzg_dcsrmv", steps)
         out["zipf1e6_s1.5_fp64"]["mean_row"] = nnz / n
         # y = trans(S) * x (TSMatDVecMultExpr) on the same matrix: the host pack gives the CSR of
@@ -541,36 +564,152 @@ class Bench:
                 "gbps": it_bytes / t_it / 1e9, "frac_hbm": it_bytes / t_it / 1e9 / PEAK["hbm"],
                 "timing": "one bzg_dcg call captured in a HIP graph, replayed; per iteration = call / (iterations + 1)"}
 
+    # -- plain GEMM at 512^3 (the reference's CPU-runnable GEMM case) --------------------------
+    def gemm512_leg(self, nn=512, calls=200, reps=5):
+        """C = A * B, M = N = K = 512, A then B from one mt19937_64 stream (u = (x >> 11) 2^-53 * 2
+        - 1, row-major); calls replayed back to back from a HIP graph, timed with events.  Rate
+        in 2 N^3 flops (and blazemark's 2 N^3 - N^2) against the FP64 MFMA peak."""
+        torch, bz = self.torch, self.bz
+        A = np.empty((nn, nn)); B = np.empty((nn, nn))
+        self.gen.bzg_gen_uniform_mt(C.c_uint64(SEED), C.c_int64(A.size), A.ctypes.data_as(C.c_void_p),
+                                    C.c_int64(B.size), B.ctypes.data_as(C.c_void_p))
+        self.gemm512_inputs = (A, B)
+        dA, dB = self.dev(A), self.dev(B)
+        dC = torch.empty((nn, nn), dtype=torch.float64, device="cuda")
+        s = torch.cuda.Stream()
+        gctx = bz.Context(stream=s.cuda_stream)
+
+        def many():
+            for _ in range(calls):
+                gctx.call("bzg_dgemm", nn, nn, nn, 1.0, vp(dA), nn, vp(dB), nn, 0.0, vp(dC), nn)
+        with torch.cuda.stream(s):
+            many()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            many()
+        g.replay()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(reps):
+            g.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        us = ev[0].elapsed_time(ev[1]) * 1e3 / (reps * calls)
+        del g
+        gctx.close()
+        # single host-launched calls (ctypes launch included), for comparison
+        t1 = self.timed(lambda: self.ctx.call("bzg_dgemm", nn, nn, nn, 1.0, vp(dA), nn, vp(dB), nn, 0.0, vp(dC), nn),
+                        20, 3)
+        fl = 2.0 * nn ** 3
+        return {"workload": f"C = A*B, DynamicMatrix<double,rowMajor> {nn}^3, A,B uniform[-1,1) from mt19937_64",
+                "us": us, "tflops": fl / us / 1e6, "frac_fp64_mfma_peak": fl / us / 1e6 / PEAK["fp64"],
+                "tflops_blazemark_convention": (2.0 * nn ** 3 - nn ** 2) / us / 1e6,
+                "us_host_launched": t1 * 1e6,
+                "timing": "calls replayed back to back from a HIP graph, events around the replays"}
+
     # -- CPU baseline -------------------------------------------------------------------------
-    def cpu_baseline(self):
-        # The reference's own GEMM (Blaze DMatDMatMultExpr, OpenMP) compiled from its headers
-        # into oracle/_ref; if that build is absent, the oracle's C restatement (1 core).
-        import oracle
+    def cpu_cores(self):
         cores = os.cpu_count() or 1
         try:
-            share = len(os.sched_getaffinity(0))
-            cores = min(cores, share)
+            cores = min(cores, len(os.sched_getaffinity(0)))
         except AttributeError:
             pass
-        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-        if oracle.have_ref():
-            L = oracle.ref()
-            L.ref_set_threads(C.c_int(cores))
-            nn = 4096
+        return min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+
+    def cpu_baseline(self, rec):
+        """The reference's own kernels (Blaze compiled from its headers into oracle/_ref; OpenMP
+        over the host cores; blazemark protocol: one warm-up, then the best of a few timed
+        assignments) on the operands the GPU legs used.  Returns the headline object (GEMM at the
+        headline size) and fills each GPU leg's record with its own `cpu_baseline`."""
+        import oracle
+        cores = self.cpu_cores()
+        if not oracle.have_ref():
+            nn = 512
             A = self.uniform((nn, nn), 21); B = self.uniform((nn, nn), 22)
-            t = L.ref_time_dgemm(C.c_int64(nn), C.c_int64(nn), C.c_int64(nn), A.ctypes.data_as(C.c_void_p),
-                                 B.ctypes.data_as(C.c_void_p), C.c_int(3))
-            return {"value": 2.0 * nn ** 3 / t / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "reference",
-                    "sample": f"Blaze C = A * B, DynamicMatrix<double,rowMajor> {nn}^3, uniform[-1,1), "
-                              f"OpenMP {cores} threads, min of 3 after warm-up (blazemark protocol); "
-                              "FP64 (the CPU has no split-precision path)"}
-        nn = 512
-        A = self.uniform((nn, nn), 21); B = self.uniform((nn, nn), 22)
-        t0 = time.perf_counter()
-        oracle.gemm(A, B)
-        t = time.perf_counter() - t0
-        return {"value": 2.0 * nn ** 3 / t / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-                "sample": f"oracle C restatement of Blaze's GEMM, {nn}^3, 1 core"}
+            t0 = time.perf_counter()
+            oracle.gemm(A, B)
+            t = time.perf_counter() - t0
+            return {"value": 2.0 * nn ** 3 / t / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+                    "sample": f"oracle C restatement of Blaze's GEMM, {nn}^3, 1 core"}
+        L = oracle.ref()
+        L.ref_set_threads(C.c_int(cores))
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        proto = f"OpenMP {cores} threads, best of the timed runs after one warm-up (blazemark protocol)"
+        # headline: C = A * B at the headline size, the GEMM leg's uniform[-1,1) operands
+        A, B = getattr(self, "gemm_inputs", (None, None))
+        nn = self.args.size
+        if A is None:
+            A = self.uniform((nn, nn), 1); B = self.uniform((nn, nn), 2)
+        log(f"[cpu] Blaze C = A * B {nn}^3 on {cores} cores")
+        t = L.ref_time_dgemm(C.c_int64(nn), C.c_int64(nn), C.c_int64(nn), ptr(A), ptr(B), C.c_int(2))
+        head = {"value": 2.0 * nn ** 3 / t / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "reference",
+                "sample": f"Blaze C = A * B, DynamicMatrix<double,rowMajor> {nn}^3, uniform[-1,1) (the GEMM leg's "
+                          f"operands), {proto}, 2 timed runs; FP64 (the CPU has no split-precision path)",
+                "ms": t * 1e3}
+        if "value" in rec and rec.get("value"):
+            head["gpu_over_cpu"] = rec["value"] / head["value"]
+        self.gemm_inputs = None
+        # config 1: 512^3 from mt19937_64
+        if getattr(self, "gemm512_inputs", None) is not None and "plain_gemm_512" in rec:
+            A5, B5 = self.gemm512_inputs
+            t5 = L.ref_time_dgemm(C.c_int64(512), C.c_int64(512), C.c_int64(512), ptr(A5), ptr(B5), C.c_int(50))
+            rec["plain_gemm_512"]["cpu_baseline"] = {
+                "value": 2.0 * 512 ** 3 / t5 / 1e12, "unit": "TFLOP/s", "us": t5 * 1e6, "cores": cores,
+                "kind": "reference", "sample": f"Blaze C = A * B 512^3, same mt19937_64 operands, {proto}, 50 timed runs"}
+            rec["plain_gemm_512"]["gpu_over_cpu"] = rec["plain_gemm_512"]["tflops"] / rec["plain_gemm_512"]["cpu_baseline"]["value"]
+        # SpMV: y = S * x on the SpMV leg's matrices (CompressedMatrix<double,rowMajor>)
+        for key, inp in (getattr(self, "spmv_inputs", None) or {}).items():
+            rp, ci, va, x, ncols = inp
+            log(f"[cpu] Blaze y = S * x ({key}) on {cores} cores")
+            ts = L.ref_time_dcsrmv(C.c_int64(len(rp) - 1), C.c_int64(ncols), ptr(rp), ptr(ci), ptr(va), ptr(x),
+                                   C.c_int(10))
+            leg = rec.get("spmv", {}).get(key)
+            if isinstance(leg, dict):
+                leg["cpu_baseline"] = {"value": leg["bytes"] / ts / 1e9, "unit": "GB/s", "us": ts * 1e6, "cores": cores,
+                                       "kind": "reference",
+                                       "sample": f"Blaze CompressedMatrix<double,rowMajor> y = S * x, same matrix and x "
+                                                 f"(algorithmic bytes as the GPU leg), {proto}, 10 timed runs"}
+                leg["gpu_over_cpu"] = leg["graph_gbps"] / leg["cpu_baseline"]["value"]
+        self.spmv_inputs = None
+        # GEMV: y = A * x, 16384^2 FP64
+        gv = rec.get("gemv", {})
+        if isinstance(gv, dict) and "dgemv_16384x16384" in gv:
+            m = n = 16384
+            Ag = self.uniform((m, n), 41); xg = self.uniform(n, 42)
+            log(f"[cpu] Blaze y = A * x {m}x{n} on {cores} cores")
+            tg = L.ref_time_dgemv(C.c_int64(m), C.c_int64(n), ptr(Ag), ptr(xg), C.c_int(5))
+            by = gv["dgemv_16384x16384"]["bytes"]
+            gv["dgemv_16384x16384"]["cpu_baseline"] = {
+                "value": by / tg / 1e9, "unit": "GB/s", "us": tg * 1e6, "cores": cores, "kind": "reference",
+                "sample": f"Blaze DynamicMatrix<double,rowMajor> {m}x{n} y = A * x, uniform[-1,1), {proto}, 5 timed runs"}
+            gv["dgemv_16384x16384"]["gpu_over_cpu"] = gv["dgemv_16384x16384"]["gbps"] / (by / tg / 1e9)
+            del Ag
+        # batched solves: the reference's solve(A, X, B) per system (closed forms n <= 6, LAPACK
+        # dgesv_ above), OpenMP over the systems, LAPACK sequential
+        sv = rec.get("solve", {})
+        inp = getattr(self, "solve_inputs", None)
+        if inp is not None and isinstance(sv, dict) and "fp64_lu" in sv:
+            if oracle.have_ref_lapack():
+                n, r, a_off, b_off, As, Bs = inp
+                log(f"[cpu] Blaze solve() on {len(n)} systems, {cores} threads")
+                _, ts, fails = oracle.ref_solve_batch(n, r, a_off, b_off, As, Bs, threads=cores, reps=1)
+                sv["cpu_baseline"] = {"value": len(n) / ts, "unit": "systems/s", "ms": ts * 1e3, "cores": cores,
+                                      "kind": "reference", "fails": fails,
+                                      "sample": f"all {len(n)} systems of the GPU leg: Blaze solve(A, X, B) per system "
+                                                "(closed forms n <= 6, dgesv_ from MKL's libmkl_rt, sequential, "
+                                                f"n >= 7) under an OpenMP loop over the systems ({cores} threads), one "
+                                                "warm-up pass then one timed pass",
+                                      "gflops": sv["flops_getrf_getrs"] / ts / 1e9}
+                for key in ("fp64_lu", "mixed_fp32_lu_fp64_ir"):
+                    if key in sv:
+                        sv[key]["gpu_over_cpu"] = sv[key]["systems_per_s"] / sv["cpu_baseline"]["value"]
+            else:
+                sv["cpu_baseline"] = {"error": "the reference's solve() needs a LAPACK; none was found when "
+                                               "oracle/_ref was built"}
+        self.solve_inputs = None
+        return head
 
 
 def gemm_leg(b, rec, args):
@@ -600,12 +739,25 @@ def gemm_leg(b, rec, args):
                                      else "BF16 x BF16 -> FP32 residues mod m<=511"),
                      "recombination": ("FP64 CRT, weights split in " + ("40" if head.endswith("int8") else "39")
                                        + "-bit chunks (exact partial sums), one final rounding"),
-                     "moduli_choice": "automatic: smallest count with estimated rel. Frobenius error <= 8e-15",
-                     "rel_frob_err": res[head]["rel_frob_err"]}
+                     "moduli_choice": "automatic: smallest count with estimated rel. Frobenius error <= 8e-15 (3.2e-15 when single entries dominate rows/columns)",
+                     "rel_frob_err": res[head]["rel_frob_err"], "err_sample": res[head]["err_sample"]}
     rec["native_fp64"] = res["native_fp64"]
     rec["fp64acc_vs_native"] = res[head]["vs_native"]
     rec["fp64acc_8192"] = {v: res[v] for v, _ in variants}
     rec["roofline"] = b.roofline_fp64acc(m, n, k, A, B, fn, slices)
+    b.gemm_inputs = (A, B)  # the CPU baseline times the reference on the same operands
+    # the same product for graded exponents u 2^e, e = round(N(0, sigma)): rate of the automatic
+    # moduli count and its error over whole rows (INT8 slices, the headline variant)
+    by_sigma = {}
+    for sigma in (4, 10):
+        log(f"[gemm] sigma={sigma}: generating {m}x{k} graded inputs")
+        Ag = b.graded((m, k), 130 + sigma, sigma); Bg = b.graded((k, n), 140 + sigma, sigma)
+        r_s = b.gemm_case(m, n, k, Ag, Bg, max(3, args.steps // 2), 1, [("fp64acc_int8", "bzg_dgemm_fp64acc_i8")])
+        by_sigma[f"sigma{sigma}"] = r_s
+        log(f"[gemm] sigma={sigma}: {json.dumps(r_s)}")
+        del Ag, Bg
+    by_sigma["sigma0"] = {"native_fp64": res["native_fp64"], "fp64acc_int8": res["fp64acc_int8"]}
+    rec[f"fp64acc_{m}_by_sigma"] = by_sigma
     # error per exponent spread (smaller size keeps the run short; K is what matters)
     # error per exponent spread: the full relative Frobenius error of a 512 x 512 x k product
     # (for wide spreads a few entries carry most of ||C||: a sampled ratio would mislead)
@@ -651,6 +803,7 @@ def gemm_leg(b, rec, args):
         rec["fp64acc_16384x1024x16384"] = b.gemm_case(mr, nr_, kr, Ar, Br, max(3, args.steps // 2), 1, variants)
         del Ar, Br
     del A, B
+    rec["plain_gemm_512"] = b.gemm512_leg()
 
 
 def main():
@@ -712,7 +865,7 @@ def main():
         except Exception as e:
             rec["serve"] = {"error": repr(e)}
     if "cpu" in legs:
-        rec["cpu_baseline"] = b.cpu_baseline()
+        rec["cpu_baseline"] = b.cpu_baseline(rec)
     if "copy" in legs:
         # counter calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE: one 1 GiB device copy
         src = b.torch.ones(1 << 27, dtype=b.torch.float64, device="cuda")
