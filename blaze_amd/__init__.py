@@ -66,6 +66,7 @@ SIGNATURES = {
     "bzg_icsrmv": (i32, [vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dcg": (i32, [vp, vp, i64, vp, vp, vp, vp, vp, i32, vp]),
     "bzg_dgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bzg_dgesv_batched_noalloc": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrf_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrs_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_ddet_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]),

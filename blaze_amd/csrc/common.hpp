@@ -84,6 +84,9 @@ struct ProfScope {
   }
 };
 
+// Resolves the device address of the no-allocation solver's scratch array on `device` (current).
+bzg_status noalloc_scratch_init(int device);
+
 // On-chip mixed-precision batched solver (lu_mixed.hip): stable partition of a batch into
 // order classes, and the launches of the class kernels.
 bzg_status launch_partition(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,

@@ -55,6 +55,13 @@ bzg_status bzg_create(bzg_context** out) {
     return BZG_ERR_NO_DEVICE;
   }
   ctx->num_cus = prop.multiProcessorCount;
+  {
+    const bzg_status st = bzg::noalloc_scratch_init(ctx->device);
+    if (st) {
+      delete ctx;
+      return st;
+    }
+  }
   BZG_HIP_CHECK(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
   ctx->stream = ctx->own_stream;
   *out = ctx;

@@ -23,6 +23,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -796,6 +797,45 @@ __global__ __launch_bounds__(256) void classify_kernel(int count, const int32_t*
 
 }  // namespace
 
+// Scratch of bzg_dgesv_batched_noalloc: the solver's counters and the list of systems of order
+// > NSMALL, in a device array of fixed size that every call on one GPU shares.  Host code never
+// passes the array's name to a kernel (in host code it is only a stand-in for the symbol): the
+// device address comes from hipGetSymbolAddress, once per device, kept in g_noalloc_ptr.
+constexpr int64_t kNoAllocMaxSystems = BZG_NOALLOC_MAX_SYSTEMS;
+constexpr size_t kNoAllocCounterBytes = 256;
+constexpr size_t kNoAllocBytes = kNoAllocCounterBytes + (size_t)kNoAllocMaxSystems * sizeof(int32_t);
+constexpr int kMaxDevices = 64;
+
+}  // namespace bzg
+
+static __device__ __align__(256) unsigned char scratch[bzg::kNoAllocBytes];
+
+namespace bzg {
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kMaxDevices] = {};
+
+// Resolves this device's copy of `scratch` (the caller has made `device` current).  Called by
+// bzg_create; idempotent per device.
+bzg_status noalloc_scratch_init(int device) {
+  BZG_REQUIRE(device >= 0 && device < kMaxDevices, BZG_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return BZG_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return BZG_ERR_HIP;
+  }
+  g_noalloc_ptr[device] = ptr;
+  return BZG_OK;
+}
+
+static void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  return g_noalloc_ptr[device];
+}
+
 // Occupancy of the persistent grids (workgroups per CU); bzg_solver_autotune picks them.
 struct SolverTuning {
   int wg_per_cu_fp64 = 8;
@@ -1104,6 +1144,27 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n, 
   BZG_REQUIRE(ipiv == nullptr || p_off != nullptr, BZG_ERR_INVALID_ARGUMENT,
               "dgesv_batched: ipiv without p_off");
   return bzg::lu_batched(ctx, 0, count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order);
+}
+
+bzg_status bzg_dgesv_batched_noalloc(bzg_context* ctx, int64_t count, const int32_t* n, const int32_t* nrhs,
+                                     const int64_t* a_off, const int64_t* b_off, const int64_t* p_off,
+                                     double* A, double* B, int32_t* ipiv, int32_t* info, const int32_t* order) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "dgesv_batched_noalloc: null context");
+  BZG_REQUIRE(count >= 0, BZG_ERR_INVALID_ARGUMENT, "dgesv_batched_noalloc: bad count");
+  BZG_REQUIRE(count <= bzg::kNoAllocMaxSystems, BZG_ERR_UNSUPPORTED,
+              "dgesv_batched_noalloc: more than BZG_NOALLOC_MAX_SYSTEMS (1048576) systems in one call; "
+              "split the batch or use bzg_dgesv_batched");
+  if (count == 0) return BZG_OK;
+  BZG_REQUIRE(n && nrhs && a_off && b_off && A && B && info, BZG_ERR_INVALID_ARGUMENT,
+              "dgesv_batched_noalloc: null array");
+  BZG_REQUIRE(ipiv == nullptr || p_off != nullptr, BZG_ERR_INVALID_ARGUMENT,
+              "dgesv_batched_noalloc: ipiv without p_off");
+  char* base = static_cast<char*>(bzg::noalloc_scratch(ctx->device));
+  BZG_REQUIRE(base != nullptr, BZG_ERR_HIP, "dgesv_batched_noalloc: scratch not resolved on this device");
+  int* counters = reinterpret_cast<int*>(base);
+  int32_t* big = reinterpret_cast<int32_t*>(base + bzg::kNoAllocCounterBytes);
+  return bzg::lu_batched_scratch(ctx, 0, count, n, nrhs, a_off, b_off, p_off, A, B, ipiv, info, order,
+                                 counters, big);
 }
 
 bzg_status bzg_dgetrf_batched(bzg_context* ctx, int64_t count, const int32_t* n, const int64_t* a_off,

@@ -299,6 +299,21 @@ bzg_status bzg_dgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
                              const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
                              const int64_t* p_off, double* A, double* B, int32_t* ipiv,
                              int32_t* info, const int32_t* order);
+/* No reference counterpart: bzg_dgesv_batched for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results (bit for bit) as bzg_dgesv_batched; it calls no hipMalloc,
+ * hipFree or allocating routine and does not synchronise.  Its scratch (work counters and the
+ * list of systems of order > 256: 256 B + 4 B per system) is a fixed device array of
+ * 256 B + 4 * BZG_NOALLOC_MAX_SYSTEMS bytes (about 4 MiB) that the library holds on every GPU it
+ * runs on until the process ends; count > BZG_NOALLOC_MAX_SYSTEMS returns BZG_ERR_UNSUPPORTED.
+ * Every call on one GPU shares that array (counters are cleared by a kernel at the start of each
+ * call), so calls that overlap on one GPU (several streams or host threads) race: serialise
+ * them, e.g. on one stream. */
+#define BZG_NOALLOC_MAX_SYSTEMS 1048576
+bzg_status bzg_dgesv_batched_noalloc(bzg_context* ctx, int64_t count, const int32_t* n,
+                                     const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
+                                     const int64_t* p_off, double* A, double* B, int32_t* ipiv,
+                                     int32_t* info, const int32_t* order);
 /* Factorisation only (LAPACK dgetrf per system, replaces getrf, clapack/getrf.h:190-197):
  * A_i := L U in place (unit-lower L below the diagonal), ipiv_i 1-based, info as above.
  * The factors and pivots are exactly those bzg_dgesv_batched computes for the same A_i. */
