@@ -81,6 +81,7 @@ struct Shm {
   static constexpr int NT = Cfg<NM>::NT, W = Cfg<NM>::W;
   union {
     struct {
+      float prw[16];           // pivot row being broadcast (panel factorisation)
       float pbuf[2][NM][20];   // factored panel p (rows at final positions), by panel parity
       float linv[2][16][20];   // L11^-1 of panel p, by parity
       float stg[W][32][16 * Cfg<NM>::NTC + 1];  // per-wavefront interchange staging: 32 row slots
@@ -191,10 +192,11 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) {
     // candidates: logical rows jj .. m-1, ranked by v = |a| bits + 2 (1 for NaN, 0 for none):
-    // the wavefront maximum of v, then the first logical row holding it (ties are rare: a
-    // second reduction only when several lanes hold the maximum)
-    unsigned vb = 0u;
-    int lb = 1 << 20;
+    // per lane the best of its rows by the 64-bit key (v, -row), so the first logical row of
+    // maximal v wins; then the wavefront maximum of v and the lane holding it (a second
+    // reduction only when several lanes hold the maximum)
+    unsigned long long kb = 0ull;
+    int bq = 0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       if (q >= qm) break;
@@ -203,43 +205,43 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
       unsigned hi = (__float_as_uint(a) & 0x7FFFFFFFu) + 2u;
       hi = (a != a) ? 1u : hi;
       hi = cand ? hi : 0u;
-      const bool better = hi > vb || (hi == vb && lpos[q] < lb);
-      lb = better ? lpos[q] : lb;
-      vb = better ? hi : vb;
+      const unsigned long long key = ((unsigned long long)hi << 32) | (unsigned)(~lpos[q]);
+      const bool better = key > kb;
+      kb = better ? key : kb;
+      bq = better ? q : bq;
     }
+    const unsigned vb = (unsigned)(kb >> 32);
+    const int lb = (int)~(unsigned)kb;
     const unsigned vmax = __ockl_wfred_max_u32(vb);
-    const unsigned long long tie = __ballot(vb == vmax);
+    unsigned long long hold = __ballot(vb == vmax);
     int pr;
-    if (__popcll(tie) == 1) {
-      pr = __builtin_amdgcn_readlane(lb, __ffsll((long long)tie) - 1);
+    if (__popcll(hold) == 1) {
+      pr = __builtin_amdgcn_readlane(lb, __ffsll((long long)hold) - 1);
     } else {
       pr = (int)__ockl_wfred_min_u32(vb == vmax ? (unsigned)lb : 0xFFFFFFFFu);
+      pr = __builtin_amdgcn_readfirstlane(pr);
+      hold = __ballot(vb == vmax && lb == pr);
     }
     pr = __builtin_amdgcn_readfirstlane(pr);
-    bool mine = false;
-    int myq = 0;
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-      if (lpos[q] == pr) { mine = true; myq = q; }
-    const unsigned long long hold = __ballot(mine);
     const int pl = __builtin_amdgcn_readfirstlane(__ffsll((long long)hold) - 1);
-    const int pq = __builtin_amdgcn_readlane(myq, pl);
-    float prow[16];  // the pivot row from column jj on
+    const int pq = __builtin_amdgcn_readlane(bq, pl);
+    // the pivot row from column jj on, read from its lane into scalar registers (a uniform
+    // branch selects the register row that holds it)
+    float prow[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-      if (c >= jj) {
-        float cand = P[0][c];
+    for (int q = 0; q < Q; ++q) {
+      if (pq == q) {
 #pragma unroll
-        for (int q = 1; q < Q; ++q) cand = (pq == q) ? P[q][c] : cand;
-        prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cand), pl));
+        for (int c = jj; c < 16; ++c)
+          prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P[q][c]), pl));
       }
     }
-    const float piv = prow[jj];
+    const float piv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(prow[jj])));
     if (piv != 0.0f) {
       if (lane == 0) sh.piv[par][jj] = j0 + pr;
 #pragma unroll
       for (int q = 0; q < Q; ++q) lpos[q] = (lpos[q] == jj) ? pr : ((lpos[q] == pr) ? jj : lpos[q]);
-      const float rinv = 1.0f / piv;
+      const float rinv = __builtin_amdgcn_rcpf(piv);  // reciprocal, then scale (sgetf2)
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         if (q >= qm) break;
