@@ -68,6 +68,11 @@ SIGNATURES = {
     "bzg_dgesv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgesv_batched_noalloc": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrf_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
+    "bzg_dgetrf": (i32, [vp, i64, vp, i64, vp, vp]),
+    "bzg_dgetrs": (i32, [vp, i64, i64, vp, i64, vp, vp, i64]),
+    "bzg_dgesv": (i32, [vp, i64, i64, vp, i64, vp, vp, i64, vp]),
+    "bzg_dgetri": (i32, [vp, i64, vp, i64, vp]),
+    "bzg_ddet_lu": (i32, [vp, i64, vp, i64, vp, vp, vp]),
     "bzg_dgetrs_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_ddet_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dinv_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),

@@ -36,7 +36,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["context.hip", "gemm.hip", "gemv.hip", "spmv.hip", "cg.hip", "lu.hip", "lu_mixed.hip", "ozaki.hip",
+HIP_SOURCES = ["context.hip", "gemm.hip", "gemv.hip", "spmv.hip", "cg.hip", "lu.hip", "lu_mixed.hip", "getrf.hip", "ozaki.hip",
                "server.hip", "mmio.hip"]
 
 

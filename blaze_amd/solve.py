@@ -201,3 +201,68 @@ def getrf_batched_device(ctx: bz.Context, dA, n: int):
     ctx.call("bzg_dgetrf_batched", 1, _p(dn), _p(dz), _p(dz), _p(dA), _p(dpiv), _p(dinfo), C.c_void_p(0))
     ctx.sync()
     return dA.cpu().numpy().reshape(n, n), dpiv.cpu().numpy(), dinfo.cpu().numpy()[0]
+
+
+# ---- single system of any order (bzg_dgetrf / bzg_dgesv / bzg_dgetri / bzg_ddet_lu) ---------
+def _pad(torch, A, lda):
+    """Device copy of the row-major A with leading dimension lda (>= n columns)."""
+    n, m = A.shape
+    buf = np.zeros((n, lda))
+    buf[:, :m] = A
+    return _dev(torch, buf)
+
+
+def getrf(ctx: bz.Context, A, lda: int | None = None):
+    """(LU, ipiv 1-based, info) of the n x n A through bzg_dgetrf (factors of P A = L U)."""
+    import torch
+    n = A.shape[0]
+    lda = lda or max(n, 1)
+    dA = _pad(torch, np.ascontiguousarray(A, np.float64), lda)
+    dp = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    di = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.call("bzg_dgetrf", n, _p(dA), lda, _p(dp), _p(di))
+    torch.cuda.synchronize()
+    return dA.cpu().numpy()[:, :n].copy(), dp.cpu().numpy()[:n].copy(), int(di.cpu()[0])
+
+
+def gesv(ctx: bz.Context, A, B):
+    """(X, LU, ipiv, info) of A X = B through bzg_dgesv (B: n x nrhs)."""
+    import torch
+    n, r = B.shape
+    dA = _dev(torch, np.ascontiguousarray(A, np.float64))
+    dB = _dev(torch, np.ascontiguousarray(B, np.float64))
+    dp = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    di = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.call("bzg_dgesv", n, r, _p(dA), max(n, 1), _p(dp), _p(dB), max(r, 1), _p(di))
+    torch.cuda.synchronize()
+    return dB.cpu().numpy(), dA.cpu().numpy(), dp.cpu().numpy()[:n].copy(), int(di.cpu()[0])
+
+
+def inv(ctx: bz.Context, A):
+    """(A^-1, info) through bzg_dgetrf + bzg_dgetri (A^-1 only when info == 0)."""
+    import torch
+    n = A.shape[0]
+    dA = _dev(torch, np.ascontiguousarray(A, np.float64))
+    dp = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    di = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.call("bzg_dgetrf", n, _p(dA), max(n, 1), _p(dp), _p(di))
+    torch.cuda.synchronize()
+    info = int(di.cpu()[0])
+    if info == 0:
+        ctx.call("bzg_dgetri", n, _p(dA), max(n, 1), _p(dp))
+        torch.cuda.synchronize()
+    return dA.cpu().numpy(), info
+
+
+def det(ctx: bz.Context, A):
+    """(det A, info) through bzg_dgetrf + bzg_ddet_lu."""
+    import torch
+    n = A.shape[0]
+    dA = _dev(torch, np.ascontiguousarray(A, np.float64))
+    dp = torch.zeros(max(n, 1), dtype=torch.int32, device="cuda")
+    di = torch.zeros(1, dtype=torch.int32, device="cuda")
+    dd = torch.zeros(1, dtype=torch.float64, device="cuda")
+    ctx.call("bzg_dgetrf", n, _p(dA), max(n, 1), _p(dp), _p(di))
+    ctx.call("bzg_ddet_lu", n, _p(dA), max(n, 1), _p(dp), _p(di), _p(dd))
+    torch.cuda.synchronize()
+    return float(dd.cpu()[0]), int(di.cpu()[0])

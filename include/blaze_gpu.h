@@ -31,6 +31,10 @@
  *   bzg_ddet_batched                   <- det() (blaze/math/expressions/DMatDetExpr.h:384-424)
  *   bzg_dinv_batched                   <- invert() / inv() by LU (blaze/math/dense/Inversion.h:3337-3347)
  *   bzg_dsgesv_batched                 <- same call site; LAPACK dsgesv semantics (mixed precision)
+ *   bzg_dgesv / bzg_dgetrf / bzg_dgetrs <- the same LAPACK calls for one system of any order
+ *        (clapack/gesv.h:207, getrf.h:190-197, getrs.h:207-215; solveNxN, LSE.h:5229-5257)
+ *   bzg_ddet_lu / bzg_dgetri            <- det() (DMatDetExpr.h:324-357) and invert() by LU
+ *        (Inversion.h:3337-3347) from those factors
  *   bzg_dgemm_fp64acc[_i8]             <- no reference function: FP64-accurate GEMM from BF16
  *        (INT8) residue slices; drop-in for bzg_dgemm at the same call site
  *        (DMatDMatMultExpr.h:516)
@@ -376,6 +380,33 @@ bzg_status bzg_solver_autotune(bzg_context* ctx, int32_t mixed, int64_t count, c
  * entries) the system indices sorted by descending n (stable), computed from n_host. */
 bzg_status bzg_batched_order(bzg_context* ctx, int64_t count, const int32_t* n_host,
                              int32_t* order_dev);
+
+/* ---- single-system dense LU (any order up to 32768, any number of right-hand sides) -------
+ * A is a row-major n x n DEVICE matrix (leading dimension lda), B row-major n x nrhs (ldb),
+ * ipiv (n entries) and info (one entry) DEVICE arrays.  P A = L U as LAPACK dgetrf computes it
+ * for A: ipiv 1-based in interchange order, info = 0 or the 1-based column of the first exactly
+ * zero pivot (the factorisation completes).  The products inside run on bzg_dgemm; workspace
+ * (n ints + n * nrhs doubles for the solves) is cached in the context.  Orders above 32768 return
+ * BZG_ERR_UNSUPPORTED. */
+/* dgetrf: A := L \ U in place (replaces getrf, blaze/math/lapack/clapack/getrf.h:190-197 and
+ * the factorisation under lu(), blaze/math/dense/LU.h:98-140). */
+bzg_status bzg_dgetrf(bzg_context* ctx, int64_t n, double* A, int64_t lda, int32_t* ipiv, int32_t* info);
+/* dgetrs ('N'): B := A^-1 B from bzg_dgetrf's factors (replaces getrs, clapack/getrs.h:207-215). */
+bzg_status bzg_dgetrs(bzg_context* ctx, int64_t n, int64_t nrhs, const double* A, int64_t lda,
+                      const int32_t* ipiv, double* B, int64_t ldb);
+/* dgesv = dgetrf + dgetrs (replaces solve() for one system: blaze/math/dense/LSE.h:6343-6373 ->
+ * solveNxN LSE.h:5229-5257 -> dgesv_, clapack/gesv.h:207).  B is left unchanged when info > 0;
+ * reading info back synchronises the context stream once. */
+bzg_status bzg_dgesv(bzg_context* ctx, int64_t n, int64_t nrhs, double* A, int64_t lda, int32_t* ipiv,
+                     double* B, int64_t ldb, int32_t* info);
+/* A := A^-1 from bzg_dgetrf's factors (info must have been 0): the inverse by LU of invert()
+ * (blaze/math/dense/Inversion.h:3337-3347), as the solution of A X = I. */
+bzg_status bzg_dgetri(bzg_context* ctx, int64_t n, double* A, int64_t lda, const int32_t* ipiv);
+/* *det = det(A) from bzg_dgetrf's factors by detNxN's rule (blaze/math/expressions/
+ * DMatDetExpr.h:324-357): 0 when info > 0, else the product, left to right from 1, of U's
+ * diagonal with the sign flipped at every interchange (det is a DEVICE pointer). */
+bzg_status bzg_ddet_lu(bzg_context* ctx, int64_t n, const double* A, int64_t lda, const int32_t* ipiv,
+                       const int32_t* info, double* det);
 
 /* ---- FP64-accurate GEMM from BF16 residue slices (Ozaki scheme II on MFMA) --------------
  * Same contract as bzg_dgemm with alpha = 1, beta = 0.  `moduli` selects the number of
