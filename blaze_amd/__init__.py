@@ -69,6 +69,9 @@ SIGNATURES = {
     "bzg_dgesv_batched_noalloc": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrf_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp]),
     "bzg_dgetrf": (i32, [vp, i64, vp, i64, vp, vp]),
+    "bzg_daxpy": (i32, [vp, i64, C.c_double, vp, vp]),
+    "bzg_saxpy": (i32, [vp, i64, C.c_float, vp, vp]),
+    "bzg_iaxpy": (i32, [vp, i64, i32, vp, vp]),
     "bzg_dgetrs": (i32, [vp, i64, i64, vp, i64, vp, vp, i64]),
     "bzg_dgesv": (i32, [vp, i64, i64, vp, i64, vp, vp, i64, vp]),
     "bzg_dgetri": (i32, [vp, i64, vp, i64, vp]),

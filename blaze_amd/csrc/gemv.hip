@@ -10,6 +10,8 @@
 // run to run.  One load of x is reused for the R rows, which cuts L2 traffic for x by R.
 #include "common.hpp"
 
+#include <type_traits>
+
 #include <algorithm>
 
 namespace bzg {
@@ -341,6 +343,33 @@ __global__ __launch_bounds__(256) void transpose_kernel(int64_t m, int64_t n, co
 }  // namespace
 }  // namespace bzg
 
+namespace bzg {
+namespace {
+// y += alpha x, element by element (integers wrap modulo 2^32, as the INT32 products do)
+template <typename T>
+__global__ __launch_bounds__(256) void axpy_kernel(long n, T alpha, const T* __restrict__ x, T* __restrict__ y) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    if constexpr (std::is_integral<T>::value)
+      y[i] = (T)((uint32_t)y[i] + (uint32_t)alpha * (uint32_t)x[i]);
+    else
+      y[i] = y[i] + alpha * x[i];
+  }
+}
+
+template <typename T>
+bzg_status axpy(bzg_context* ctx, int64_t n, T alpha, const T* x, T* y) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "axpy: null context");
+  BZG_REQUIRE(n >= 0, BZG_ERR_INVALID_ARGUMENT, "axpy: negative size");
+  if (n == 0) return BZG_OK;
+  BZG_REQUIRE(x && y, BZG_ERR_INVALID_ARGUMENT, "axpy: null pointer");
+  const long blocks = std::min<long>((n + 255) / 256, (long)ctx->num_cus * 8);
+  hipLaunchKernelGGL(axpy_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (long)n, alpha, x, y);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
+}
+}  // namespace
+}  // namespace bzg
+
 extern "C" {
 
 bzg_status bzg_dtranspose(bzg_context* ctx, int64_t m, int64_t n, const double* A, int64_t lda, double* B,
@@ -383,6 +412,16 @@ bzg_status bzg_sgemv(bzg_context* ctx, int64_t m, int64_t n, float alpha, const 
 bzg_status bzg_igemv(bzg_context* ctx, int64_t m, int64_t n, int32_t alpha, const int32_t* A,
                      int64_t lda, const int32_t* x, int32_t beta, int32_t* y) {
   return bzg::gemv_dispatch<int32_t>(ctx, m, n, alpha, A, lda, x, beta, y);
+}
+
+bzg_status bzg_daxpy(bzg_context* ctx, int64_t n, double alpha, const double* x, double* y) {
+  return bzg::axpy<double>(ctx, n, alpha, x, y);
+}
+bzg_status bzg_saxpy(bzg_context* ctx, int64_t n, float alpha, const float* x, float* y) {
+  return bzg::axpy<float>(ctx, n, alpha, x, y);
+}
+bzg_status bzg_iaxpy(bzg_context* ctx, int64_t n, int32_t alpha, const int32_t* x, int32_t* y) {
+  return bzg::axpy<int32_t>(ctx, n, alpha, x, y);
 }
 
 }  // extern "C"

@@ -21,8 +21,10 @@
 // Storage: row-major containers, and column-major ones stored as their row-major transposes
 // (which is the column-major layout); elements live in HBM and are reached from the host
 // through explicit copies (upload / download / operator() reads).  All work is
-// issued on the per-thread default context's stream and completes before an assignment
-// returns (a download, read or the end of the assignment synchronises).
+// issued in order on the per-thread default context's stream; an assignment returns once its
+// kernels are enqueued, and every host read (download, toVector, operator(), the info of a
+// solve) is a stream-ordered copy that waits for them.  Code that hands data() to work on
+// another stream calls synchronize() first.
 //
 // The element types follow the reference's tests: double and float (tolerance parity) and
 // int (bit-exact parity).  FP64 products can be routed to the FP64-accurate split GEMM with
@@ -114,6 +116,9 @@ inline bzg_context* context() {
   return holder.ctx;
 }
 
+// Waits for the work enqueued on this thread's context.
+inline void synchronize() { detail::check(bzg_synchronize(context()), "synchronize"); }
+
 // Route FP64 C = A * B through the FP64-accurate residue-slice GEMM (this thread).
 inline void setFp64AccurateGemm(bool on) { detail::fp64_accurate_flag() = on; }
 inline bool fp64AccurateGemm() { return detail::fp64_accurate_flag(); }
@@ -162,6 +167,10 @@ inline bzg_status gemv(int64_t m, int64_t n, int al, const int* A, int64_t lda, 
   if (ta) return bzg_igemv_t(context(), m, n, al, A, lda, x, be, y);
   return bzg_igemv(context(), m, n, al, A, lda, x, be, y);
 }
+// y += alpha x over n elements (the aliased accumulations' element-wise add)
+inline bzg_status axpy(size_t n, double al, const double* x, double* y) { return bzg_daxpy(context(), n, al, x, y); }
+inline bzg_status axpy(size_t n, float al, const float* x, float* y) { return bzg_saxpy(context(), n, al, x, y); }
+inline bzg_status axpy(size_t n, int al, const int* x, int* y) { return bzg_iaxpy(context(), n, al, x, y); }
 inline bzg_status csrmv(const bzg_csr_plan* p, const int32_t* rp, const int32_t* ci,
                         const double* v, const double* x, double* y) {
   return bzg_dcsrmv(context(), p, rp, ci, v, x, y);
@@ -314,7 +323,6 @@ class DynamicMatrix {  // row-major (the column-major form is the specialisation
   }
   void reset() {
     if (m_ * n_) detail::check(bzg_memset_zero(context(), buf_.ptr, m_ * n_ * sizeof(T)), "reset");
-    detail::check(bzg_synchronize(context()), "sync");
   }
   void swap(DynamicMatrix& o) noexcept {
     std::swap(m_, o.m_);
@@ -371,18 +379,15 @@ class DynamicMatrix {  // row-major (the column-major form is the specialisation
     detail::check(detail::gemm(m, n, k, e.scale, e.lhs.data(), std::max<size_t>(e.lhs.columns(), 1), e.rhs.data(),
                                std::max<size_t>(e.rhs.columns(), 1), T(0), buf_.ptr, n, e.ta, e.tb),
                   "C = A * B");
-    detail::check(bzg_synchronize(context()), "sync");
   }
   DynamicMatrix& accumulate(const DMatDMatMultExpr<T>& e, T sign) {
     if (e.rows() != m_ || e.columns() != n_)
       throw std::invalid_argument("Matrix sizes do not match");
     if (e.lhs.isAliased(this) || e.rhs.isAliased(this)) {
+      // the product into a temporary, then added element-wise on the device (the reference's
+      // DynamicMatrix.h:1670-1672)
       DynamicMatrix tmp(e);
-      const size_t mn = m_ * n_;
-      // this += sign * tmp  ==  gemm(I, tmp) would waste work; use GEMM with K = n: C = C + s*T*I
-      std::vector<T> a = toVector(), b = tmp.toVector();
-      for (size_t q = 0; q < mn; ++q) a[q] += sign * b[q];
-      upload(a.data());
+      detail::check(detail::axpy(m_ * n_, sign, tmp.data(), buf_.ptr), "C += A * C");
       return *this;
     }
     const size_t k = e.inner();
@@ -391,7 +396,6 @@ class DynamicMatrix {  // row-major (the column-major form is the specialisation
                                  e.rhs.data(), std::max<size_t>(e.rhs.columns(), 1), T(1), buf_.ptr, n_,
                                  e.ta, e.tb),
                     "C += A * B");
-    detail::check(bzg_synchronize(context()), "sync");
     return *this;
   }
 
@@ -504,19 +508,16 @@ class DynamicVector {
   // y += s * (A * x) (DMatDVecMultExpr addAssign / subAssign): one GEMV with beta = 1
   DynamicVector& accumulate(const DMatDVecMultExpr<T>& e, T sign) {
     if (e.size() != n_) throw std::invalid_argument("Vector sizes do not match");
-    if (e.vec.isAliased(this)) {
+    if (e.vec.isAliased(this)) {  // DynamicVector.h:1412-1414: temporary, then the add
       DynamicVector tmp;
       tmp.assign(e);
-      std::vector<T> a = toVector(), b = tmp.toVector();
-      for (size_t i = 0; i < n_; ++i) a[i] += sign * b[i];
-      upload(a.data());
+      detail::check(detail::axpy(n_, sign, tmp.data(), buf_.ptr), "y += A * y");
       return *this;
     }
     if (n_)
       detail::check(detail::gemv(e.mat.rows(), e.mat.columns(), sign * e.scale, e.mat.data(),
                                  std::max<size_t>(e.mat.columns(), 1), e.vec.data(), T(1), buf_.ptr, e.ta),
                     sign == T(1) ? "y += A * x" : "y -= A * x");
-    detail::check(bzg_synchronize(context()), "sync");
     return *this;
   }
   DynamicVector& operator=(const DMatDVecSolveExpr<T>& e);
@@ -528,7 +529,6 @@ class DynamicVector {
       detail::check(detail::gemv(m, n, e.scale, e.mat.data(), std::max<size_t>(n, 1), e.vec.data(), T(0),
                                  buf_.ptr, e.ta),
                     "y = A * x");
-    detail::check(bzg_synchronize(context()), "sync");
   }
   void assign(const SMatDVecMultExpr<T>& e);
   void assign(const TSMatDVecMultExpr<T>& e);
@@ -743,7 +743,6 @@ void DynamicVector<T>::assign(const TSMatDVecMultExpr<T>& e) {
     detail::check(detail::csrmv(e.mat.t_plan(), e.mat.d_t_rowptr(), e.mat.d_t_colidx(), e.mat.d_t_values(),
                                 e.vec.data(), buf_.ptr),
                   "y = trans(S) * x");
-  detail::check(bzg_synchronize(context()), "sync");
 }
 
 template <typename T>
@@ -754,7 +753,6 @@ void DynamicVector<T>::assign(const SMatDVecMultExpr<T>& e) {
     detail::check(detail::csrmv(e.mat.plan(), e.mat.d_rowptr(), e.mat.d_colidx(), e.mat.d_values(),
                                 e.vec.data(), buf_.ptr),
                   "y = S * x");
-  detail::check(bzg_synchronize(context()), "sync");
 }
 
 // ---------------------------------------------------------------------------------------
@@ -834,34 +832,63 @@ DynamicMatrix<T, SO>& DynamicMatrix<T, SO>::operator*=(const DynamicMatrix<T, SO
 // solve (LU with partial pivoting on the GPU; reference: LSE.h:6343-6373, 6452-6482)
 // ---------------------------------------------------------------------------------------
 namespace detail {
-struct SolveArrays {
-  DeviceBuffer<int32_t> n, r, info;
-  DeviceBuffer<int64_t> ao, bo;
+// The batched solver's per-call descriptors for one system (n, nrhs, offsets, info), kept per
+// thread: one upload of 32 bytes and one download of info per solve.
+struct SolveMeta {
+  DeviceBuffer<int64_t> buf;  // [0] n | nrhs << 32, [1] a_off = 0, [2] b_off = 0, [3] info
+  SolveMeta() { buf.allocate(4); }
 };
+inline SolveMeta& solve_meta() {
+  static thread_local SolveMeta m;
+  return m;
+}
+inline void throw_singular(size_t n) {
+  if (n <= 6) throw std::runtime_error("Solving LSE with singular system matrix failed");
+  throw std::runtime_error("Inversion of singular matrix failed");
+}
 // Solves A X = B (B: n x nrhs, row-major, overwritten with X). A is copied (not modified).
+// Orders <= 512 with <= 8 right-hand sides take the batched kernels, everything else the
+// single-system LU (bzg_dgesv).
 inline void solve_into(const DynamicMatrix<double>& A, double* B_dev, size_t nrhs) {
   const size_t n = A.rows();
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
   if (n == 0) return;
-  if (n > 512 || nrhs > 8)
-    throw std::invalid_argument("solve: this library supports n <= 512 and up to 8 right-hand sides");
   DeviceBuffer<double> Acopy(n * n);
   d2d(Acopy.ptr, A.data(), n * n);
-  const int32_t nn = (int32_t)n, rr = (int32_t)nrhs;
-  const int64_t zero = 0;
-  SolveArrays s;
-  s.n.allocate(1); s.r.allocate(1); s.info.allocate(1); s.ao.allocate(1); s.bo.allocate(1);
-  h2d(s.n.ptr, &nn, 1); h2d(s.r.ptr, &rr, 1); h2d(s.ao.ptr, &zero, 1); h2d(s.bo.ptr, &zero, 1);
-  check(bzg_dgesv_batched(context(), 1, s.n.ptr, s.r.ptr, s.ao.ptr, s.bo.ptr, nullptr, Acopy.ptr, B_dev,
-                          nullptr, s.info.ptr, nullptr),
-        "solve");
   int32_t info = 0;
-  d2h(&info, s.info.ptr, 1);
-  if (info > 0) {
-    if (n <= 6) throw std::runtime_error("Solving LSE with singular system matrix failed");
-    throw std::runtime_error("Inversion of singular matrix failed");
+  if (n <= 512 && nrhs <= 8) {
+    SolveMeta& m = solve_meta();
+    int64_t meta[4] = {(int64_t)((uint64_t)n | ((uint64_t)nrhs << 32)), 0, 0, 0};
+    h2d(m.buf.ptr, meta, 4);
+    int32_t* base = reinterpret_cast<int32_t*>(m.buf.ptr);
+    check(bzg_dgesv_batched(context(), 1, base, base + 1, m.buf.ptr + 1, m.buf.ptr + 2, nullptr, Acopy.ptr,
+                            B_dev, nullptr, base + 6, nullptr),
+          "solve");
+    d2h(&info, base + 6, 1);
+  } else {
+    DeviceBuffer<int32_t> ipiv(n), dinfo(1);
+    check(bzg_dgesv(context(), n, nrhs, Acopy.ptr, n, ipiv.ptr, B_dev, nrhs, dinfo.ptr), "solve");
+    d2h(&info, dinfo.ptr, 1);
   }
+  if (info > 0) throw_singular(n);
   if (info < 0) throw std::invalid_argument("solve: unsupported system");
+}
+}  // namespace detail
+
+namespace detail {
+// getrf in place on the row-major n x n matrix at T: the batched kernel for n <= 512, the
+// single-system LU beyond (same pivots: both are LAPACK dgetrf's).  ipiv (n) and info (1) are
+// device arrays.
+inline void getrf_inplace(double* T, size_t n, int32_t* dpiv, int32_t* dinfo, const char* what) {
+  if (n > 512) {
+    check(bzg_dgetrf(context(), n, T, n, dpiv, dinfo), what);
+    return;
+  }
+  SolveMeta& m = solve_meta();
+  int64_t meta[4] = {(int64_t)n, 0, 0, 0};
+  h2d(m.buf.ptr, meta, 4);
+  int32_t* base = reinterpret_cast<int32_t*>(m.buf.ptr);
+  check(bzg_dgetrf_batched(context(), 1, base, m.buf.ptr + 1, m.buf.ptr + 1, T, dpiv, dinfo, nullptr), what);
 }
 }  // namespace detail
 
@@ -894,29 +921,21 @@ inline DMatDVecSolveExpr<double> solve(const DynamicMatrix<double>& A, const Dyn
 // the factorisation), U unit upper triangular, P the column permutation P(i, permut[i]) = 1.
 // Here: bzg_dtranspose + bzg_dgetrf_batched on the device, the triangles and the permutation
 // are assembled from the factors on the host.  Like the reference, it does not fail for a
-// singular A.  Square matrices of order <= 512.
+// singular A.  Square matrices of any order (the batched LU up to 512, bzg_dgetrf beyond).
 // ---------------------------------------------------------------------------------------
 inline void lu(const DynamicMatrix<double>& A, DynamicMatrix<double>& L, DynamicMatrix<double>& U,
                DynamicMatrix<double>& P) {
   const size_t n = A.rows();
   if (A.rows() != A.columns())
     throw std::invalid_argument("lu: this library supports square matrices");
-  if (n > 512) throw std::invalid_argument("lu: this library supports n <= 512");
   L.resize(n, n, false);
   U.resize(n, n, false);
   P.resize(n, n, false);
   if (n == 0) return;
   detail::DeviceBuffer<double> T(n * n);
   detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "lu: transpose");
-  const int32_t nn = (int32_t)n;
-  const int64_t zero = 0;
-  detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
-  detail::DeviceBuffer<int64_t> doff;
-  dn.allocate(1); dpiv.allocate(n); dinfo.allocate(1); doff.allocate(1);
-  detail::h2d(dn.ptr, &nn, 1);
-  detail::h2d(doff.ptr, &zero, 1);
-  detail::check(bzg_dgetrf_batched(context(), 1, dn.ptr, doff.ptr, doff.ptr, T.ptr, dpiv.ptr, dinfo.ptr, nullptr),
-                "lu: getrf");
+  detail::DeviceBuffer<int32_t> dpiv(n), dinfo(1);
+  detail::getrf_inplace(T.ptr, n, dpiv.ptr, dinfo.ptr, "lu: getrf");
   std::vector<double> f(n * n), l(n * n, 0.0), u(n * n, 0.0), pm(n * n, 0.0);
   std::vector<int32_t> ipiv(n);
   detail::d2h(f.data(), T.ptr, n * n);
@@ -947,9 +966,17 @@ inline double det(const DynamicMatrix<double>& A) {
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
   const size_t n = A.rows();
   if (n == 0) return 0.0;
-  if (n > 512) throw std::invalid_argument("det: this library supports n <= 512");
   detail::DeviceBuffer<double> T(n * n);
   detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "det: transpose");
+  if (n > 512) {  // single-system LU, then detNxN's product from its factors
+    detail::DeviceBuffer<int32_t> dpiv(n), dinfo(1);
+    detail::DeviceBuffer<double> dd(1);
+    detail::check(bzg_dgetrf(context(), n, T.ptr, n, dpiv.ptr, dinfo.ptr), "det: getrf");
+    detail::check(bzg_ddet_lu(context(), n, T.ptr, n, dpiv.ptr, dinfo.ptr, dd.ptr), "det");
+    double d = 0.0;
+    detail::d2h(&d, dd.ptr, 1);
+    return d;
+  }
   const int32_t nn = (int32_t)n;
   const int64_t zero = 0;
   detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
@@ -974,9 +1001,18 @@ inline void invert(DynamicMatrix<double>& A) {
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
   const size_t n = A.rows();
   if (n == 0) return;
-  if (n > 512) throw std::invalid_argument("inv: this library supports n <= 512");
   detail::DeviceBuffer<double> T(n * n);
   detail::check(bzg_dtranspose(context(), n, n, A.data(), n, T.ptr, n), "inv: transpose");
+  if (n > 512) {  // single-system LU, then A^-1 from the factors
+    detail::DeviceBuffer<int32_t> dpiv(n), dinfo(1);
+    detail::check(bzg_dgetrf(context(), n, T.ptr, n, dpiv.ptr, dinfo.ptr), "inv: getrf");
+    int32_t info = 0;
+    detail::d2h(&info, dinfo.ptr, 1);
+    if (info != 0) throw std::runtime_error("Inversion of singular matrix failed");
+    detail::check(bzg_dgetri(context(), n, T.ptr, n, dpiv.ptr), "inv");
+    detail::check(bzg_dtranspose(context(), n, n, T.ptr, n, A.data(), n), "inv: transpose");
+    return;
+  }
   const int32_t nn = (int32_t)n;
   const int64_t zero = 0;
   detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
@@ -1124,21 +1160,13 @@ inline void lu(const DynamicMatrix<double, columnMajor>& A, DynamicMatrix<double
                DynamicMatrix<double, columnMajor>& U, DynamicMatrix<double, columnMajor>& P) {
   const size_t n = A.rows();
   if (A.rows() != A.columns()) throw std::invalid_argument("lu: this library supports square matrices");
-  if (n > 512) throw std::invalid_argument("lu: this library supports n <= 512");
   L.resize(n, n, false);
   U.resize(n, n, false);
   P.resize(n, n, false);
   if (n == 0) return;
   DynamicMatrix<double> R = rowMajorCopy(A);
-  const int32_t nn = (int32_t)n;
-  const int64_t zero = 0;
-  detail::DeviceBuffer<int32_t> dn, dpiv, dinfo;
-  detail::DeviceBuffer<int64_t> doff;
-  dn.allocate(1); dpiv.allocate(n); dinfo.allocate(1); doff.allocate(1);
-  detail::h2d(dn.ptr, &nn, 1);
-  detail::h2d(doff.ptr, &zero, 1);
-  detail::check(bzg_dgetrf_batched(context(), 1, dn.ptr, doff.ptr, doff.ptr, R.data(), dpiv.ptr, dinfo.ptr, nullptr),
-                "lu: getrf");
+  detail::DeviceBuffer<int32_t> dpiv(n), dinfo(1);
+  detail::getrf_inplace(R.data(), n, dpiv.ptr, dinfo.ptr, "lu: getrf");
   std::vector<double> f = R.toVector(), l(n * n, 0.0), u(n * n, 0.0), pm(n * n, 0.0);  // l, u, pm by columns
   std::vector<int32_t> ipiv(n);
   detail::d2h(ipiv.data(), dpiv.ptr, n);

@@ -200,6 +200,14 @@ bzg_status bzg_sgemv(bzg_context* ctx, int64_t m, int64_t n, float alpha, const 
 bzg_status bzg_igemv(bzg_context* ctx, int64_t m, int64_t n, int32_t alpha, const int32_t* A,
                      int64_t lda, const int32_t* x, int32_t beta, int32_t* y);
 
+/* y += alpha * x over n contiguous elements (device pointers; INT32 wraps modulo 2^32).  The
+ * element-wise update of an accumulation whose operand aliases its target: the product is
+ * evaluated into a temporary and added (blaze/math/dense/DynamicMatrix.h:1670-1672,
+ * blaze/math/dense/DynamicVector.h:1412-1414). */
+bzg_status bzg_daxpy(bzg_context* ctx, int64_t n, double alpha, const double* x, double* y);
+bzg_status bzg_saxpy(bzg_context* ctx, int64_t n, float alpha, const float* x, float* y);
+bzg_status bzg_iaxpy(bzg_context* ctx, int64_t n, int32_t alpha, const int32_t* x, int32_t* y);
+
 /* ---- CSR SpMV:  y = A * x  (row-major CompressedMatrix packed as CSR) --------------------
  * rowptr has n_rows+1 int32 entries (rowptr[0] == 0, nnz = rowptr[n_rows] < 2^31); colidx
  * int32 strictly increasing per row.  Every row's sum runs sequentially in storage order

@@ -723,6 +723,82 @@ int main() {
     for (size_t q = 0; q < m * n; ++q) { num += (g1[q] - g2[q]) * (g1[q] - g2[q]); den += g1[q] * g1[q]; }
     check(std::sqrt(num / den) < 1e-14, "FP64-accurate split GEMM agrees with native FP64 (rel. Frobenius < 1e-14)");
   }
+  // orders beyond the batched kernels (n > 512) and more than 8 right-hand sides: solve(), lu(),
+  // det(), inv() through the single-system LU (LSE.h:5229-5257, LU.h:98-140, Inversion.h:3337-3347)
+  {
+    const size_t n = 700, r = 12;
+    std::vector<double> a(n * n), b(n * r);
+    for (auto& v : a) v = U(rng);
+    for (size_t i = 0; i < n; ++i) a[i * n + i] += 30.0;
+    for (auto& v : b) v = U(rng);
+    DynamicMatrix<double> A(n, n, a.data()), B(n, r, b.data()), X, AX;
+    DynamicVector<double> bv(n, b.data()), x, Ax;
+    solve(A, X, B);
+    solve(A, x, bv);
+    AX = A * X;
+    Ax = A * x;
+    auto g = AX.toVector(), gx = Ax.toVector();
+    bool ok = X.rows() == n && X.columns() == r;
+    for (size_t q = 0; q < n * r; ++q) ok = ok && relaxed(g[q], b[q], 1e-12);
+    for (size_t i = 0; i < n; ++i) ok = ok && relaxed(gx[i], b[i], 1e-12);
+    check(ok, "solve(A,X,B) with N = 700 and 12 right-hand sides, solve(A,x,b) with N = 700");
+    const size_t m = 40;
+    std::vector<double> c(m * m), d(m * 20);
+    for (auto& v : c) v = U(rng);
+    for (size_t i = 0; i < m; ++i) c[i * m + i] += 5.0;
+    for (auto& v : d) v = U(rng);
+    DynamicMatrix<double> C(m, m, c.data()), D(m, 20, d.data()), Y, CY;
+    solve(C, Y, D);
+    CY = C * Y;
+    auto h = CY.toVector();
+    ok = true;
+    for (size_t q = 0; q < m * 20; ++q) ok = ok && relaxed(h[q], d[q], 1e-12);
+    check(ok, "solve(A,X,B) with N = 40 and 20 right-hand sides");
+    // det(A) for N = 700 against det(A^T) (a different factorisation: other pivots and
+    // rounding, the same value) and inv(A) * A == I
+    // (G = I + 0.02 U(-1, 1): |det| of order one)
+    std::vector<double> gm(n * n), at(n * n);
+    for (size_t q = 0; q < n * n; ++q) gm[q] = 0.02 * U(rng) + (q % (n + 1) == 0 ? 1.0 : 0.0);
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = 0; j < n; ++j) at[j * n + i] = gm[i * n + j];
+    DynamicMatrix<double> G(n, n, gm.data()), Gt(n, n, at.data());
+    const double d1 = det(G), d2 = det(Gt);
+    check(std::isfinite(d1) && d1 != 0.0 && relaxed(d1, d2, 1e-10), "det(G) for N = 700 (== det(G^T))");
+    DynamicMatrix<double> Ai = inv(G), I1;
+    I1 = Ai * G;
+    auto e = I1.toVector();
+    ok = true;
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = 0; j < n; ++j) ok = ok && std::fabs(e[i * n + j] - (i == j ? 1.0 : 0.0)) <= 1e-12;
+    check(ok, "inv(G) * G == I for N = 700");
+    DynamicMatrix<double> L, Uu, P, LU, LUP;
+    lu(G, L, Uu, P);
+    LU = L * Uu;
+    LUP = LU * P;
+    auto f = LUP.toVector();
+    ok = true;
+    for (size_t q = 0; q < n * n; ++q) ok = ok && relaxed(f[q], gm[q], 1e-12);
+    check(ok, "lu(G,L,U,P), G == L*U*P, N = 700");
+  }
+  // aliased accumulation on the device: C += A * C equals C + (A * C) computed separately
+  {
+    const size_t n = 64;
+    std::vector<double> a(n * n), c(n * n);
+    for (auto& v : a) v = U(rng);
+    for (auto& v : c) v = U(rng);
+    DynamicMatrix<double> A(n, n, a.data()), C(n, n, c.data()), T;
+    T = A * C;
+    C -= A * C;
+    auto t = T.toVector(), g = C.toVector();
+    bool ok = true;
+    for (size_t q = 0; q < n * n; ++q) ok = ok && g[q] == c[q] - t[q];
+    DynamicVector<double> y(n, c.data()), z;
+    z = A * y;
+    y += A * y;
+    auto zz = z.toVector(), yy = y.toVector();
+    for (size_t i = 0; i < n; ++i) ok = ok && yy[i] == c[i] + zz[i];
+    check(ok, "aliased C -= A * C and y += A * y (device add of the evaluated product)");
+  }
   std::printf("%d failure(s)\n", failures);
   return failures;
 }
