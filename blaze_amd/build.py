@@ -3,6 +3,7 @@
 Produces, in-tree:
   blaze_amd/lib/libblaze_gpu.so   HIP kernels + the extern "C" ABI (include/blaze_gpu.h)
   blaze_amd/lib/libbzg_gen.so     host-only synthetic-input generators (gen.c)
+  blaze_amd/lib/libblaze_seam.so  cblas_dgemm / cblas_dgemv / dgesv_ / dgetrf_ / dgetrs_ over the ABI
 
 hipcc cross-compiles for gfx950 without a GPU, so this runs anywhere ROCm is installed.
 Every HIP translation unit is compiled with packed FP32 instructions disabled.
@@ -65,6 +66,13 @@ def build(verbose: bool = True) -> Path:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
           *map(str, objs), "-lpthread"])
     os.replace(tmp, out)
+
+    # the BLAS/LAPACK seam (host code over the C ABI), linked against the library above
+    seam_out = LIB / "libblaze_seam.so"
+    seam_tmp = LIB / "libblaze_seam.so.tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INCLUDE}", str(CSRC / "seam.cpp"), "-o",
+          str(seam_tmp), f"-L{LIB}", "-lblaze_gpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(seam_tmp, seam_out)
 
     gen_out = LIB / "libbzg_gen.so"
     gen_tmp = LIB / "libbzg_gen.so.tmp"
