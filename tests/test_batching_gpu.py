@@ -124,6 +124,37 @@ def test_spmv_autotune_keeps_results():
     assert np.array_equal(y2.cpu().numpy()[short], ref[short])
 
 
+def test_spmv_every_short_row_kernel_is_bit_identical():
+    # LDS row blocks (2048 / 4096 nonzeros, cached / non-temporal loads) and lane groups: every short row is the
+    # oracle's sequential sum bit for bit (SVecDVecInnerExpr.h:127-135), in fp64 and fp32, with
+    # runs of short rows broken by long ones and empty rows in between
+    import torch
+    import blaze_amd as bz
+    from tests.test_gemv_spmv_gpu import Plan
+    rng = np.random.default_rng(83)
+    n = 30000
+    lens = rng.integers(0, 9, n)
+    lens[rng.choice(n, 60, replace=False)] = rng.integers(33, 700, 60)
+    rp = np.zeros(n + 1, np.int32); rp[1:] = np.cumsum(lens)
+    ci = np.concatenate([np.sort(rng.choice(n, int(c), replace=False)) for c in lens]).astype(np.int32)
+    short = lens <= 32
+    for dt, fn in ((np.float64, "bzg_dcsrmv"), (np.float32, "bzg_scsrmv")):
+        va = rng.standard_normal(int(rp[-1])).astype(dt); x = rng.uniform(-1, 1, n).astype(dt)
+        ref = oracle.csrmv(rp, ci, va, x)
+        plan = Plan(rp, n)
+        d = [torch.from_numpy(a).cuda() for a in (rp, ci, va, x)]
+        first = None
+        for cfg in ((0, 1, 0), (0, 2, 0), (0, 1, 1), (4, 1, 4), (1, 2, 8), (16, 1, 2)):
+            bz.check("tune", bz.lib().bzg_csr_plan_tune(plan.h, *cfg))
+            y = torch.full((n,), 7.0, dtype=d[2].dtype, device="cuda")
+            ctx().call(fn, plan.h, *[_p(t) for t in d], _p(y))
+            got = y.cpu().numpy()
+            assert np.array_equal(got[short].view(np.uint8), ref[short].view(np.uint8)), (dt, cfg)
+            if first is None:
+                first = got
+            assert np.array_equal(got.view(np.uint8), first.view(np.uint8)), (dt, cfg)
+
+
 def _solve_batch(torch, rng, sizes, nrhs=2):
     from blaze_amd import solve
     As = [rng.uniform(-1, 1, (n, n)) + n * np.eye(n) for n in sizes]
