@@ -6,18 +6,22 @@
 // dgesv_ at lapack/clapack/gesv.h:207).  Here one call solves many independent systems.
 //
 // Execution model: a persistent grid (a few workgroups per CU) pulls system indices from an
-// atomic work counter in longest-first order, so uneven sizes (n = 1..256) balance without a
-// host round trip.  One workgroup (256 threads) owns one system at a time:
-//   factor: blocked right-looking LU, panel width 16.  The 16-column panel lives in registers
-//           (thread r holds panel row r); each column step is an argmax over the workgroup,
-//           a row swap and a rank-1 update (two barriers).  The swaps of the panel are then
-//           applied to the other columns in one gather (each thread follows the 16 swaps
-//           backwards to find the source row of every touched position), the U12 block row
-//           is solved against the unit-lower L11 in registers, and the trailing matrix gets
-//           A22 -= L21 * U12 with MFMA 16x16x4 tiles (operands from LDS, A22 in place in HBM).
+// atomic work counter in longest-first order, so uneven sizes (n = 1..512) balance without a
+// host round trip.  One workgroup of one wavefront (64 threads) owns one system at a time:
+//   factor: blocked right-looking LU, panel width 8 (FP64) or 16 (FP32).  The panel lives in
+//           the wavefront's registers (NM / 64 rows per lane); each column step is a per-lane
+//           argmax, a butterfly over the wavefront, the pivot row's broadcast and a rank-1
+//           update.  The panel's interchanges are then applied to the other columns in one
+//           gather (each lane follows the swaps backwards to find the source row of every
+//           touched position), the U12 block row is solved against the unit-lower L11, and the
+//           trailing matrix gets A22 -= L21 * U12 with MFMA 16x16x4 tiles (operands from LDS,
+//           A22 in place in HBM).
 //   solve:  P*B is loaded into LDS; forward/backward substitution by 16-row blocks: the
-//           off-diagonal part is a 16-lane-per-row dot product over the factored rows, the
-//           16x16 diagonal block is solved by one thread per right-hand side.
+//           off-diagonal part is a dot product over the factored rows, the 16x16 diagonal block
+//           is solved by one lane per right-hand side.
+// The mixed-precision entry point (bzg_dsgesv_batched) sends orders 1..256 to the on-chip
+// kernels of lu_mixed.hip and keeps the streaming kernel below for orders 257..512; systems
+// that do not converge in FP32 are re-solved by the FP64 kernel (gesv_fallback_kernel).
 // Pivot rule = dgetf2/idamax: the first row of maximal |a| in the column; an exactly-zero pivot
 // sets info (1-based) and the column is left unscaled, as LAPACK does.
 #include "common.hpp"
@@ -836,7 +840,9 @@ static void* noalloc_scratch(int device) {
   return g_noalloc_ptr[device];
 }
 
-// Occupancy of the persistent grids (workgroups per CU); bzg_solver_autotune picks them.
+// Occupancy of the persistent grids (workgroups per CU); bzg_solver_autotune picks them: the
+// FP64 batched LU's, and for the mixed solver the grid of its FP64 fallback kernel (the on-chip
+// class kernels always run at full residency, g_onchip_wg_per_cu).
 struct SolverTuning {
   int wg_per_cu_fp64 = 8;
   int wg_per_cu_mixed = 8;
@@ -1208,7 +1214,7 @@ bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
   // so that they are not regrown as batch sizes vary.
   const int64_t cap = std::max<int64_t>(count, 1);
   const int grid_big_full = ctx->num_cus * 2;
-  const int grid_fb_full = ctx->num_cus * bzg::g_tuning.wg_per_cu_fp64;
+  const int grid_fb_full = ctx->num_cus * bzg::g_tuning.wg_per_cu_mixed;  // FP64 fallback grid
   const int grid_big = (int)std::min<int64_t>(grid_big_full, cap);
   const int grid_fb = (int)std::min<int64_t>(grid_fb_full, cap);
   // scratch: slot 0 counters ([0..3] on-chip class work counters, [4] fallback count, [5] and

@@ -369,14 +369,20 @@ bzg_status bzg_dgetrs_batched(bzg_context* ctx, int64_t count, const int32_t* n,
  * Stopping test per RHS column: ||r||_inf <= ||x||_inf * ||A||_inf * eps * sqrt(n) with
  * eps = tol (pass 0 to use LAPACK's 2^-53).  Refinement also stops (-> FP64) when the
  * largest residual fails to halve on two consecutive steps.  info[i] as in
- * bzg_dgesv_batched.  Scratch (per-workgroup FP32/FP64 matrices) is cached in the context. */
+ * bzg_dgesv_batched.  Orders 1..256 are factored with the FP32 matrix held on chip (one
+ * multi-wavefront workgroup per system, in four order classes), orders 257..512 by a streamed
+ * FP32 kernel; the FP64 fallback runs after them in the same call.  Scratch (class lists,
+ * counters, the streamed kernel's FP32 matrices, the fallback's FP64 matrices) is cached in
+ * the context. */
 bzg_status bzg_dsgesv_batched(bzg_context* ctx, int64_t count, const int32_t* n,
                               const int32_t* nrhs, const int64_t* a_off, const int64_t* b_off,
                               const double* A, const double* B, double* X, int32_t* iter,
                               int32_t* info, double tol, int32_t max_iter, const int32_t* order);
 /* Autotuner of the batched solvers' persistent grids (no reference counterpart): times
- * 1, 2, 3, 4, 6 and 8 workgroups per CU on the given batch (mixed != 0: bzg_dsgesv_batched,
- * else bzg_dgesv_batched) and keeps the fastest for later calls.  A and B are not modified
+ * 1, 2, 3, 4, 6 and 8 workgroups per CU on the given batch and keeps the fastest for later
+ * calls: the grid of bzg_dgesv_batched (mixed == 0), or of bzg_dsgesv_batched's FP64 fallback
+ * kernel (mixed != 0; its on-chip kernels always run at full residency: 12, 6, 2 and 1
+ * workgroups per CU for the orders 1..32, 33..64, 65..128, 129..256).  A and B are not modified
  * (the FP64 solver runs on copies; a_elems / b_elems are the pool sizes in doubles).  Results
  * do not depend on the grid.  Writes the choice and its time in ms (either may be NULL).
  * Synchronises the stream. */
