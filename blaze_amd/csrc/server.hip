@@ -102,6 +102,7 @@ struct Request {
   Group* group = nullptr;
   double t_submit = 0, t_admit = 0, t_done = 0;
   bool done = false;
+  bool direct = false;        // solve of order > 256: alone, through the single-system FP64 LU
 };
 
 struct Csr {
@@ -189,10 +190,23 @@ struct SolveLayout {
   }
 };
 
+// Byte layout of a direct solve's device pool: a copy of A (factored in place), X, the pivots
+// and info.
+struct DirectLayout {
+  size_t off_X, off_piv, off_info, total;
+  DirectLayout(int64_t n, int64_t k) {
+    off_X = ((size_t)n * n * 8 + 255) / 256 * 256;
+    off_piv = off_X + ((size_t)n * k * 8 + 255) / 256 * 256;
+    off_info = off_piv + ((size_t)n * 4 + 255) / 256 * 256;
+    total = off_info + 256;
+  }
+};
+
 // Budget bytes a request holds until released (a solve alone: its own batch pool).
 size_t request_bytes(const bzg_server* s, const Request* r) {
   if (r->kind == kGemm) return size_class((size_t)r->m * r->n * sizeof(double));
   if (r->kind == kSpmv) return size_class((size_t)s->csrs[r->csr].n_rows * sizeof(double));
+  if (r->direct) return size_class(DirectLayout(r->n, r->k).total);
   return size_class(SolveLayout(1, r->n * r->k).total);
 }
 
@@ -341,6 +355,37 @@ bzg_status launch_solves(bzg_server* s, Group* g) {
   return BZG_OK;
 }
 
+// A solve of order 257..512 on its own: the batched mixed solver's kernel for these orders
+// takes one wavefront per system (about 10 ms for n = 512), the single-system FP64 LU
+// (bzg_dgetrf + bzg_dgetrs: recursive, MFMA trailing updates) a few ms.  The result is FP64
+// LU's (iterations = -1, as for a fallback); info comes back with the group's event.
+bzg_status launch_direct(bzg_server* s, Group* g) {
+  bzg_context* c = s->ctx[g->stream];
+  Request* r = g->reqs[0];
+  const DirectLayout L(r->n, r->k);
+  bzg_status st = s->dev.get(L.total, &g->dpool, &g->dcls);
+  if (!st) st = s->host.get(2 * sizeof(int32_t), &g->hpool, &g->hcls);
+  if (st) return st;
+  char* d = static_cast<char*>(g->dpool);
+  double* Ap = reinterpret_cast<double*>(d);
+  double* Xp = reinterpret_cast<double*>(d + L.off_X);
+  int32_t* piv = reinterpret_cast<int32_t*>(d + L.off_piv);
+  int32_t* dinfo = reinterpret_cast<int32_t*>(d + L.off_info);
+  g->h_iter = static_cast<int32_t*>(g->hpool);
+  g->h_info = g->h_iter + 1;
+  g->h_iter[0] = -1;
+  g->h_info[0] = 0;
+  BZG_HIP_CHECK(hipMemcpyAsync(Ap, r->A, (size_t)r->n * r->n * 8, hipMemcpyDeviceToDevice, c->stream));
+  BZG_HIP_CHECK(hipMemcpyAsync(Xp, r->B, (size_t)r->n * r->k * 8, hipMemcpyDeviceToDevice, c->stream));
+  if ((st = bzg_dgetrf(c, r->n, Ap, r->n, piv, dinfo))) return st;
+  if ((st = bzg_dgetrs(c, r->n, r->k, Ap, r->n, piv, Xp, r->k))) return st;
+  BZG_HIP_CHECK(hipMemcpyAsync(g->h_info, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  r->out = Xp;
+  r->out_cls = 0;
+  g->live = 1;
+  return BZG_OK;
+}
+
 void release_group_pool(bzg_server* s, Group* g) {
   s->dev.put(g->dpool, g->dcls);
   s->host.put(g->hpool, g->hcls);
@@ -442,14 +487,24 @@ void dispatcher(bzg_server* s) {
         s->used += need;
         head->t_admit = t;
         st = launch_single(s, head, g);
+      } else if (head->direct) {
+        s->pending.pop_front();
+        g->reqs.push_back(head);
+        head->group = g;
+        head->t_admit = t;
+        g->budget_bytes = need;
+        s->used += need;
+        st = launch_direct(s, g);
+        s->st.solve_batches++;
+        s->st.max_solve_batch = std::max<int64_t>(s->st.max_solve_batch, 1);
       } else {
-        // the head's batch: every admissible waiting solve, in arrival order (the pool of
-        // the whole batch must fit; the head fits: checked above)
+        // the head's batch: every admissible waiting solve of order <= 256, in arrival order
+        // (the pool of the whole batch must fit; the head fits: checked above)
         size_t bytes = 0;
         int64_t elems = 0;
         for (auto it = s->pending.begin(); it != s->pending.end();) {
           Request* r = *it;
-          if (r->kind != kSolve) { ++it; continue; }
+          if (r->kind != kSolve || r->direct) { ++it; continue; }
           const int64_t cnt = (int64_t)g->reqs.size() + 1;
           const size_t b = size_class(SolveLayout(cnt, elems + r->n * r->k).total);
           if (cnt > s->cfg.solve_batch_max || s->used + b > s->cfg.hbm_budget_bytes) break;
@@ -682,6 +737,7 @@ bzg_status bzg_submit_solve(bzg_server* s, int32_t n, int32_t nrhs, const double
   r->kind = bzg::kSolve;
   r->n = n; r->k = nrhs; r->A = A; r->B = B;
   r->out_elems = (int64_t)n * nrhs;
+  r->direct = n > 256;
   return submit(s, r, id);
 }
 

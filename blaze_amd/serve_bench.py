@@ -193,8 +193,16 @@ def run_rate(wl, srv, rate, count):
         else:
             d["achieved_TFLOP_s"] = fl / wall / 1e12
         if cls == "solve":
-            its = [results[i][5] for i in idx]
-            d["fallback_fp64"] = int(sum(1 for t in its if t < 0))
+            # orders <= 256: FP32 LU + refinement, a negative count = the FP64 fallback; orders
+            # 257..512 run on the single-system FP64 LU by design (count -1)
+            big = [i for i in idx if wl.sys[reqs[i][1]][0] > 256]
+            d["fallback_fp64"] = int(sum(1 for i in idx if results[i][5] < 0 and wl.sys[reqs[i][1]][0] <= 256))
+            d["fp64_lu_n_gt_256"] = len(big)
+            if big:
+                lat_big = np.array([(results[i][2] - results[i][0]) / 1e3 for i in big])
+                d["p99_ms_n_le_256"] = float(np.percentile(
+                    [(results[i][2] - results[i][0]) / 1e3 for i in idx if i not in set(big)] or [0.0], 99))
+                d["p99_ms_n_gt_256"] = float(np.percentile(lat_big, 99))
             d["nonzero_info"] = int(sum(1 for i in idx if results[i][4] != 0))
         out[cls] = d
     return out
