@@ -554,12 +554,13 @@ __device__ __forceinline__ int swz(int r, int c) { return r * RBYTES + ((c ^ ((r
 // Output residues are symmetric: int8 for INT8 slices (m <= 255 -> |r| <= 127), int16 for BF16.
 //
 // Pipeline per K stage (128 bytes of K, 4 MFMA k-steps): the fragments of k-step ks+1 are read
-// from LDS into a second register set while the 4 x FN MFMAs of k-step ks run; on the last
-// k-step the staged registers (stage st+1, loaded one stage earlier) are written into the other
-// LDS buffer (last read before the previous barrier) and the global loads of stage st+2 are
-// issued into them at once, so each load has a whole stage and a barrier to land (measured
-// 6.76 ms against 7.05 ms with the loads issued at the top of the stage, 8192^3, 14 moduli);
-// one barrier closes the stage.
+// from LDS into a second register set while the 4 x FN MFMAs of k-step ks run; every k-step
+// also writes a quarter of the staged registers (stage st+1, loaded one stage earlier) into the
+// other LDS buffer (last read before the previous barrier) and issues the global loads of stage
+// st+2 into the registers it has just written out, so each load has a whole stage to land
+// (8192^3, 14 moduli: 6.60 ms, against 6.76 ms with all stores and loads on the last k-step and
+// 7.05 ms with the loads issued at the top of the stage for the next one); one barrier closes
+// the stage.
 //
 // MODE 2 (INT32 GEMM from balanced base-256 digits): Ares / Bres are single arrays whose rows
 // hold the 4 digit planes side by side along K (A: d0 d1 d2 d3, B^T: d3 d2 d1 d0, kp each), and
@@ -578,8 +579,6 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
   constexpr int ESZ = (MODE == 0) ? 2 : 1;          // bytes per element
   constexpr int KSTEP_B = 32;                        // bytes of K per MFMA (16 bf16 / 32 i8)
   constexpr int NKS = RBYTES / KSTEP_B;              // MFMA k-steps per stage
-  constexpr int kStoreStep = NKS - 1;  // k-step that carries the next stage's LDS stores (the
-                                       // last: the global loads get the whole stage to land)
   constexpr int BN = 64 * FN;                        // workgroup tile columns
   constexpr int NBCH = BN * 8 / 256;                 // B chunks staged per thread
   extern __shared__ __attribute__((aligned(16))) char smem[];
@@ -636,6 +635,34 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
     for (int e = 0; e < NBCH; ++e) {
       const int q = tid + 256 * e, r = q >> 3, c = q & 7;
       rb[e] = *reinterpret_cast<const v4i*>(Bg + (col0 + r) * rowb + kb + c * 16);
+    }
+  };
+  // the stores and loads of the flattened staging chunks e in [lo, hi) (A: 0..7, B: 8..)
+  auto sstore_part = [&](int buf, int lo, int hi) {
+#pragma unroll
+    for (int e = 0; e < 8 + NBCH; ++e) {
+      if (e < lo || e >= hi) continue;
+      if (e < 8) {
+        const int q = tid + 256 * e, r = q >> 3, c = q & 7;
+        *reinterpret_cast<v4i*>(As + buf * RSTAGE + swz(r, c)) = ra[e];
+      } else {
+        const int q = tid + 256 * (e - 8), r = q >> 3, c = q & 7;
+        *reinterpret_cast<v4i*>(Bs + buf * RSTAGE + swz(r, c)) = rb[e - 8];
+      }
+    }
+  };
+  auto gload_part = [&](int st, int lo, int hi) {
+    const long kb = (long)st * RBYTES;
+#pragma unroll
+    for (int e = 0; e < 8 + NBCH; ++e) {
+      if (e < lo || e >= hi) continue;
+      if (e < 8) {
+        const int q = tid + 256 * e, r = q >> 3, c = q & 7;
+        ra[e] = *reinterpret_cast<const v4i*>(Ag + (row0 + r) * rowb + kb + c * 16);
+      } else {
+        const int q = tid + 256 * (e - 8), r = q >> 3, c = q & 7;
+        rb[e - 8] = *reinterpret_cast<const v4i*>(Bg + (col0 + r) * rowb + kb + c * 16);
+      }
     }
   };
   auto sstore = [&](int buf) {
@@ -708,15 +735,14 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
       const int cur = ks & 1;
       if (ks + 1 < NKS) lfrag(buf, ks + 1, fa[cur ^ 1], fb[cur ^ 1]);
       kstep(cur);
-      if (ks == kStoreStep) {
-        sstore(buf ^ 1);
-        gload(min(st + 2, nst - 1));
-      }
+      constexpr int QS = (8 + NBCH) / NKS;
+      sstore_part(buf ^ 1, ks * QS, (ks + 1) * QS);
+      gload_part(min(st + 2, nst - 1), ks * QS, (ks + 1) * QS);
     }
     // instruction order of the stage: the first 4 + FN MFMAs of k-steps 0..NKS-2 are each
-    // followed by one LDS fragment read of the following k-step; on the last k-step each of
-    // the first 8 + NBCH MFMAs is followed by one LDS store of the next stage and one global
-    // load of the stage after it (the load reuses the register the store just read).
+    // followed by one LDS fragment read of the following k-step; the next (8 + NBCH) / NKS
+    // MFMAs of every k-step by one LDS store of the next stage and one global load of the
+    // stage after it (the load reuses the register the store just read).
     constexpr int NM = 4 * FN;
     if constexpr (MODE != 0)  // (the BF16 tile measured faster with the compiler's own order)
 #pragma unroll
@@ -725,8 +751,8 @@ __global__ __launch_bounds__(256, 1) void residue_gemm_kernel(int mp, int np, in
       for (int t = 0; t < NM; ++t) {
         __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);               // MFMA
         if (ks + 1 < NKS && t < 4 + FN) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
-        if (ks == kStoreStep && t < 8 + NBCH) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
-        if (ks == kStoreStep && t < 8 + NBCH) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+        if (t >= 4 + FN && t < 4 + FN + (8 + NBCH) / NKS) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+        if (t >= 4 + FN && t < 4 + FN + (8 + NBCH) / NKS) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
       }
     if ((st & 3) == 3) reduce_acc();
     __syncthreads();
