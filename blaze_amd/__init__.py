@@ -74,6 +74,11 @@ SIGNATURES = {
     "bzg_iaxpy": (i32, [vp, i64, i32, vp, vp]),
     "bzg_dgetrs": (i32, [vp, i64, i64, vp, i64, vp, vp, i64]),
     "bzg_dgesv": (i32, [vp, i64, i64, vp, i64, vp, vp, i64, vp]),
+    "bzg_zgemm_ex": (i32, [vp, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64]),
+    "bzg_cgemm_ex": (i32, [vp, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64]),
+    "bzg_zgemv_ex": (i32, [vp, i32, i64, i64, vp, vp, i64, vp, vp, vp]),
+    "bzg_cgemv_ex": (i32, [vp, i32, i64, i64, vp, vp, i64, vp, vp, vp]),
+    "bzg_zgesv": (i32, [vp, i64, i64, vp, i64, vp, i64, vp]),
     "bzg_dgetri": (i32, [vp, i64, vp, i64, vp]),
     "bzg_ddet_lu": (i32, [vp, i64, vp, i64, vp, vp, vp]),
     "bzg_dgetrs_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -38,7 +38,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["context.hip", "gemm.hip", "gemv.hip", "spmv.hip", "cg.hip", "lu.hip", "lu_mixed.hip", "getrf.hip", "ozaki.hip",
-               "server.hip", "mmio.hip"]
+               "server.hip", "mmio.hip", "complex.hip"]
 
 
 def _run(cmd: list[str]) -> None:

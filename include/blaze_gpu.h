@@ -35,6 +35,12 @@
  *        (clapack/gesv.h:207, getrf.h:190-197, getrs.h:207-215; solveNxN, LSE.h:5229-5257)
  *   bzg_ddet_lu / bzg_dgetri            <- det() (DMatDetExpr.h:324-357) and invert() by LU
  *        (Inversion.h:3337-3347) from those factors
+ *   bzg_zgemm_ex / bzg_cgemm_ex        <- DMatDMatMultExpr for complex<double> / complex<float>
+ *        elements and its BLAS call cblas_zgemm / cblas_cgemm (blaze/math/blas/cblas/gemm.h:197-283)
+ *   bzg_zgemv_ex / bzg_cgemv_ex        <- DMatDVecMultExpr for complex elements and cblas_zgemv /
+ *        cblas_cgemv (blaze/math/blas/cblas/gemv.h:186-245)
+ *   bzg_zgesv                          <- solve() of a complex<double> system -> gesv -> zgesv_
+ *        (blaze/math/lapack/clapack/gesv.h:317-333)
  *   bzg_dgemm_fp64acc[_i8]             <- no reference function: FP64-accurate GEMM from BF16
  *        (INT8) residue slices; drop-in for bzg_dgemm at the same call site
  *        (DMatDMatMultExpr.h:516)
@@ -421,6 +427,40 @@ bzg_status bzg_dgetri(bzg_context* ctx, int64_t n, double* A, int64_t lda, const
  * diagonal with the sign flipped at every interchange (det is a DEVICE pointer). */
 bzg_status bzg_ddet_lu(bzg_context* ctx, int64_t n, const double* A, int64_t lda, const int32_t* ipiv,
                        const int32_t* info, double* det);
+
+/* ---- complex element types (interleaved re, im: std::complex<T>, C99 T _Complex) -----------
+ * Pointers are to the first real of the first element; leading dimensions count complex
+ * elements; alpha and beta point to (re, im) pairs in host memory.  trans codes: 0 = N,
+ * 1 = T (transpose), 2 = C (conjugate transpose), as CBLAS_TRANSPOSE.
+ *
+ * C = alpha op(A) op(B) + beta C, row-major, m x n x k (replaces cblas_zgemm / cblas_cgemm,
+ * blaze/math/blas/cblas/gemm.h:197-283, the BLAS path of DMatDMatMultExpr for complex
+ * elements).  op(A) is m x k: A stored m x k (N) or k x m (T, C); op(B) k x n likewise.
+ * beta = 0 never reads C.  Runs on the real MFMA GEMM through the 2 x 2 real form of the
+ * product (8 m n k flops); uses 32 k n bytes of workspace (complex<double>; half for float),
+ * plus 16 m k for a transposed A. */
+bzg_status bzg_zgemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        const double* alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                        const double* beta, double* C, int64_t ldc);
+bzg_status bzg_cgemm_ex(bzg_context* ctx, int32_t transa, int32_t transb, int64_t m, int64_t n, int64_t k,
+                        const float* alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+                        const float* beta, float* C, int64_t ldc);
+/* y = alpha op(A) x + beta y, A stored m x n row-major (replaces cblas_zgemv / cblas_cgemv,
+ * blaze/math/blas/cblas/gemv.h:186-245, for DMatDVecMultExpr with complex elements): op = N
+ * takes x of n and y of m entries, op = T / C x of m and y of n.  beta = 0 never reads y. */
+bzg_status bzg_zgemv_ex(bzg_context* ctx, int32_t trans, int64_t m, int64_t n, const double* alpha,
+                        const double* A, int64_t lda, const double* x, const double* beta, double* y);
+bzg_status bzg_cgemv_ex(bzg_context* ctx, int32_t trans, int64_t m, int64_t n, const float* alpha,
+                        const float* A, int64_t lda, const float* x, const float* beta, float* y);
+/* Solves A X = B for one complex<double> system of order n <= 16384 (replaces solve() of a
+ * complex system: gesv -> zgesv_, blaze/math/lapack/clapack/gesv.h:317-333).  B (n x nrhs, ldb)
+ * is overwritten with X; A is not modified.  Solved as the real 2n x 2n system
+ * [[Re A, -Im A], [Im A, Re A]] [Re X; Im X] = [Re B; Im B] by bzg_dgesv, so there are no complex
+ * pivots to return (no ipiv).  *info (DEVICE pointer): 0, or > 0 when A is singular (an index
+ * of the real system's elimination, <= 2n; B is left unchanged then).  Workspace: 32 n^2 +
+ * 16 n nrhs bytes. */
+bzg_status bzg_zgesv(bzg_context* ctx, int64_t n, int64_t nrhs, const double* A, int64_t lda, double* B,
+                     int64_t ldb, int32_t* info);
 
 /* ---- FP64-accurate GEMM from BF16 residue slices (Ozaki scheme II on MFMA) --------------
  * Same contract as bzg_dgemm with alpha = 1, beta = 0.  `moduli` selects the number of

@@ -106,6 +106,28 @@ def _p(a: np.ndarray) -> C.c_void_p:
 
 
 # ---- oracle (restatement) ---------------------------------------------------------------
+def op(M: np.ndarray, code: int) -> np.ndarray:
+    """op(M) for the BLAS transpose codes 0 = N, 1 = T, 2 = C (conjugate transpose)."""
+    return M if code == 0 else (M.T if code == 1 else M.conj().T)
+
+
+def gemm_op(opa: int, opb: int, A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """C = op(A) op(B) for complex operands (DMatDMatMultExpr with complex elements; the
+    textbook product, evaluated in complex128)."""
+    return (op(A, opa).astype(np.complex128) @ op(B, opb).astype(np.complex128))
+
+
+def gemm_op_bound(opa: int, opb: int, A: np.ndarray, B: np.ndarray, eps: float) -> np.ndarray:
+    """Entry-wise error bound of a computed complex product of inner dimension k, in any
+    summation order: |c - c_exact| <= (k + 2) eps (|Re A| + |Im A|)(|Re B| + |Im B|), plus a
+    few units of the smallest normal (the real 2 x 2 form rounds each of the 2k real products
+    once and adds them in some order)."""
+    a = op(np.abs(A.real) + np.abs(A.imag), opa)
+    b = op(np.abs(B.real) + np.abs(B.imag), opb)
+    k = a.shape[1]
+    return 2.0 * (k + 2) * eps * (a @ b) + 1e-300
+
+
 def gemm(A: np.ndarray, B: np.ndarray, fused: bool = True) -> np.ndarray:
     m, k = A.shape
     k2, n = B.shape
@@ -296,6 +318,38 @@ def ref_csrmv(rowptr, col, val, x, n_cols: int) -> np.ndarray:
     y = np.empty(n, val.dtype)
     getattr(ref(), fn)(i64(n), i64(n_cols), _p(rowptr), _p(col), _p(val), _p(x), _p(y))
     return y
+
+
+def ref_gemm_op(opa: int, opb: int, A: np.ndarray, B: np.ndarray, m: int, n: int, k: int) -> np.ndarray:
+    """C = op(A) * op(B) by the reference's own DMatDMatMultExpr for complex elements (op 0 =
+    none, 1 = trans(), 2 = ctrans(); A stored m x k or k x m, B stored k x n or n x k)."""
+    fn = {np.dtype(np.complex128): "ref_zgemm_op", np.dtype(np.complex64): "ref_cgemm_op"}[A.dtype]
+    Cm = np.empty((m, n), A.dtype)
+    getattr(ref(), fn)(C.c_int(opa), C.c_int(opb), i64(m), i64(n), i64(k), _p(np.ascontiguousarray(A)),
+                       _p(np.ascontiguousarray(B)), _p(Cm))
+    return Cm
+
+
+def ref_gemv_op(op: int, A: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """y = op(A) * x by the reference's DMatDVecMultExpr for complex elements."""
+    m, n = A.shape
+    fn = {np.dtype(np.complex128): "ref_zgemv_op", np.dtype(np.complex64): "ref_cgemv_op"}[A.dtype]
+    y = np.empty(n if op else m, A.dtype)
+    getattr(ref(), fn)(C.c_int(op), i64(m), i64(n), _p(np.ascontiguousarray(A)), _p(np.ascontiguousarray(x)),
+                       _p(y))
+    return y
+
+
+def ref_zsolve(A: np.ndarray, B: np.ndarray):
+    """(X, threw): the reference's solve(A, X, B) for one complex<double> system (zgesv_ for
+    n >= 7, through the image's LAPACK)."""
+    n, nrhs = B.shape
+    A = np.ascontiguousarray(A, np.complex128); B = np.ascontiguousarray(B, np.complex128)
+    X = np.zeros((n, nrhs), np.complex128)
+    L = ref_lapack()
+    L.ref_zsolve.restype = i32
+    rc = L.ref_zsolve(i64(n), i64(nrhs), _p(A), _p(B), _p(X))
+    return X, rc
 
 
 def ref_solve_small(A: np.ndarray, b: np.ndarray):

@@ -8,6 +8,7 @@
 // lu()) need an external LAPACK and are not built here.
 #include <blaze/Math.h>
 #include <chrono>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -85,6 +86,42 @@ void tcsrmv(int64_t n_rows, int64_t n_cols, const int32_t* rowptr, const int32_t
   yv = trans(S) * xv;
   for (int64_t j = 0; j < n_cols; ++j) y[j] = yv[j];
 }
+// C = op(A) * op(B) with op 0 = none, 1 = trans(), 2 = ctrans(); A stored m x k (op 0) or
+// k x m, B stored k x n (op 0) or n x k.
+template <typename T>
+void gemm_op(int opa, int opb, int64_t m, int64_t n, int64_t k, const T* A, const T* B, T* C) {
+  blaze::DynamicMatrix<T, rowMajor> a, b, c;
+  to_blaze(a, opa ? k : m, opa ? m : k, A);
+  to_blaze(b, opb ? n : k, opb ? k : n, B);
+  switch (opa * 3 + opb) {
+    case 0: c = a * b; break;
+    case 1: c = a * trans(b); break;
+    case 2: c = a * ctrans(b); break;
+    case 3: c = trans(a) * b; break;
+    case 4: c = trans(a) * trans(b); break;
+    case 5: c = trans(a) * ctrans(b); break;
+    case 6: c = ctrans(a) * b; break;
+    case 7: c = ctrans(a) * trans(b); break;
+    default: c = ctrans(a) * ctrans(b); break;
+  }
+  from_blaze(c, C);
+}
+// y = op(A) * x, A stored m x n
+template <typename T>
+void gemv_op(int op, int64_t m, int64_t n, const T* A, const T* x, T* y) {
+  blaze::DynamicMatrix<T, rowMajor> a;
+  to_blaze(a, m, n, A);
+  const int64_t xl = op ? m : n;
+  blaze::DynamicVector<T, columnVector> xv(xl), yv;
+  for (int64_t j = 0; j < xl; ++j) xv[j] = x[j];
+  if (op == 0) yv = a * xv;
+  else if (op == 1) yv = trans(a) * xv;
+  else yv = ctrans(a) * xv;
+  for (size_t i = 0; i < yv.size(); ++i) y[i] = yv[i];
+}
+using zd = std::complex<double>;
+using zf = std::complex<float>;
+
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -102,6 +139,11 @@ void ref_igemm(int64_t m, int64_t n, int64_t k, const int32_t* A, const int32_t*
 void ref_dgemv(int64_t m, int64_t n, const double* A, const double* x, double* y) { gemv(m, n, A, x, y); }
 void ref_sgemv(int64_t m, int64_t n, const float* A, const float* x, float* y) { gemv(m, n, A, x, y); }
 void ref_igemv(int64_t m, int64_t n, const int32_t* A, const int32_t* x, int32_t* y) { gemv(m, n, A, x, y); }
+
+void ref_zgemm_op(int opa, int opb, int64_t m, int64_t n, int64_t k, const zd* A, const zd* B, zd* C) { gemm_op(opa, opb, m, n, k, A, B, C); }
+void ref_cgemm_op(int opa, int opb, int64_t m, int64_t n, int64_t k, const zf* A, const zf* B, zf* C) { gemm_op(opa, opb, m, n, k, A, B, C); }
+void ref_zgemv_op(int op, int64_t m, int64_t n, const zd* A, const zd* x, zd* y) { gemv_op(op, m, n, A, x, y); }
+void ref_cgemv_op(int op, int64_t m, int64_t n, const zf* A, const zf* x, zf* y) { gemv_op(op, m, n, A, x, y); }
 
 void ref_dcsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const double* v, const double* x, double* y) { csrmv(r, c, rp, ci, v, x, y); }
 void ref_scsrmv(int64_t r, int64_t c, const int32_t* rp, const int32_t* ci, const float* v, const float* x, float* y) { csrmv(r, c, rp, ci, v, x, y); }

@@ -12,6 +12,7 @@
 #include <omp.h>
 
 #include <chrono>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -65,6 +66,24 @@ double ref_time_dsolve_batch(int64_t count, const int32_t* n, const int32_t* nrh
     if (t < best) best = t;
   }
   return best;
+}
+
+// X = solve(A, B) for one complex<double> system (row-major n x n and n x nrhs) with the
+// reference's solve() (closed forms for n <= 6, zgesv_ beyond); returns 1 when it threw.
+int ref_zsolve(int64_t n, int64_t nrhs, const std::complex<double>* A, const std::complex<double>* B,
+               std::complex<double>* X) {
+  using ZMat = blaze::CustomMatrix<std::complex<double>, blaze::unaligned, blaze::unpadded, blaze::rowMajor>;
+  const ZMat a(const_cast<std::complex<double>*>(A), (size_t)n, (size_t)n);
+  const ZMat b(const_cast<std::complex<double>*>(B), (size_t)n, (size_t)nrhs);
+  blaze::DynamicMatrix<std::complex<double>, blaze::rowMajor> x;
+  try {
+    blaze::solve(a, x, b);
+  } catch (const std::exception&) {
+    return 1;
+  }
+  ZMat out(X, (size_t)n, (size_t)nrhs);
+  out = x;
+  return 0;
 }
 
 }  // extern "C"

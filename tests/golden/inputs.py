@@ -94,3 +94,50 @@ def solve_small_case(n, t):
 def load(name):
     with np.load(HERE / name, allow_pickle=False) as f:
         return {k: f[k] for k in f.files}
+
+
+def complex_uniform(seed, stream, shape, dtype=np.complex128):
+    """Complex entries with real and imaginary parts uniform[-1, 1) from counter streams 2s, 2s + 1
+    (complex64: the double values rounded)."""
+    size = int(np.prod(shape))
+    z = uniform(seed, 2 * stream, size) + 1j * uniform(seed, 2 * stream + 1, size)
+    return z.reshape(shape).astype(dtype)
+
+
+COMPLEX_GEMM_SHAPES = [(m, n, k) for m in (0, 1, 2, 3, 5) for n in (0, 1, 2, 3, 5) for k in (0, 1, 2, 3, 5)] + [
+    (15, 37, 63), (37, 15, 63), (64, 64, 64), (33, 35, 67)]
+COMPLEX_GEMM_OPS = [(0, 0), (1, 0), (0, 1), (0, 2), (2, 0), (2, 1), (1, 2), (2, 2)]
+COMPLEX_GEMM_OPS_LARGE = [(0, 0), (2, 1), (1, 2)]  # shapes beyond 5 x 5 x 5
+
+
+def complex_gemm_ops(idx):
+    """The op pairs of case idx: all eight on the small grid, three on the larger shapes."""
+    return COMPLEX_GEMM_OPS if max(COMPLEX_GEMM_SHAPES[idx]) <= 5 else COMPLEX_GEMM_OPS_LARGE
+
+
+def complex_gemm_case(seed, idx, opa, opb, dtype=np.complex128):
+    """Stored operands of complex GEMM case idx (shape from COMPLEX_GEMM_SHAPES, op pair):
+    A is m x k (op 0) or k x m, B is k x n (op 0) or n x k."""
+    m, n, k = COMPLEX_GEMM_SHAPES[idx]
+    sa = (k, m) if opa else (m, k)
+    sb = (n, k) if opb else (k, n)
+    return (complex_uniform(seed, 2 * idx, sa, dtype), complex_uniform(seed, 2 * idx + 1, sb, dtype))
+
+
+COMPLEX_GEMV_SHAPES = [(m, n) for m in (0, 1, 2, 4) for n in (0, 1, 3, 4)] + [(67, 127), (127, 67), (300, 1000)]
+
+
+def complex_gemv_case(seed, idx, op, dtype=np.complex128):
+    """A (m x n) and x (n entries for op 0, m for op 1 / 2) of complex GEMV case idx."""
+    m, n = COMPLEX_GEMV_SHAPES[idx]
+    return (complex_uniform(seed, 2 * idx, (m, n), dtype), complex_uniform(seed, 2 * idx + 1, (m if op else n,), dtype))
+
+
+COMPLEX_SOLVE_CASES = [(1, 1), (2, 1), (3, 2), (5, 1), (6, 3), (7, 1), (16, 2), (50, 4), (129, 3)]
+
+
+def complex_solve_case(seed, idx):
+    """A (n x n, complex uniform plus n on the diagonal) and B (n x nrhs) of solve case idx."""
+    n, nrhs = COMPLEX_SOLVE_CASES[idx]
+    A = complex_uniform(seed, 2 * idx, (n, n)) + n * np.eye(n)
+    return A, complex_uniform(seed, 2 * idx + 1, (n, nrhs))

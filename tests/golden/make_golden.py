@@ -26,8 +26,9 @@ sys.path.insert(0, str(HERE))
 
 import oracle  # noqa: E402
 
-from inputs import (GEMM_SHAPES, gemm_case, gemv_case, ints, laplace3d, solve_small_case, uniform,  # noqa: E402
-                    uniform_mt, zipf)
+from inputs import (COMPLEX_GEMM_SHAPES, COMPLEX_GEMV_SHAPES, COMPLEX_SOLVE_CASES,  # noqa: E402
+                    GEMM_SHAPES, complex_gemm_case, complex_gemm_ops, complex_gemv_case, complex_solve_case, gemm_case, gemv_case,
+                    ints, laplace3d, solve_small_case, uniform, uniform_mt, zipf)
 
 GEMM_SEED = 0x0B1A2E
 
@@ -53,6 +54,43 @@ def gemm_fixtures():
     rows = rng.integers(0, n, 4096); cols = rng.integers(0, n, 4096)
     np.savez_compressed(HERE / "gemm_512_mt.npz", seed=np.uint64(seed), n=np.int64(n), rows=rows,
                         cols=cols, c_sample=Cb[rows, cols], c_frob=np.linalg.norm(Cb))
+
+
+COMPLEX_SEED = 0xC0FFEE
+
+
+def complex_fixtures():
+    # complex<double> / complex<float> products for every op pair (none, trans, ctrans) on a size
+    # grid, GEMV for every op, and complex<double> solve() (closed forms n <= 6, zgesv_ beyond):
+    # expected outputs from the reference's own expression kernels and solve()
+    z, c = [], []
+    for idx, (m, n, k) in enumerate(COMPLEX_GEMM_SHAPES):
+        for opa, opb in complex_gemm_ops(idx):
+            A, B = complex_gemm_case(COMPLEX_SEED, idx, opa, opb)
+            z.append(oracle.ref_gemm_op(opa, opb, A, B, m, n, k).ravel())
+            if (opa, opb) == (0, 0) or (opa, opb) == (2, 1):
+                A32, B32 = complex_gemm_case(COMPLEX_SEED, idx, opa, opb, np.complex64)
+                c.append(oracle.ref_gemm_op(opa, opb, A32, B32, m, n, k).ravel())
+    gz, gc = [], []
+    for idx, (m, n) in enumerate(COMPLEX_GEMV_SHAPES):
+        for op in (0, 1, 2):
+            A, x = complex_gemv_case(COMPLEX_SEED, idx, op)
+            gz.append(oracle.ref_gemv_op(op, A, x))
+            A32, x32 = complex_gemv_case(COMPLEX_SEED, idx, op, np.complex64)
+            gc.append(oracle.ref_gemv_op(op, A32, x32))
+    xs = []
+    for idx in range(len(COMPLEX_SOLVE_CASES)):
+        A, B = complex_solve_case(COMPLEX_SEED, idx)
+        X, rc = oracle.ref_zsolve(A, B)
+        assert rc == 0, idx
+        xs.append(X.ravel())
+    # a singular system (an exactly zero column): the reference's solve() throws
+    A, B = complex_solve_case(COMPLEX_SEED, 7)
+    A[:, 5] = 0
+    _, rc_singular = oracle.ref_zsolve(A, B)
+    np.savez_compressed(HERE / "complex_blazetest.npz", seed=np.uint64(COMPLEX_SEED),
+                        gemm_z=np.concatenate(z), gemm_c=np.concatenate(c), gemv_z=np.concatenate(gz),
+                        gemv_c=np.concatenate(gc), solve_z=np.concatenate(xs), singular_throws=np.int32(rc_singular))
 
 
 def spmv_fixtures():
@@ -152,6 +190,7 @@ def main() -> None:
     spmv_trans_fixtures()
     gemv_fixtures()
     solve_fixtures()
+    complex_fixtures()
     for f in sorted(HERE.glob("*.npz")):
         print(f"{f.name}: {f.stat().st_size} bytes")
 
