@@ -142,13 +142,33 @@ __global__ __launch_bounds__(256) void zgemv_n_kernel(int64_t m, int64_t n, R ar
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   if (row >= m) return;  // the whole group leaves together
   const R* a = A + 2 * row * lda;
-  R sre = 0, sim = 0;
-  for (int64_t j = lig; j < n; j += G) {
+  // four elements per lane and step, loads issued before the arithmetic, one partial sum per
+  // slot (added in slot order afterwards)
+  R pre[4] = {0, 0, 0, 0}, pim[4] = {0, 0, 0, 0};
+  int64_t j = lig;
+  for (; j + 3 * G < n; j += 4 * G) {
+    R xr[4], xi[4], are[4], aim[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t c = j + u * G;
+      are[u] = a[2 * c];
+      aim[u] = a[2 * c + 1];
+      xr[u] = x[2 * c];
+      xi[u] = x[2 * c + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      pre[u] += are[u] * xr[u] - aim[u] * xi[u];
+      pim[u] += are[u] * xi[u] + aim[u] * xr[u];
+    }
+  }
+  for (int u = 0; j < n; j += G, ++u) {
     const R xr = x[2 * j], xi = x[2 * j + 1];
     const R are = a[2 * j], aim = a[2 * j + 1];
-    sre += are * xr - aim * xi;
-    sim += are * xi + aim * xr;
+    pre[u] += are * xr - aim * xi;
+    pim[u] += are * xi + aim * xr;
   }
+  R sre = (pre[0] + pre[1]) + (pre[2] + pre[3]), sim = (pim[0] + pim[1]) + (pim[2] + pim[3]);
 #pragma unroll
   for (int off = G / 2; off >= 1; off >>= 1) {
     sre += __shfl_xor(sre, off, 64);
@@ -179,6 +199,7 @@ __global__ __launch_bounds__(256) void zgemv_t_part_kernel(int64_t m, int64_t n,
   const int64_t i0 = (int64_t)blockIdx.y * kRowChunk, i1 = (m < i0 + kRowChunk) ? m : i0 + kRowChunk;
   const R sg = conj ? R(-1) : R(1);
   R sre = 0, sim = 0;
+#pragma unroll 8
   for (int64_t i = i0; i < i1; ++i) {
     const R are = A[2 * (i * lda + j)], aim = sg * A[2 * (i * lda + j) + 1];
     const R xr = x[2 * i], xi = x[2 * i + 1];

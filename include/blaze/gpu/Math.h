@@ -7,6 +7,7 @@
 //                                                               y = S * x;        // CSR SpMV
 //   solve(A, x, b);  x = solve(A, b);  solve(A, X, B);                           // LU solve
 //   y = trans(A) * x;  C = trans(A) * B;  y = trans(S) * x;    lu / det / inv / invert
+//   blaze::gpu::DynamicMatrix<std::complex<double>> Z;  C = ctrans(Z) * Z;  solve(Z, x, b)
 //   blaze::gpu::DynamicMatrix<double, columnMajor>, CompressedMatrix<double, columnMajor>
 // Semantics follow the reference: operator* throws std::invalid_argument("Matrix sizes do not
 // match") (blaze/math/expressions/DMatDMatMultExpr.h:9646), ("Matrix and vector sizes do not
@@ -27,7 +28,8 @@
 // another stream calls synchronize() first.
 //
 // The element types follow the reference's tests: double and float (tolerance parity) and
-// int (bit-exact parity).  FP64 products can be routed to the FP64-accurate split GEMM with
+// int (bit-exact parity); dense products, GEMV and (complex<double>) solve() also take
+// std::complex<double> / std::complex<float>, with ctrans(A) as a product operand.  FP64 products can be routed to the FP64-accurate split GEMM with
 // setFp64AccurateGemm(true) (same results within 1e-14 relative, higher throughput).
 #ifndef BLAZE_GPU_MATH_H
 #define BLAZE_GPU_MATH_H
@@ -41,6 +43,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <complex>
 #include <vector>
 
 #include "blaze_gpu.h"
@@ -151,6 +154,21 @@ inline bzg_status gemm(int64_t m, int64_t n, int64_t k, int al, const int* A, in
                        bool tb = false) {
   return bzg_igemm_ex(context(), ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
 }
+// complex elements: ta / tb are op codes (0 none, 1 trans, 2 ctrans)
+inline bzg_status gemm(int64_t m, int64_t n, int64_t k, std::complex<double> al, const std::complex<double>* A,
+                       int64_t lda, const std::complex<double>* B, int64_t ldb, std::complex<double> be,
+                       std::complex<double>* C, int64_t ldc, int ta = 0, int tb = 0) {
+  const double a[2] = {al.real(), al.imag()}, b[2] = {be.real(), be.imag()};
+  return bzg_zgemm_ex(context(), ta, tb, m, n, k, a, reinterpret_cast<const double*>(A), lda,
+                      reinterpret_cast<const double*>(B), ldb, b, reinterpret_cast<double*>(C), ldc);
+}
+inline bzg_status gemm(int64_t m, int64_t n, int64_t k, std::complex<float> al, const std::complex<float>* A,
+                       int64_t lda, const std::complex<float>* B, int64_t ldb, std::complex<float> be,
+                       std::complex<float>* C, int64_t ldc, int ta = 0, int tb = 0) {
+  const float a[2] = {al.real(), al.imag()}, b[2] = {be.real(), be.imag()};
+  return bzg_cgemm_ex(context(), ta, tb, m, n, k, a, reinterpret_cast<const float*>(A), lda,
+                      reinterpret_cast<const float*>(B), ldb, b, reinterpret_cast<float*>(C), ldc);
+}
 // m x n = the STORED shape of A; ta: y = A^T x (y has n entries) instead of A x
 inline bzg_status gemv(int64_t m, int64_t n, double al, const double* A, int64_t lda,
                        const double* x, double be, double* y, bool ta = false) {
@@ -167,10 +185,33 @@ inline bzg_status gemv(int64_t m, int64_t n, int al, const int* A, int64_t lda, 
   if (ta) return bzg_igemv_t(context(), m, n, al, A, lda, x, be, y);
   return bzg_igemv(context(), m, n, al, A, lda, x, be, y);
 }
+inline bzg_status gemv(int64_t m, int64_t n, std::complex<double> al, const std::complex<double>* A, int64_t lda,
+                       const std::complex<double>* x, std::complex<double> be, std::complex<double>* y,
+                       int ta = 0) {
+  const double a[2] = {al.real(), al.imag()}, b[2] = {be.real(), be.imag()};
+  return bzg_zgemv_ex(context(), ta, m, n, a, reinterpret_cast<const double*>(A), lda,
+                      reinterpret_cast<const double*>(x), b, reinterpret_cast<double*>(y));
+}
+inline bzg_status gemv(int64_t m, int64_t n, std::complex<float> al, const std::complex<float>* A, int64_t lda,
+                       const std::complex<float>* x, std::complex<float> be, std::complex<float>* y, int ta = 0) {
+  const float a[2] = {al.real(), al.imag()}, b[2] = {be.real(), be.imag()};
+  return bzg_cgemv_ex(context(), ta, m, n, a, reinterpret_cast<const float*>(A), lda,
+                      reinterpret_cast<const float*>(x), b, reinterpret_cast<float*>(y));
+}
 // y += alpha x over n elements (the aliased accumulations' element-wise add)
 inline bzg_status axpy(size_t n, double al, const double* x, double* y) { return bzg_daxpy(context(), n, al, x, y); }
 inline bzg_status axpy(size_t n, float al, const float* x, float* y) { return bzg_saxpy(context(), n, al, x, y); }
 inline bzg_status axpy(size_t n, int al, const int* x, int* y) { return bzg_iaxpy(context(), n, al, x, y); }
+// complex y += s x for a real s (the +1 / -1 of the aliased accumulations): the real kernel
+// over the 2n interleaved parts
+inline bzg_status axpy(size_t n, std::complex<double> al, const std::complex<double>* x, std::complex<double>* y) {
+  if (al.imag() != 0.0) throw std::invalid_argument("axpy: complex scale");
+  return bzg_daxpy(context(), 2 * n, al.real(), reinterpret_cast<const double*>(x), reinterpret_cast<double*>(y));
+}
+inline bzg_status axpy(size_t n, std::complex<float> al, const std::complex<float>* x, std::complex<float>* y) {
+  if (al.imag() != 0.0f) throw std::invalid_argument("axpy: complex scale");
+  return bzg_saxpy(context(), 2 * n, al.real(), reinterpret_cast<const float*>(x), reinterpret_cast<float*>(y));
+}
 inline bzg_status csrmv(const bzg_csr_plan* p, const int32_t* rp, const int32_t* ci,
                         const double* v, const double* x, double* y) {
   return bzg_dcsrmv(context(), p, rp, ci, v, x, y);
@@ -205,7 +246,7 @@ struct DMatDMatMultExpr {
   const DynamicMatrix<T>& lhs;
   const DynamicMatrix<T>& rhs;
   T scale;
-  bool ta = false, tb = false;  // lhs / rhs used transposed (trans(A) * B and friends)
+  int ta = 0, tb = 0;  // lhs / rhs op: 0 as stored, 1 trans(), 2 ctrans() (trans(A) * B and friends)
   size_t rows() const { return ta ? lhs.columns() : lhs.rows(); }
   size_t columns() const { return tb ? rhs.rows() : rhs.columns(); }
   size_t inner() const { return ta ? lhs.rows() : lhs.columns(); }
@@ -214,13 +255,15 @@ struct DMatDMatMultExpr {
 template <typename T>
 struct DMatTransExpr {
   const DynamicMatrix<T>& mat;
+  bool conj = false;  // ctrans(A): the conjugate transpose (DMatMapExpr.h:1501)
+  int op() const { return conj ? 2 : 1; }
 };
 template <typename T>
 struct DMatDVecMultExpr {
   const DynamicMatrix<T>& mat;
   const DynamicVector<T>& vec;
   T scale;
-  bool ta = false;  // trans(A) * x (TDMatDVecMultExpr)
+  int ta = 0;  // 1: trans(A) * x (TDMatDVecMultExpr), 2: ctrans(A) * x
   size_t size() const { return ta ? mat.columns() : mat.rows(); }
 };
 template <typename T>
@@ -251,8 +294,9 @@ struct DMatDVecSolveExpr {
 template <typename T, bool SO>
 class DynamicMatrix {  // row-major (the column-major form is the specialisation below)
   static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value ||
-                    std::is_same<T, int>::value,
-                "blaze::gpu::DynamicMatrix supports double, float and int");
+                    std::is_same<T, int>::value || std::is_same<T, std::complex<double>>::value ||
+                    std::is_same<T, std::complex<float>>::value,
+                "blaze::gpu::DynamicMatrix supports double, float, int, complex<double> and complex<float>");
 
  public:
   using ElementType = T;
@@ -773,29 +817,34 @@ inline DMatDMatMultExpr<T> operator*(const DMatDMatMultExpr<T>& e, T s) {
 }
 template <typename T>
 inline DMatTransExpr<T> trans(const DynamicMatrix<T>& A) {
-  return {A};
+  return {A, false};
+}
+// ctrans(A) = conj(trans(A)) as an operand of a product (for real elements, trans(A))
+template <typename T>
+inline DMatTransExpr<T> ctrans(const DynamicMatrix<T>& A) {
+  return {A, true};
 }
 // trans(A) * B, A * trans(B), trans(A) * trans(B) (TDMatDMatMultExpr, DMatTDMatMultExpr,
 // TDMatTDMatMultExpr): one bzg_*gemm_ex call on the stored operands, no transposed copies
 template <typename T>
 inline DMatDMatMultExpr<T> operator*(const DMatTransExpr<T>& A, const DynamicMatrix<T>& B) {
   if (A.mat.rows() != B.rows()) throw std::invalid_argument("Matrix sizes do not match");
-  return {A.mat, B, T(1), true, false};
+  return {A.mat, B, T(1), A.op(), 0};
 }
 template <typename T>
 inline DMatDMatMultExpr<T> operator*(const DynamicMatrix<T>& A, const DMatTransExpr<T>& B) {
   if (A.columns() != B.mat.columns()) throw std::invalid_argument("Matrix sizes do not match");
-  return {A, B.mat, T(1), false, true};
+  return {A, B.mat, T(1), 0, B.op()};
 }
 template <typename T>
 inline DMatDMatMultExpr<T> operator*(const DMatTransExpr<T>& A, const DMatTransExpr<T>& B) {
   if (A.mat.rows() != B.mat.columns()) throw std::invalid_argument("Matrix sizes do not match");
-  return {A.mat, B.mat, T(1), true, true};
+  return {A.mat, B.mat, T(1), A.op(), B.op()};
 }
 template <typename T>
 inline DMatDVecMultExpr<T> operator*(const DMatTransExpr<T>& A, const DynamicVector<T>& x) {
   if (A.mat.rows() != x.size()) throw std::invalid_argument("Matrix and vector sizes do not match");
-  return {A.mat, x, T(1), true};
+  return {A.mat, x, T(1), A.op()};
 }
 template <typename T>
 inline DMatDVecMultExpr<T> operator*(const DynamicMatrix<T>& A, const DynamicVector<T>& x) {
@@ -909,6 +958,48 @@ inline void solve(const DynamicMatrix<double>& A, DynamicMatrix<double>& X, cons
 }
 
 inline DMatDVecSolveExpr<double> solve(const DynamicMatrix<double>& A, const DynamicVector<double>& b) {
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
+  if (A.rows() != b.size()) throw std::invalid_argument("Invalid right-hand side vector provided");
+  return {A, b};
+}
+
+// complex<double> systems of any order up to 16384 (LSE.h:6343-6373 -> gesv -> zgesv_,
+// clapack/gesv.h:317-333): bzg_zgesv, the real 2n x 2n embedding on the FP64 LU.
+namespace detail {
+inline void zsolve_into(const DynamicMatrix<std::complex<double>>& A, std::complex<double>* B_dev, size_t nrhs) {
+  const size_t n = A.rows();
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
+  if (n == 0) return;
+  DeviceBuffer<int32_t> dinfo(1);
+  check(bzg_zgesv(context(), n, nrhs, reinterpret_cast<const double*>(A.data()), n,
+                  reinterpret_cast<double*>(B_dev), std::max<size_t>(nrhs, 1), dinfo.ptr),
+        "solve");
+  int32_t info = 0;
+  d2h(&info, dinfo.ptr, 1);
+  if (info > 0) throw_singular(n);
+}
+}  // namespace detail
+
+inline void solve(const DynamicMatrix<std::complex<double>>& A, DynamicVector<std::complex<double>>& x,
+                  const DynamicVector<std::complex<double>>& b) {
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
+  if (A.rows() != b.size()) throw std::invalid_argument("Invalid right-hand side vector provided");
+  DynamicVector<std::complex<double>> tmp(b);
+  detail::zsolve_into(A, tmp.data(), 1);
+  x.swap(tmp);
+}
+
+inline void solve(const DynamicMatrix<std::complex<double>>& A, DynamicMatrix<std::complex<double>>& X,
+                  const DynamicMatrix<std::complex<double>>& B) {
+  if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
+  if (A.rows() != B.rows()) throw std::invalid_argument("Invalid right-hand side matrix provided");
+  DynamicMatrix<std::complex<double>> tmp(B);
+  detail::zsolve_into(A, tmp.data(), B.columns());
+  X.swap(tmp);
+}
+
+inline DMatDVecSolveExpr<std::complex<double>> solve(const DynamicMatrix<std::complex<double>>& A,
+                                                     const DynamicVector<std::complex<double>>& b) {
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square system matrix provided");
   if (A.rows() != b.size()) throw std::invalid_argument("Invalid right-hand side vector provided");
   return {A, b};
@@ -1035,7 +1126,8 @@ inline DynamicMatrix<double> inv(const DynamicMatrix<double>& A) {
 
 template <typename T>
 DynamicVector<T>& DynamicVector<T>::operator=(const DMatDVecSolveExpr<T>& e) {
-  static_assert(std::is_same<T, double>::value, "solve() is provided for double");
+  static_assert(std::is_same<T, double>::value || std::is_same<T, std::complex<double>>::value,
+                "solve() is provided for double and complex<double>");
   solve(e.mat, *this, e.rhs);
   return *this;
 }

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -798,6 +799,95 @@ int main() {
     auto zz = z.toVector(), yy = y.toVector();
     for (size_t i = 0; i < n; ++i) ok = ok && yy[i] == c[i] + zz[i];
     check(ok, "aliased C -= A * C and y += A * y (device add of the evaluated product)");
+  }
+  // complex<double> / complex<float> elements: products with trans() / ctrans(), scaled and
+  // accumulated products, GEMV with ctrans(A), and solve() of a complex system
+  {
+    using cd = std::complex<double>;
+    using cf = std::complex<float>;
+    auto close = [](cd a, cd b, double eps) { return std::abs(a - b) <= eps * std::max(1.0, std::abs(b)); };
+    const size_t m = 37, k = 53, n = 29;
+    std::vector<cd> a(m * k), b(k * n), bh(n * k), at(k * m);
+    for (auto& v : a) v = cd(U(rng), U(rng));
+    for (auto& v : b) v = cd(U(rng), U(rng));
+    for (size_t p = 0; p < k; ++p)
+      for (size_t j = 0; j < n; ++j) bh[j * k + p] = std::conj(b[p * n + j]);
+    for (size_t i = 0; i < m; ++i)
+      for (size_t p = 0; p < k; ++p) at[p * m + i] = a[i * k + p];
+    DynamicMatrix<cd> A(m, k, a.data()), B(k, n, b.data()), Bh(n, k, bh.data()), At(k, m, at.data()), C1, C2, C3;
+    C1 = A * B;
+    C2 = A * ctrans(Bh);
+    C3 = trans(At) * B;
+    const auto ref = host_gemm(a, b, m, n, k);
+    auto c1 = C1.toVector(), c2 = C2.toVector(), c3 = C3.toVector();
+    bool ok = C1.rows() == m && C1.columns() == n;
+    for (size_t q = 0; q < m * n; ++q)
+      ok = ok && close(c1[q], ref[q], 1e-13) && close(c2[q], ref[q], 1e-13) && close(c3[q], ref[q], 1e-13);
+    check(ok, "complex<double>: A * B == A * ctrans(B^H) == trans(A^T) * B (37 x 53 x 29)");
+    const cd s(0.5, -1.0);
+    C2 = s * (A * B);
+    C2 -= A * B;
+    c2 = C2.toVector();
+    ok = true;
+    for (size_t q = 0; q < m * n; ++q) ok = ok && close(c2[q], (s - 1.0) * ref[q], 1e-13);
+    check(ok, "complex<double>: C = s * (A * B); C -= A * B");
+    std::vector<cd> xv(m);
+    for (auto& v : xv) v = cd(U(rng), U(rng));
+    DynamicVector<cd> x(m), y;
+    x.upload(xv.data());
+    y = ctrans(A) * x;
+    auto yv = y.toVector();
+    ok = y.size() == k;
+    for (size_t p = 0; p < k; ++p) {
+      cd t(0.0, 0.0);
+      for (size_t i = 0; i < m; ++i) t += std::conj(a[i * k + p]) * xv[i];
+      ok = ok && close(yv[p], t, 1e-13);
+    }
+    check(ok, "complex<double>: y = ctrans(A) * x");
+    // solve(): N = 300 and N = 5 (a closed-form order in the reference), one and three RHS
+    for (size_t N : {size_t(5), size_t(300)}) {
+      std::vector<cd> g(N * N), r(N * 3);
+      for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) g[i * N + j] = cd(U(rng), U(rng)) + (i == j ? cd(double(N), 0.0) : cd(0.0));
+      for (auto& v : r) v = cd(U(rng), U(rng));
+      DynamicMatrix<cd> G(N, N, g.data()), Rm(N, 3, r.data()), X;
+      solve(G, X, Rm);
+      auto xs = X.toVector();
+      ok = X.rows() == N && X.columns() == 3;
+      for (size_t i = 0; i < N && ok; ++i)
+        for (size_t c = 0; c < 3; ++c) {
+          cd t(0.0, 0.0);
+          for (size_t j = 0; j < N; ++j) t += g[i * N + j] * xs[j * 3 + c];
+          ok = ok && close(t, r[i * 3 + c], 1e-12);
+        }
+      DynamicVector<cd> bv(N), xx;
+      std::vector<cd> r0(N);
+      for (size_t i = 0; i < N; ++i) r0[i] = r[i * 3];
+      bv.upload(r0.data());
+      xx = solve(G, bv);
+      auto x0 = xx.toVector();
+      for (size_t i = 0; i < N; ++i) ok = ok && close(x0[i], xs[i * 3], 1e-12);
+      check(ok, "complex<double>: solve(A, X, B) and x = solve(A, b), N = " + std::to_string(N));
+    }
+    {
+      std::vector<cd> g(49, cd(0.0));
+      for (size_t i = 0; i < 7; ++i) g[i * 7 + i] = cd(1.0, 1.0);
+      for (size_t i = 0; i < 7; ++i) g[i * 7 + 3] = cd(0.0);  // a zero column: singular
+      DynamicMatrix<cd> G(7, 7, g.data());
+      DynamicVector<cd> bv(7, cd(1.0, 0.0)), xx;
+      check(throws<std::runtime_error>([&] { solve(G, xx, bv); }, "Inversion of singular matrix failed"),
+            "complex<double>: singular solve() throws (N = 7)");
+    }
+    std::vector<cf> af(m * k), bf(k * n);
+    for (size_t q = 0; q < m * k; ++q) af[q] = cf(a[q]);
+    for (size_t q = 0; q < k * n; ++q) bf[q] = cf(b[q]);
+    DynamicMatrix<cf> Af(m, k, af.data()), Bf(k, n, bf.data()), Cf;
+    Cf = Af * Bf;
+    auto cfv = Cf.toVector();
+    const auto reff = host_gemm(af, bf, m, n, k);
+    ok = true;
+    for (size_t q = 0; q < m * n; ++q) ok = ok && std::abs(cfv[q] - reff[q]) <= 1e-4f * std::max(1.0f, std::abs(reff[q]));
+    check(ok, "complex<float>: A * B (37 x 53 x 29)");
   }
   std::printf("%d failure(s)\n", failures);
   return failures;
