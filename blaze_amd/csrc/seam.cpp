@@ -1,9 +1,12 @@
 // The reference's BLAS/LAPACK seam served by the device library: host-memory cblas_dgemm,
-// cblas_dgemv, dgesv_, dgetrf_ and dgetrs_ with the standard (LP64, 32-bit int) prototypes.
+// cblas_dgemv, their complex forms cblas_zgemm / cblas_cgemm / cblas_zgemv / cblas_cgemv, and
+// dgesv_, dgetrf_ and dgetrs_ with the standard (LP64, 32-bit int) prototypes.
 //
 // The reference reaches external BLAS/LAPACK through these calls only:
 //   cblas_dgemm  blaze/math/blas/cblas/gemm.h:156-162 (DMatDMatMultExpr's BLAS kernel)
 //   cblas_dgemv  blaze/math/blas/cblas/gemv.h:153     (DMatDVecMultExpr's BLAS kernel)
+//   cblas_zgemm / cblas_cgemm  blaze/math/blas/cblas/gemm.h:197-283 (complex elements)
+//   cblas_zgemv / cblas_cgemv  blaze/math/blas/cblas/gemv.h:186-245
 //   dgesv_       blaze/math/lapack/clapack/gesv.h:200-208 (solve() for N >= 7)
 //   dgetrf_      blaze/math/lapack/clapack/getrf.h:190-197 (lu(), det(), invert())
 //   dgetrs_      blaze/math/lapack/clapack/getrs.h:207-220
@@ -86,9 +89,148 @@ void scatter(int n, const std::vector<double>& v, double* y, int inc) {
   for (int i = 0; i < n; ++i) y[(inc < 0 ? (int64_t)(n - 1 - i) : (int64_t)i) * step] = v[(size_t)i];
 }
 
+// complex vectors (n entries of 2 reals), optionally conjugated on the way
+template <typename R>
+void cgather(int n, const R* x, int inc, bool conj, std::vector<R>& out) {
+  out.resize(2 * (size_t)n);
+  const int64_t step = inc < 0 ? -inc : inc;
+  for (int i = 0; i < n; ++i) {
+    const int64_t e = (inc < 0 ? (int64_t)(n - 1 - i) : (int64_t)i) * step;
+    out[2 * (size_t)i] = x[2 * e];
+    out[2 * (size_t)i + 1] = conj ? -x[2 * e + 1] : x[2 * e + 1];
+  }
+}
+template <typename R>
+void cscatter(int n, const std::vector<R>& v, R* y, int inc, bool conj) {
+  const int64_t step = inc < 0 ? -inc : inc;
+  for (int i = 0; i < n; ++i) {
+    const int64_t e = (inc < 0 ? (int64_t)(n - 1 - i) : (int64_t)i) * step;
+    y[2 * e] = v[2 * (size_t)i];
+    y[2 * e + 1] = conj ? -v[2 * (size_t)i + 1] : v[2 * (size_t)i + 1];
+  }
+}
+
+// CBLAS transpose codes to the ABI's op codes (0 N, 1 T, 2 C); -1: not a CBLAS code
+int op_code(int t) { return t == kNoTrans ? 0 : t == kTrans ? 1 : t == kConjTrans ? 2 : -1; }
+
+template <typename R>
+struct CxAbi;
+template <>
+struct CxAbi<double> {
+  static bzg_status gemm(bzg_context* c, int ta, int tb, int64_t m, int64_t n, int64_t k, const double* al,
+                         const double* A, int64_t lda, const double* B, int64_t ldb, const double* be, double* C,
+                         int64_t ldc) {
+    return bzg_zgemm_ex(c, ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+  }
+  static bzg_status gemv(bzg_context* c, int t, int64_t m, int64_t n, const double* al, const double* A,
+                         int64_t lda, const double* x, const double* be, double* y) {
+    return bzg_zgemv_ex(c, t, m, n, al, A, lda, x, be, y);
+  }
+};
+template <>
+struct CxAbi<float> {
+  static bzg_status gemm(bzg_context* c, int ta, int tb, int64_t m, int64_t n, int64_t k, const float* al,
+                         const float* A, int64_t lda, const float* B, int64_t ldb, const float* be, float* C,
+                         int64_t ldc) {
+    return bzg_cgemm_ex(c, ta, tb, m, n, k, al, A, lda, B, ldb, be, C, ldc);
+  }
+  static bzg_status gemv(bzg_context* c, int t, int64_t m, int64_t n, const float* al, const float* A,
+                         int64_t lda, const float* x, const float* be, float* y) {
+    return bzg_cgemv_ex(c, t, m, n, al, A, lda, x, be, y);
+  }
+};
+
+// Complex CBLAS gemm: the same row-major mapping as cblas_dgemm.  A column-major operand X is
+// the row-major X^T, and (op(A) op(B))^T = op(B)^T op(A)^T keeps each operand's op code: for
+// op = C, op(B)^T = conj(B) = (B^T)^H, the conjugate transpose of the stored row-major matrix.
+template <typename R>
+void cblas_cx_gemm(const char* name, int order, int transa, int transb, int m, int n, int k, const void* alpha,
+                   const void* A, int lda, const void* B, int ldb, const void* beta, void* C, int ldc) {
+  if (m <= 0 || n <= 0) return;
+  SeamContext& s = seam();
+  if (!s.ctx) fail(name, BZG_ERR_NO_DEVICE);
+  const int ta = op_code(transa), tb = op_code(transb);
+  if (ta < 0 || tb < 0) fail(name, BZG_ERR_INVALID_ARGUMENT);
+  const bool row = order == kRowMajor;
+  const int rm = row ? m : n, rn = row ? n : m;
+  const R* X = static_cast<const R*>(row ? A : B);
+  const R* Y = static_cast<const R*>(row ? B : A);
+  const int tx = row ? ta : tb, ty = row ? tb : ta;
+  const int ldx = row ? lda : ldb, ldy = row ? ldb : lda;
+  const size_t ex = k > 0 ? (tx ? extent(k, rm, ldx) : extent(rm, k, ldx)) : 0;
+  const size_t ey = k > 0 ? (ty ? extent(rn, k, ldy) : extent(k, rn, ldy)) : 0;
+  const size_t ec = extent(rm, rn, ldc);
+  const size_t es = 2 * sizeof(R);
+  R* dX = static_cast<R*>(s.get(0, ex * es));
+  R* dY = static_cast<R*>(s.get(1, ey * es));
+  R* dC = static_cast<R*>(s.get(2, ec * es));
+  if (!dX || !dY || !dC) fail(name, BZG_ERR_OUT_OF_MEMORY);
+  const R* al = static_cast<const R*>(alpha);
+  const R* be = static_cast<const R*>(beta);
+  if (ex) SEAM_CHECK(bzg_memcpy(s.ctx, dX, X, ex * es, 0), name);
+  if (ey) SEAM_CHECK(bzg_memcpy(s.ctx, dY, Y, ey * es, 0), name);
+  if (be[0] != R(0) || be[1] != R(0) || ldc > rn) SEAM_CHECK(bzg_memcpy(s.ctx, dC, C, ec * es, 0), name);
+  SEAM_CHECK(CxAbi<R>::gemm(s.ctx, tx, ty, rm, rn, k, al, dX, ldx, dY, ldy, be, dC, ldc), name);
+  SEAM_CHECK(bzg_memcpy(s.ctx, C, dC, ec * es, 1), name);
+}
+
+// Complex CBLAS gemv.  Row-major: the stored A with the given op.  Column-major: the stored
+// row-major matrix S = A^T, so A x = S^T x (op T), A^T x = S x (op N), and A^H x = conj(S) x,
+// computed as conj(conj(alpha) S conj(x) + conj(beta) conj(y)) (conjugations on the host copies).
+template <typename R>
+void cblas_cx_gemv(const char* name, int order, int trans, int m, int n, const void* alpha, const void* A, int lda,
+                   const void* x, int incx, const void* beta, void* y, int incy) {
+  if (m <= 0 || n <= 0) return;
+  SeamContext& s = seam();
+  if (!s.ctx) fail(name, BZG_ERR_NO_DEVICE);
+  const int t = op_code(trans);
+  if (t < 0) fail(name, BZG_ERR_INVALID_ARGUMENT);
+  const bool row = order == kRowMajor;
+  const int sm = row ? m : n, sn = row ? n : m;
+  const bool conj_all = !row && t == 2;
+  const int dev_op = row ? t : (t == 0 ? 1 : 0);
+  const int lx = t ? m : n, ly = t ? n : m;
+  const R* al = static_cast<const R*>(alpha);
+  const R* be = static_cast<const R*>(beta);
+  const R alc[2] = {al[0], conj_all ? -al[1] : al[1]}, bec[2] = {be[0], conj_all ? -be[1] : be[1]};
+  const bool use_y = be[0] != R(0) || be[1] != R(0);
+  std::vector<R> hx, hy;
+  cgather(lx, static_cast<const R*>(x), incx, conj_all, hx);
+  if (use_y) cgather(ly, static_cast<const R*>(y), incy, conj_all, hy);
+  else hy.assign(2 * (size_t)ly, R(0));
+  const size_t es = 2 * sizeof(R), ea = extent(sm, sn, lda);
+  R* dA = static_cast<R*>(s.get(0, ea * es));
+  R* dx = static_cast<R*>(s.get(1, (size_t)lx * es));
+  R* dy = static_cast<R*>(s.get(2, (size_t)ly * es));
+  if (!dA || !dx || !dy) fail(name, BZG_ERR_OUT_OF_MEMORY);
+  SEAM_CHECK(bzg_memcpy(s.ctx, dA, A, ea * es, 0), name);
+  SEAM_CHECK(bzg_memcpy(s.ctx, dx, hx.data(), (size_t)lx * es, 0), name);
+  if (use_y) SEAM_CHECK(bzg_memcpy(s.ctx, dy, hy.data(), (size_t)ly * es, 0), name);
+  SEAM_CHECK(CxAbi<R>::gemv(s.ctx, dev_op, sm, sn, alc, dA, lda, dx, bec, dy), name);
+  SEAM_CHECK(bzg_memcpy(s.ctx, hy.data(), dy, (size_t)ly * es, 1), name);
+  cscatter(ly, hy, static_cast<R*>(y), incy, conj_all);
+}
+
 }  // namespace
 
 extern "C" {
+
+void cblas_zgemm(int order, int transa, int transb, int m, int n, int k, const void* alpha, const void* A, int lda,
+                 const void* B, int ldb, const void* beta, void* C, int ldc) {
+  cblas_cx_gemm<double>("cblas_zgemm", order, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+void cblas_cgemm(int order, int transa, int transb, int m, int n, int k, const void* alpha, const void* A, int lda,
+                 const void* B, int ldb, const void* beta, void* C, int ldc) {
+  cblas_cx_gemm<float>("cblas_cgemm", order, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+void cblas_zgemv(int order, int trans, int m, int n, const void* alpha, const void* A, int lda, const void* x,
+                 int incx, const void* beta, void* y, int incy) {
+  cblas_cx_gemv<double>("cblas_zgemv", order, trans, m, n, alpha, A, lda, x, incx, beta, y, incy);
+}
+void cblas_cgemv(int order, int trans, int m, int n, const void* alpha, const void* A, int lda, const void* x,
+                 int incx, const void* beta, void* y, int incy) {
+  cblas_cx_gemv<float>("cblas_cgemv", order, trans, m, n, alpha, A, lda, x, incx, beta, y, incy);
+}
 
 // C = alpha op(A) op(B) + beta C (CBLAS).  Row-major: the device call as given.  Column-major:
 // C^T = alpha op(B)^T op(A)^T + beta C^T on the row-major transposes, i.e. the same call with

@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 ROW, COL = 101, 102
-NT, TR = 111, 112
+NT, TR, CT = 111, 112, 113
 ci = C.c_int
 cd = C.c_double
 vp = C.c_void_p
@@ -18,6 +18,12 @@ def bind(lib):
     lib.cblas_dgemv.argtypes = [ci, ci, ci, ci, cd, vp, ci, vp, ci, cd, vp, ci]
     lib.cblas_dgemv.restype = None
     for f in (lib.dgetrf_, lib.dgesv_, lib.dgetrs_):
+        f.restype = None
+    for f in (lib.cblas_zgemm, lib.cblas_cgemm):
+        f.argtypes = [ci, ci, ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
+        f.restype = None
+    for f in (lib.cblas_zgemv, lib.cblas_cgemv):
+        f.argtypes = [ci, ci, ci, ci, vp, vp, ci, vp, ci, vp, vp, ci]
         f.restype = None
     return lib
 
@@ -143,3 +149,75 @@ def check_lapack(lib, n, nrhs, seed, tol):
     lib.dgetrs_(b"T", C.byref(ci(n)), C.byref(ci(nrhs)), _p(sA), C.byref(ci(lda)), _p(ipiv), _p(sB),
                 C.byref(ci(ldb)), C.byref(info), C.c_size_t(1))
     assert info.value == -1
+
+
+def _op(X, t):
+    return X if t == NT else (X.T if t == TR else X.conj().T)
+
+
+def cstore(X, order, ld, dtype):
+    """Flat complex storage of X (r x c) in the given order with leading dimension ld."""
+    r, c = X.shape
+    S = np.full((r, ld) if order == ROW else (c, ld), 7.25 - 1.5j, dtype)
+    if order == ROW:
+        S[:, :c] = X
+    else:
+        S[:, :r] = X.T
+    return S.ravel().copy()
+
+
+def check_cgemm(lib, m, n, k, seed, tol, dtype=np.complex128):
+    fn = lib.cblas_zgemm if dtype == np.complex128 else lib.cblas_cgemm
+    rdt = np.float64 if dtype == np.complex128 else np.float32
+    rng = np.random.default_rng(seed)
+    z = lambda *sh: (rng.uniform(-1, 1, sh) + 1j * rng.uniform(-1, 1, sh)).astype(dtype)  # noqa: E731
+    alpha = np.array([1.5, -0.5], rdt)
+    for order in (ROW, COL):
+        for ta in (NT, TR, CT):
+            for tb in (NT, TR, CT):
+                for beta in (0.0, -0.5 + 0.25j):
+                    A = z(*((k, m) if ta != NT else (m, k)))
+                    B = z(*((n, k) if tb != NT else (k, n)))
+                    C0 = z(m, n)
+                    lda = A.shape[1 if order == ROW else 0] + 3
+                    ldb = B.shape[1 if order == ROW else 0] + 2
+                    ldc = (n if order == ROW else m) + 1
+                    sA, sB, sC = cstore(A, order, lda, dtype), cstore(B, order, ldb, dtype), cstore(C0, order, ldc, dtype)
+                    if beta == 0.0:
+                        sC[:] = np.nan
+                    be = np.array([beta.real, beta.imag] if beta != 0.0 else [0.0, 0.0], rdt)
+                    fn(order, ta, tb, m, n, k, _p(alpha), _p(sA), lda, _p(sB), ldb, _p(be), _p(sC), ldc)
+                    want = (1.5 - 0.5j) * (_op(A, ta).astype(np.complex128) @ _op(B, tb)) + (
+                        beta * C0 if beta != 0.0 else 0.0)
+                    got = load(sC, order, m, n, ldc)
+                    assert np.abs(got - want).max() <= tol * max(1.0, np.abs(want).max()), (order, ta, tb, beta)
+                    if beta == 0.0:
+                        pad = sC.reshape(-1, ldc)[:, (n if order == ROW else m):]
+                        assert np.isnan(pad).all()
+
+
+def check_cgemv(lib, m, n, seed, tol, dtype=np.complex128):
+    fn = lib.cblas_zgemv if dtype == np.complex128 else lib.cblas_cgemv
+    rdt = np.float64 if dtype == np.complex128 else np.float32
+    rng = np.random.default_rng(seed)
+    z = lambda *sh: (rng.uniform(-1, 1, sh) + 1j * rng.uniform(-1, 1, sh)).astype(dtype)  # noqa: E731
+    alpha = np.array([0.75, 0.5], rdt)
+    for order in (ROW, COL):
+        for t in (NT, TR, CT):
+            for incx, incy, beta in ((1, 1, 2.0 - 1.0j), (2, -1, 0.0), (-3, 2, 0.5j)):
+                A = z(m, n)
+                lda = (n if order == ROW else m) + 2
+                sA = cstore(A, order, lda, dtype)
+                lx, ly = (n, m) if t == NT else (m, n)
+                x, y0 = z(lx), z(ly)
+                bx = np.full(max(1, 1 + (lx - 1) * abs(incx)), 3.5 + 0.5j, dtype)
+                by = np.full(max(1, 1 + (ly - 1) * abs(incy)), 3.5 + 0.5j, dtype)
+                for i in range(lx):
+                    bx[(lx - 1 - i if incx < 0 else i) * abs(incx)] = x[i]
+                for i in range(ly):
+                    by[(ly - 1 - i if incy < 0 else i) * abs(incy)] = y0[i]
+                be = np.array([beta.real, beta.imag] if beta != 0.0 else [0.0, 0.0], rdt)
+                fn(order, t, m, n, _p(alpha), _p(sA), lda, _p(bx), incx, _p(be), _p(by), incy)
+                want = (0.75 + 0.5j) * (_op(A, t).astype(np.complex128) @ x) + (beta * y0 if beta != 0.0 else 0.0)
+                got = np.array([by[(ly - 1 - i if incy < 0 else i) * abs(incy)] for i in range(ly)])
+                assert np.abs(got - want).max() <= tol * max(1.0, np.abs(want).max()), (order, t, incx, incy)

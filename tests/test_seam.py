@@ -1,7 +1,8 @@
 """CPU: the BLAS/LAPACK seam's argument mapping (blaze_amd/csrc/seam.cpp), linked against a host
 stand-in of the ABI functions it calls (tests/seam_stub.cpp), against numpy and LAPACK
 (scipy): CBLAS row/column-major with every transpose, padded leading dimensions, vector strides
-(negative too), LAPACK pivots, factors, solves, info codes."""
+(negative too), the complex forms with conjugate transposes, LAPACK pivots, factors, solves,
+info codes."""
 import ctypes as C
 import shutil
 import subprocess
@@ -35,3 +36,15 @@ def test_cblas_dgemv_mapping(stub_seam):
 
 def test_lapack_mapping(stub_seam):
     sc.check_lapack(stub_seam, 9, 3, seed=3, tol=1e-12)
+
+
+def test_cblas_complex_gemm_mapping(stub_seam):
+    import numpy as np
+    sc.check_cgemm(stub_seam, 7, 5, 4, seed=4, tol=1e-13)
+    sc.check_cgemm(stub_seam, 6, 3, 5, seed=5, tol=1e-5, dtype=np.complex64)
+
+
+def test_cblas_complex_gemv_mapping(stub_seam):
+    import numpy as np
+    sc.check_cgemv(stub_seam, 6, 9, seed=6, tol=1e-13)
+    sc.check_cgemv(stub_seam, 5, 4, seed=7, tol=1e-5, dtype=np.complex64)

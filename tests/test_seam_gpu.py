@@ -1,4 +1,5 @@
-"""GPU: the BLAS/LAPACK seam library (blaze_amd/lib/libblaze_seam.so: cblas_dgemm, cblas_dgemv,
+"""GPU: the BLAS/LAPACK seam library (blaze_amd/lib/libblaze_seam.so: cblas_dgemm, cblas_dgemv, the
+complex cblas_zgemm / cblas_cgemm / cblas_zgemv / cblas_cgemv,
 dgetrf_, dgetrs_, dgesv_ on host memory, computed by the device library) against numpy and
 LAPACK (scipy) for every storage order, transpose, leading dimension and stride."""
 import ctypes as C
@@ -30,3 +31,15 @@ def test_cblas_dgemv(seam):
 
 def test_lapack(seam):
     sc.check_lapack(seam, 700, 5, seed=13, tol=1e-10)
+
+
+def test_cblas_complex_gemm(seam):
+    import numpy as np
+    sc.check_cgemm(seam, 130, 70, 90, seed=14, tol=1e-13)
+    sc.check_cgemm(seam, 66, 40, 50, seed=15, tol=2e-5, dtype=np.complex64)
+
+
+def test_cblas_complex_gemv(seam):
+    import numpy as np
+    sc.check_cgemv(seam, 300, 170, seed=16, tol=1e-13)
+    sc.check_cgemv(seam, 120, 90, seed=17, tol=2e-5, dtype=np.complex64)
