@@ -22,10 +22,13 @@ Beside it (fields of the same JSON line):
   spmv             CSR SpMV GB/s (algorithmic bytes) for the 128^3 7-point Laplacian (fp64,
                    fp32) and the n = 1e6 power-law matrix, rotating copies past the 256 MB MALL;
   gemv, cg, serve  dense GEMV, the CG loop, the continuous-batching server (p50/p99 per class);
+  complex          complex<double> / complex<float> GEMM at 4096^3 (A B and A^H B) and
+                   complex<double> GEMV at 16384^2, with Blaze's complex GEMM on the host;
   roofline         dominant kernel (the residue-slice GEMM) from HIP events on its stream;
   cpu_baseline     the reference's own GEMM (Blaze, compiled from its headers into oracle/_ref)
                    on the host cores at the headline size; every other leg carries its own
-                   `cpu_baseline` (Blaze GEMM 512^3, SpMV, GEMV, solve() with LAPACK).
+                   `cpu_baseline` (Blaze GEMM 512^3, SpMV, GEMV, solve() with LAPACK, complex
+                   GEMM).
 """
 from __future__ import annotations
 
@@ -510,6 +513,51 @@ class Bench:
         return out
 
     # -- conjugate gradients (the reference's CG benchmark loop, blazemark CG.cpp) -------------
+    def complex_leg(self, n=4096, mv=16384, reps=5):
+        """complex<double> / complex<float> products at n^3 (8 n^3 real flops each; op pairs N.N and
+        H.N) and complex<double> GEMV at mv x mv (16 bytes per element of A read once), timed with
+        events on the library's stream; uniform[-1,1) real and imaginary parts."""
+        torch, bz = self.torch, self.bz
+        ctx = bz.Context(stream=torch.cuda.current_stream().cuda_stream)
+        out = {}
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+        def timed(f):
+            f()
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(reps):
+                f()
+            ev[1].record()
+            torch.cuda.synchronize()
+            return ev[0].elapsed_time(ev[1]) / reps / 1e3
+        for dt, fn, peak in ((torch.float64, "bzg_zgemm_ex", PEAK["fp64"]), (torch.float32, "bzg_cgemm_ex", PEAK["fp32"])):
+            A = torch.empty(2 * n * n, dtype=dt, device="cuda").uniform_(-1, 1)
+            B = torch.empty(2 * n * n, dtype=dt, device="cuda").uniform_(-1, 1)
+            Cm = torch.empty(2 * n * n, dtype=dt, device="cuda")
+            one = np.array([1.0, 0.0], np.float64 if dt == torch.float64 else np.float32)
+            zero = np.zeros(2, one.dtype)
+            for opa, tag in ((0, "nn"), (2, "hn")):
+                t = timed(lambda: ctx.call(fn, opa, 0, n, n, n, one.ctypes.data_as(C.c_void_p), vp(A), n, vp(B), n,
+                                           zero.ctypes.data_as(C.c_void_p), vp(Cm), n))
+                fl = 8.0 * n ** 3
+                out[f"{fn[4]}gemm_{tag}_{n}"] = {"ms": t * 1e3, "tflops": fl / t / 1e12, "frac_peak": fl / t / 1e12 / peak}
+            if dt == torch.float64:
+                self.zgemm_inputs = (A.cpu().numpy(), B.cpu().numpy(), out[f"zgemm_nn_{n}"]["tflops"])
+            del A, B, Cm
+        A = torch.empty(2 * mv * mv, dtype=torch.float64, device="cuda").uniform_(-1, 1)
+        x = torch.empty(2 * mv, dtype=torch.float64, device="cuda").uniform_(-1, 1)
+        y = torch.empty(2 * mv, dtype=torch.float64, device="cuda")
+        one = np.array([1.0, 0.0]); zero = np.zeros(2)
+        by = 16 * (mv * mv + 2 * mv)
+        for op, tag in ((0, "n"), (2, "h")):
+            t = timed(lambda: ctx.call("bzg_zgemv_ex", op, mv, mv, one.ctypes.data_as(C.c_void_p), vp(A), mv, vp(x),
+                                       zero.ctypes.data_as(C.c_void_p), vp(y)))
+            out[f"zgemv_{tag}_{mv}"] = {"us": t * 1e6, "gbps": by / t / 1e9, "frac_hbm": by / t / 1e9 / PEAK["hbm"]}
+        del A
+        ctx.close()
+        return out
+
     def cg_leg(self, N=2048, iterations=50, reps=5):
         """5-point matrix N^2 x N^2, b = 0, random start (as blazemark); the whole bzg_dcg call
         (SpMV + 3 streaming kernels per iteration) captured in a HIP graph and replayed."""
@@ -686,6 +734,25 @@ class Bench:
                 "sample": f"Blaze DynamicMatrix<double,rowMajor> {m}x{n} y = A * x, uniform[-1,1), {proto}, 5 timed runs"}
             gv["dgemv_16384x16384"]["gpu_over_cpu"] = gv["dgemv_16384x16384"]["gbps"] / (by / tg / 1e9)
             del Ag
+        # complex<double> GEMM: Blaze's own complex kernel at 1024^3 (a bounded sample: the CPU
+        # product at the GPU leg's 4096^3 would take minutes), the leading 1024 x 1024 blocks of
+        # the GPU leg's operands
+        cx = rec.get("complex", {})
+        zin = getattr(self, "zgemm_inputs", None)
+        if zin is not None and isinstance(cx, dict):
+            Az, Bz, gpu_tf = zin
+            nz = int(round((Az.size // 2) ** 0.5))
+            s = 1024
+            Azs = np.ascontiguousarray(Az.view(np.complex128).reshape(nz, nz)[:s, :s])
+            Bzs = np.ascontiguousarray(Bz.view(np.complex128).reshape(nz, nz)[:s, :s])
+            log(f"[cpu] Blaze complex<double> C = A * B {s}^3 on {cores} cores")
+            tz = L.ref_time_zgemm(C.c_int64(s), C.c_int64(s), C.c_int64(s), ptr(Azs), ptr(Bzs), C.c_int(2))
+            cx["cpu_baseline"] = {"value": 8.0 * s ** 3 / tz / 1e12, "unit": "TFLOP/s", "ms": tz * 1e3, "cores": cores,
+                                  "kind": "reference",
+                                  "sample": f"Blaze DynamicMatrix<complex<double>,rowMajor> C = A * B {s}^3 (the leading "
+                                            f"blocks of the GPU leg's operands), {proto}, 2 timed runs"}
+            cx["gpu_over_cpu"] = gpu_tf / cx["cpu_baseline"]["value"]
+        self.zgemm_inputs = None
         # batched solves: the reference's solve(A, X, B) per system (closed forms n <= 6, LAPACK
         # dgesv_ above), OpenMP over the systems, LAPACK sequential
         sv = rec.get("solve", {})
@@ -813,9 +880,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--systems", type=int, default=100_000)
-    ap.add_argument("--legs", default="gemm,solve,spmv,gemv,cg,serve,cpu",
-                    help="comma list of gemm, solve, spmv, gemv, cg, serve, cpu, copy (1 GiB copy for counter "
-                         "calibration)")
+    ap.add_argument("--legs", default="gemm,solve,spmv,gemv,complex,cg,serve,cpu",
+                    help="comma list of gemm, solve, spmv, gemv, complex, cg, serve, cpu, copy (1 GiB copy for "
+                         "counter calibration)")
     ap.add_argument("--variant", default="auto", choices=["auto", "bf16", "int8"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -850,6 +917,12 @@ def main():
             log(f"[gemv] {json.dumps(rec['gemv'])}")
         except Exception as e:
             rec["gemv"] = {"error": repr(e)}
+    if "complex" in legs:
+        try:
+            rec["complex"] = b.complex_leg()
+            log(f"[complex] {json.dumps(rec['complex'])}")
+        except Exception as e:
+            rec["complex"] = {"error": repr(e)}
     if "cg" in legs:
         try:
             rec["cg"] = b.cg_leg()

@@ -94,7 +94,7 @@ def ref() -> C.CDLL:
         L = C.CDLL(str(REF_SO))
         L.ref_dsolve_small.restype = i32
         L.ref_get_threads.restype = i32
-        for n in ("ref_time_dgemm", "ref_time_dcsrmv", "ref_time_dgemv"):
+        for n in ("ref_time_dgemm", "ref_time_dcsrmv", "ref_time_dgemv", "ref_time_zgemm"):
             getattr(L, n).restype = f64
         _ref = L
     return _ref

@@ -195,6 +195,22 @@ double ref_time_dgemm(int64_t m, int64_t n, int64_t k, const double* A, const do
   return best;
 }
 
+// best of `reps` timed C = A * B for complex<double> (after one untimed evaluation)
+double ref_time_zgemm(int64_t m, int64_t n, int64_t k, const zd* A, const zd* B, int reps) {
+  blaze::DynamicMatrix<zd, rowMajor> a, b, c(m, n);
+  to_blaze(a, m, k, A);
+  to_blaze(b, k, n, B);
+  c = a * b;
+  double best = 1e300;
+  for (int r = 0; r < reps; ++r) {
+    const double t0 = now();
+    c = a * b;
+    const double t = now() - t0;
+    if (t < best) best = t;
+  }
+  return best;
+}
+
 double ref_time_dcsrmv(int64_t nr, int64_t nc, const int32_t* rp, const int32_t* ci, const double* v,
                        const double* x, int reps) {
   blaze::CompressedMatrix<double, rowMajor> S;
