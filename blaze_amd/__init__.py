@@ -79,6 +79,7 @@ SIGNATURES = {
     "bzg_zgemv_ex": (i32, [vp, i32, i64, i64, vp, vp, i64, vp, vp, vp]),
     "bzg_cgemv_ex": (i32, [vp, i32, i64, i64, vp, vp, i64, vp, vp, vp]),
     "bzg_zgesv": (i32, [vp, i64, i64, vp, i64, vp, i64, vp]),
+    "bzg_ztranspose": (i32, [vp, i64, i64, vp, i64, vp, i64]),
     "bzg_dgetri": (i32, [vp, i64, vp, i64, vp]),
     "bzg_ddet_lu": (i32, [vp, i64, vp, i64, vp, vp, vp]),
     "bzg_dgetrs_batched": (i32, [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -416,6 +416,34 @@ bzg_status bzg_cgemv_ex(bzg_context* ctx, int32_t trans, int64_t m, int64_t n, c
   return bzg::complex_gemv<float>(ctx, trans, m, n, alpha, A, lda, x, beta, y);
 }
 
+bzg_status bzg_ztranspose(bzg_context* ctx, int64_t m, int64_t n, const double* A, int64_t lda, double* B,
+                          int64_t ldb) {
+  BZG_REQUIRE(ctx != nullptr, BZG_ERR_INVALID_ARGUMENT, "ztranspose: null context");
+  BZG_REQUIRE(m >= 0 && n >= 0, BZG_ERR_INVALID_ARGUMENT, "ztranspose: negative size");
+  if (m == 0 || n == 0) return BZG_OK;
+  BZG_REQUIRE(A && B && lda >= n && ldb >= m, BZG_ERR_INVALID_ARGUMENT, "ztranspose: bad pointer or leading dimension");
+  BZG_REQUIRE(A != B, BZG_ERR_INVALID_ARGUMENT, "ztranspose: not in place");
+  BZG_REQUIRE((n + bzg::kTile - 1) / bzg::kTile < 65536, BZG_ERR_UNSUPPORTED, "ztranspose: n beyond 2,097,120");
+  // B (n x m, ldb) = A^T: op_copy writes op(S) with S = A stored m x n, as an n x m matrix of
+  // leading dimension equal to its width; copy row blocks when ldb != m
+  if (ldb == m) {
+    hipLaunchKernelGGL(bzg::op_copy_kernel<double>, dim3((unsigned)((m + bzg::kTile - 1) / bzg::kTile),
+                       (unsigned)((n + bzg::kTile - 1) / bzg::kTile)), dim3(256), 0, ctx->stream, n, m, A, lda, 1, B);
+    BZG_LAUNCH_CHECK();
+    return BZG_OK;
+  }
+  void* ws = nullptr;
+  bzg_status st = bzg::workspace(ctx, bzg::kWsOpA, (size_t)2 * m * n * sizeof(double), &ws);
+  if (st) return st;
+  hipLaunchKernelGGL(bzg::op_copy_kernel<double>, dim3((unsigned)((m + bzg::kTile - 1) / bzg::kTile),
+                     (unsigned)((n + bzg::kTile - 1) / bzg::kTile)), dim3(256), 0, ctx->stream, n, m, A, lda, 1,
+                     static_cast<double*>(ws));
+  BZG_LAUNCH_CHECK();
+  BZG_HIP_CHECK(hipMemcpy2DAsync(B, (size_t)ldb * 16, ws, (size_t)m * 16, (size_t)m * 16, (size_t)n,
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+  return BZG_OK;
+}
+
 bzg_status bzg_zgesv(bzg_context* ctx, int64_t n, int64_t nrhs, const double* A, int64_t lda, double* B,
                      int64_t ldb, int32_t* info) {
   BZG_REQUIRE(ctx != nullptr && info != nullptr, BZG_ERR_INVALID_ARGUMENT, "zgesv: null context or info");

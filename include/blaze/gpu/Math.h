@@ -1229,6 +1229,18 @@ inline void solve(const DynamicMatrix<double, columnMajor>& A, DynamicVector<dou
                   const DynamicVector<double>& b) {
   solve(rowMajorCopy(A), x, b);
 }
+inline DynamicMatrix<std::complex<double>> rowMajorCopy(const DynamicMatrix<std::complex<double>, columnMajor>& A) {
+  DynamicMatrix<std::complex<double>> R(A.rows(), A.columns());
+  if (A.rows() * A.columns())
+    detail::check(bzg_ztranspose(context(), A.columns(), A.rows(), reinterpret_cast<const double*>(A.data()),
+                                 A.rows(), reinterpret_cast<double*>(R.data()), A.columns()),
+                  "transpose");
+  return R;
+}
+inline void solve(const DynamicMatrix<std::complex<double>, columnMajor>& A, DynamicVector<std::complex<double>>& x,
+                  const DynamicVector<std::complex<double>>& b) {
+  solve(rowMajorCopy(A), x, b);
+}
 inline double det(const DynamicMatrix<double, columnMajor>& A) {
   if (A.rows() != A.columns()) throw std::invalid_argument("Invalid non-square matrix provided");
   return det(A.storage());

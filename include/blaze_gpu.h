@@ -461,6 +461,11 @@ bzg_status bzg_cgemv_ex(bzg_context* ctx, int32_t trans, int64_t m, int64_t n, c
  * 16 n nrhs bytes. */
 bzg_status bzg_zgesv(bzg_context* ctx, int64_t n, int64_t nrhs, const double* A, int64_t lda, double* B,
                      int64_t ldb, int32_t* info);
+/* B = A^T for a row-major m x n complex<double> A into the n x m B (ldb >= m), not in place (no
+ * conjugation).  No reference function of its own: the storage-order change of a column-major
+ * complex system matrix before bzg_zgesv, as bzg_dtranspose for double. */
+bzg_status bzg_ztranspose(bzg_context* ctx, int64_t m, int64_t n, const double* A, int64_t lda, double* B,
+                          int64_t ldb);
 
 /* ---- FP64-accurate GEMM from BF16 residue slices (Ozaki scheme II on MFMA) --------------
  * Same contract as bzg_dgemm with alpha = 1, beta = 0.  `moduli` selects the number of

@@ -878,6 +878,43 @@ int main() {
       check(throws<std::runtime_error>([&] { solve(G, xx, bv); }, "Inversion of singular matrix failed"),
             "complex<double>: singular solve() throws (N = 7)");
     }
+    {
+      // column-major complex operands: C = A * B and solve() on a column-major system matrix
+      DynamicMatrix<cd, columnMajor> Ac(m, k), Bc(k, n), Cc;
+      std::vector<cd> as(m * k), bs(k * n);  // column-major storage = row-major transposes
+      for (size_t i = 0; i < m; ++i)
+        for (size_t p = 0; p < k; ++p) as[p * m + i] = a[i * k + p];
+      for (size_t p = 0; p < k; ++p)
+        for (size_t j = 0; j < n; ++j) bs[j * k + p] = b[p * n + j];
+      Ac.storage().upload(as.data());
+      Bc.storage().upload(bs.data());
+      Cc = Ac * Bc;
+      auto cs = Cc.storage().toVector();
+      bool okc = Cc.rows() == m && Cc.columns() == n;
+      for (size_t i = 0; i < m; ++i)
+        for (size_t j = 0; j < n; ++j) okc = okc && close(cs[j * m + i], ref[i * n + j], 1e-13);
+      check(okc, "complex<double> column-major: C = A * B");
+      const size_t N = 40;
+      std::vector<cd> g(N * N), gs(N * N), r(N);
+      for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) g[i * N + j] = cd(U(rng), U(rng)) + (i == j ? cd(double(N), 0.0) : cd(0.0));
+      for (size_t i = 0; i < N; ++i)
+        for (size_t j = 0; j < N; ++j) gs[j * N + i] = g[i * N + j];
+      for (auto& v : r) v = cd(U(rng), U(rng));
+      DynamicMatrix<cd, columnMajor> G(N, N);
+      G.storage().upload(gs.data());
+      DynamicVector<cd> bv(N), xx;
+      bv.upload(r.data());
+      solve(G, xx, bv);
+      auto xs = xx.toVector();
+      okc = xx.size() == N;
+      for (size_t i = 0; i < N; ++i) {
+        cd t(0.0, 0.0);
+        for (size_t j = 0; j < N; ++j) t += g[i * N + j] * xs[j];
+        okc = okc && close(t, r[i], 1e-12);
+      }
+      check(okc, "complex<double> column-major: solve(A, x, b), N = 40");
+    }
     std::vector<cf> af(m * k), bf(k * n);
     for (size_t q = 0; q < m * k; ++q) af[q] = cf(a[q]);
     for (size_t q = 0; q < k * n; ++q) bf[q] = cf(b[q]);

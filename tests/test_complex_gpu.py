@@ -223,3 +223,16 @@ def test_zgesv_singular_leaves_b_and_reports():
     # order 0 and no right-hand sides
     X, info, _ = _zgesv(np.zeros((0, 0), np.complex128), np.zeros((0, 1), np.complex128))
     assert info == 0
+
+
+@pytest.mark.parametrize("m,n,ldb_pad", [(37, 53, 0), (300, 7, 5), (1, 1, 0), (128, 256, 3)])
+def test_ztranspose(m, n, ldb_pad):
+    rng = np.random.default_rng(m * n + ldb_pad)
+    A = rng.uniform(-1, 1, (m, n + 2)) + 1j * rng.uniform(-1, 1, (m, n + 2))  # lda = n + 2
+    ldb = m + ldb_pad
+    Bst = np.full((n, ldb), 9.5 - 2.0j)
+    dA, dB = _d(A), _d(Bst)
+    ctx().call("bzg_ztranspose", m, n, p(dA), n + 2, p(dB), ldb)
+    got = host(dB)[:2 * n * ldb].view(np.complex128).reshape(n, ldb)
+    assert np.array_equal(got[:, :m], A[:, :n].T)
+    assert np.array_equal(got[:, m:], Bst[:, m:])
