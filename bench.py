@@ -379,6 +379,11 @@ class Bench:
         out["laplace128_fp64"] = self._spmv_run(rp, ci, va, x, nr, "bzg_dcsrmv", steps)
         out["laplace128_fp32"] = self._spmv_run(rp, ci, va.astype(np.float32), x.astype(np.float32), nr,
                                                 "bzg_scsrmv", steps)
+        # the same products on the plan's sliced-ELL copy (bzg_csr_plan_pack_sell, packed once per
+        # matrix copy), bit-identical to the CSR kernels
+        out["laplace128_fp64_sell"] = self._sell_run(rp, ci, va, x, nr, 0, steps)
+        out["laplace128_fp32_sell"] = self._sell_run(rp, ci, va.astype(np.float32), x.astype(np.float32), nr, 1,
+                                                     steps)
         n = 1_000_000
         rp = np.zeros(n + 1, np.int32)
         g.bzg_gen_zipf_rowptr(C.c_uint64(SEED), C.c_int64(n), C.c_double(1.5), C.c_int32(2000),
@@ -466,6 +471,67 @@ class Bench:
                 "copies_rotated": copies, "plan": buf.value.decode(),
                 "timing": ("us/gbps: host-launched calls (ctypes); kernel_*: HIP events around each call; "
                            "graph_* and frac_hbm: calls replayed back to back from a HIP graph")}
+
+    def _sell_run(self, rp, ci, va, x, ncols, dtype, steps):
+        """y = S * x from sliced-ELL copies (one plan and copy per rotated matrix copy), calls
+        replayed back to back from a HIP graph; `bytes` = the CSR product's algorithmic bytes (as
+        the CSR legs), `sell_bytes` = what the sliced layout reads (padded slots, no row pointers)."""
+        torch, bz = self.torch, self.bz
+        n = len(rp) - 1
+        nnz = int(rp[-1])
+        esz = va.dtype.itemsize
+        by = nnz * (esz + 4) + (n + 1) * 4 + 2 * n * esz
+        lens = np.diff(rp)
+        pad = np.zeros(((n + 63) // 64) * 64, lens.dtype)
+        pad[:n] = lens
+        slots = int(pad.reshape(-1, 64).max(axis=1).sum()) * 64
+        sell_by = slots * (esz + 4) + 2 * n * esz
+        copies = max(2, int(np.ceil(600e6 / by)))
+        fn = ("bzg_dsellmv", "bzg_ssellmv", "bzg_isellmv")[dtype]
+        plans = []
+        for _ in range(copies):
+            d = [self.dev(a) for a in (rp, ci, va)]
+            pl = C.c_void_p()
+            bz.check("plan", bz.lib().bzg_csr_plan_create(self.ctx.handle, n, ncols, rp.ctypes.data_as(C.c_void_p),
+                                                          C.byref(pl)))
+            bz.check("pack", bz.lib().bzg_csr_plan_pack_sell(self.ctx.handle, pl, vp(d[0]), vp(d[1]), vp(d[2]), dtype))
+            del d  # the plan holds its own sliced copy
+            plans.append(pl)
+        dx = self.dev(x)
+        dy = torch.empty(n, dtype=dx.dtype, device="cuda")
+        self.ctx.call(fn, plans[0], vp(dx), vp(dy))
+        self.sync()
+        s = torch.cuda.Stream()
+        gctx = bz.Context(stream=s.cuda_stream)
+        calls = 8 * copies
+
+        def many():
+            for i in range(calls):
+                gctx.call(fn, plans[i % copies], vp(dx), vp(dy))
+        with torch.cuda.stream(s):
+            many()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            many()
+        g.replay()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        reps = 5
+        ev[0].record()
+        for _ in range(reps):
+            g.replay()
+        ev[1].record()
+        torch.cuda.synchronize()
+        gt = ev[0].elapsed_time(ev[1]) / 1e3 / (reps * calls)
+        del g
+        gctx.close()
+        for pl in plans:
+            bz.lib().bzg_csr_plan_destroy(pl)
+        return {"graph_us": gt * 1e6, "graph_gbps": by / gt / 1e9, "frac_hbm": by / gt / 1e9 / PEAK["hbm"],
+                "bytes": by, "sell_bytes": sell_by, "sell_gbps": sell_by / gt / 1e9, "nnz": nnz,
+                "copies_rotated": copies,
+                "timing": "calls replayed back to back from a HIP graph over the rotated copies, timed with events"}
 
     # -- dense GEMV, y = A * x and y = trans(A) * x (DMatDVecMultExpr / TDMatDVecMultExpr) ------
     def gemv_leg(self, m=16384, n=16384, reps=5):

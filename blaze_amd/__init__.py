@@ -74,6 +74,10 @@ SIGNATURES = {
     "bzg_iaxpy": (i32, [vp, i64, i32, vp, vp]),
     "bzg_dgetrs": (i32, [vp, i64, i64, vp, i64, vp, vp, i64]),
     "bzg_dgesv": (i32, [vp, i64, i64, vp, i64, vp, vp, i64, vp]),
+    "bzg_csr_plan_pack_sell": (i32, [vp, vp, vp, vp, vp, i32]),
+    "bzg_dsellmv": (i32, [vp, vp, vp, vp]),
+    "bzg_ssellmv": (i32, [vp, vp, vp, vp]),
+    "bzg_isellmv": (i32, [vp, vp, vp, vp]),
     "bzg_zgemm_ex": (i32, [vp, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64]),
     "bzg_cgemm_ex": (i32, [vp, i32, i32, i64, i64, i64, vp, vp, i64, vp, i64, vp, vp, i64]),
     "bzg_zgemv_ex": (i32, [vp, i32, i64, i64, vp, vp, i64, vp, vp, vp]),

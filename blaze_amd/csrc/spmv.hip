@@ -47,6 +47,14 @@ struct bzg_csr_plan {
   int64_t max_row = 0;
   double mean_row = 0;
   int plus_zero = 0;        // sums start from +0 (column-major product semantics)
+  // sliced-ELL copy of an all-short-row matrix (bzg_csr_plan_pack_sell): slices of 64
+  // consecutive rows stored column by column (slot k of the slice's rows, then slot k + 1),
+  // width = the slice's longest row; padding slots hold column -1 and value 0
+  int64_t n_slices = 0;
+  int64_t* sell_off = nullptr;     // device: n_slices + 1 element offsets (multiples of 64)
+  int32_t* sell_col = nullptr;
+  void* sell_val = nullptr;
+  int sell_dtype = -1;             // 0 double, 1 float, 2 int32; -1 not packed
   std::vector<int32_t> rowptr_host;  // kept for re-planning by the autotuner
   char desc[256] = {0};
 };
@@ -227,6 +235,87 @@ __global__ __launch_bounds__(256) void csr_block_kernel(int64_t n_blocks, const 
       y[cur.x + t] = (re[h] == rs[h]) ? T(0) : s;
     }
   }
+}
+
+// Sliced ELL (SELL-64, rows in their original order): lane l of wavefront s sums row 64 s + l
+// from the slice's column-major slots, eight slots' loads and gathers in flight at a time, in
+// slot order = storage order (padding slots, column -1, are skipped, not multiplied), so every
+// row is the reference's sequential sum bit for bit.  No LDS, no barrier, no row pointers:
+// value and column loads are 64 consecutive elements per wavefront instruction.
+template <typename T>
+__global__ __launch_bounds__(256) void sell_kernel(int64_t n_rows, int64_t n_slices, const int64_t* __restrict__ off,
+                                                   const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                   const T* __restrict__ x, T* __restrict__ y, T s0) {
+  const int lane = threadIdx.x & 63;
+  const int64_t sl = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (sl >= n_slices) return;  // uniform over the wavefront
+  const int64_t base = off[sl];
+  const int w = (int)((off[sl + 1] - base) >> 6);
+  T sum = s0;
+  bool any = false;
+  for (int k0 = 0; k0 < w; k0 += 8) {
+    int32_t c[8];
+    T v[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      // slots past the width re-read the last slot (in bounds, no branch) and are dropped
+      const bool in = k0 + u < w;
+      const int64_t e = base + (int64_t)(in ? k0 + u : w - 1) * 64 + lane;
+      const int32_t cl = __builtin_nontemporal_load(col + e);
+      const T vl = __builtin_nontemporal_load(val + e);
+      c[u] = in ? cl : -1;
+      v[u] = in ? vl : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : T(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (c[u] >= 0) {
+        sum = add_rn(sum, mul_rn(v[u], xv[u]));
+        any = true;
+      }
+    }
+  }
+  const int64_t row = sl * 64 + lane;
+  if (row < n_rows) y[row] = any ? sum : T(0);
+}
+
+// CSR -> SELL-64: thread per row writes the row's slots (its nonzeros, then padding).
+template <typename E>
+__global__ __launch_bounds__(256) void sell_pack_kernel(int64_t n_rows, int64_t n_slices, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colidx, const E* __restrict__ val,
+                                                        int32_t* __restrict__ scol, E* __restrict__ sval) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t sl = r >> 6;
+  if (sl >= n_slices) return;
+  const int lane = (int)(r & 63);
+  const int64_t base = off[sl];
+  const int w = (int)((off[sl + 1] - base) >> 6);
+  const int64_t b = r < n_rows ? rowptr[r] : 0;
+  const int len = r < n_rows ? (int)(rowptr[r + 1] - b) : 0;
+  for (int k = 0; k < w; ++k) {
+    const int64_t e = base + (int64_t)k * 64 + lane;
+    scol[e] = k < len ? colidx[b + k] : -1;
+    sval[e] = k < len ? val[b + k] : E(0);
+  }
+}
+
+template <typename T>
+bzg_status sellmv(bzg_context* ctx, const bzg_csr_plan* plan, int dtype, const T* x, T* y) {
+  BZG_REQUIRE(ctx && plan, BZG_ERR_INVALID_ARGUMENT, "sellmv: null context or plan");
+  BZG_REQUIRE(plan->sell_dtype == dtype, BZG_ERR_INVALID_ARGUMENT,
+              "sellmv: the plan holds no sliced-ELL copy of this element type (bzg_csr_plan_pack_sell)");
+  if (plan->n_rows == 0) return BZG_OK;
+  BZG_REQUIRE(x && y, BZG_ERR_INVALID_ARGUMENT, "sellmv: null x or y");
+  ProfScope prof(ctx, "sellmv");
+  const T s0 = plan->plus_zero ? T(0) : T(-T(0));
+  const int64_t wgs = (plan->n_slices + 3) / 4;
+  BZG_REQUIRE(wgs < (1LL << 31), BZG_ERR_UNSUPPORTED, "sellmv: too many rows");
+  hipLaunchKernelGGL(sell_kernel<T>, dim3((unsigned)wgs), dim3(256), 0, ctx->stream, plan->n_rows, plan->n_slices,
+                     plan->sell_off, plan->sell_col, static_cast<const T*>(plan->sell_val), x, y, s0);
+  BZG_LAUNCH_CHECK();
+  return BZG_OK;
 }
 
 // Long rows: LANES threads per row (64 = one wavefront, 256 = one workgroup).  Each thread
@@ -602,6 +691,9 @@ bzg_status bzg_csc_to_csr_host(int64_t n_rows, int64_t n_cols, const int32_t* co
 
 bzg_status bzg_csr_plan_destroy(bzg_csr_plan* p) {
   if (!p) return BZG_OK;
+  if (p->sell_off) hipFree(p->sell_off);
+  if (p->sell_col) hipFree(p->sell_col);
+  if (p->sell_val) hipFree(p->sell_val);
   if (p->short_rows) hipFree(p->short_rows);
   if (p->blocks) hipFree(p->blocks);
   if (p->long_rows) hipFree(p->long_rows);
@@ -614,6 +706,67 @@ bzg_status bzg_csr_plan_describe(const bzg_csr_plan* p, char* buf, size_t len) {
   BZG_REQUIRE(p && buf && len > 0, BZG_ERR_INVALID_ARGUMENT, "csr_plan_describe: null argument");
   std::snprintf(buf, len, "%s", p->desc);
   return BZG_OK;
+}
+
+bzg_status bzg_csr_plan_pack_sell(bzg_context* ctx, bzg_csr_plan* p, const int32_t* rowptr, const int32_t* colidx,
+                                  const void* val, int32_t dtype) {
+  BZG_REQUIRE(ctx && p, BZG_ERR_INVALID_ARGUMENT, "csr_plan_pack_sell: null context or plan");
+  BZG_REQUIRE(dtype >= 0 && dtype <= 2, BZG_ERR_INVALID_ARGUMENT, "csr_plan_pack_sell: dtype must be 0, 1 or 2");
+  BZG_REQUIRE(p->n_long == 0 && p->n_huge == 0, BZG_ERR_UNSUPPORTED,
+              "csr_plan_pack_sell: rows longer than 32 nonzeros (the sliced layout is for short rows)");
+  BZG_REQUIRE(p->rowptr_host.size() == (size_t)p->n_rows + 1, BZG_ERR_INVALID_ARGUMENT,
+              "csr_plan_pack_sell: plan has no host row pointers");
+  if (p->sell_off) hipFree(p->sell_off);
+  if (p->sell_col) hipFree(p->sell_col);
+  if (p->sell_val) hipFree(p->sell_val);
+  p->sell_off = nullptr;
+  p->sell_col = nullptr;
+  p->sell_val = nullptr;
+  p->sell_dtype = -1;
+  p->n_slices = (p->n_rows + 63) / 64;
+  if (p->n_rows == 0) {
+    p->sell_dtype = dtype;
+    return BZG_OK;
+  }
+  BZG_REQUIRE(rowptr && (p->nnz == 0 || (colidx && val)), BZG_ERR_INVALID_ARGUMENT,
+              "csr_plan_pack_sell: null CSR arrays");
+  std::vector<int64_t> off((size_t)p->n_slices + 1, 0);
+  const std::vector<int32_t>& rp = p->rowptr_host;
+  for (int64_t sl = 0; sl < p->n_slices; ++sl) {
+    int w = 0;
+    for (int64_t r = sl * 64; r < std::min<int64_t>(p->n_rows, sl * 64 + 64); ++r) w = std::max(w, rp[r + 1] - rp[r]);
+    off[(size_t)sl + 1] = off[(size_t)sl] + 64 * (int64_t)w;
+  }
+  const int64_t elems = off.back();
+  const size_t esz = dtype == 0 ? 8 : 4;
+  BZG_HIP_CHECK(hipMalloc(&p->sell_off, off.size() * sizeof(int64_t)));
+  BZG_HIP_CHECK(hipMalloc(&p->sell_col, (size_t)std::max<int64_t>(elems, 1) * sizeof(int32_t)));
+  BZG_HIP_CHECK(hipMalloc(&p->sell_val, (size_t)std::max<int64_t>(elems, 1) * esz));
+  BZG_HIP_CHECK(hipMemcpyAsync(p->sell_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+  const unsigned grid = (unsigned)((p->n_slices * 64 + 255) / 256);
+  if (esz == 8)
+    hipLaunchKernelGGL(bzg::sell_pack_kernel<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream, p->n_rows, p->n_slices,
+                       p->sell_off, rowptr, colidx, static_cast<const uint64_t*>(val), p->sell_col,
+                       static_cast<uint64_t*>(p->sell_val));
+  else
+    hipLaunchKernelGGL(bzg::sell_pack_kernel<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream, p->n_rows, p->n_slices,
+                       p->sell_off, rowptr, colidx, static_cast<const uint32_t*>(val), p->sell_col,
+                       static_cast<uint32_t*>(p->sell_val));
+  BZG_LAUNCH_CHECK();
+  BZG_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // the host offsets go out of scope
+  p->sell_dtype = dtype;
+  return BZG_OK;
+}
+
+bzg_status bzg_dsellmv(bzg_context* ctx, const bzg_csr_plan* plan, const double* x, double* y) {
+  return bzg::sellmv<double>(ctx, plan, 0, x, y);
+}
+bzg_status bzg_ssellmv(bzg_context* ctx, const bzg_csr_plan* plan, const float* x, float* y) {
+  return bzg::sellmv<float>(ctx, plan, 1, x, y);
+}
+bzg_status bzg_isellmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* x, int32_t* y) {
+  return bzg::sellmv<int32_t>(ctx, plan, 2, x, y);
 }
 
 bzg_status bzg_dcsrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowptr,

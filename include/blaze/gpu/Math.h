@@ -224,6 +224,14 @@ inline bzg_status csrmv(const bzg_csr_plan* p, const int32_t* rp, const int32_t*
                         const int* v, const int* x, int* y) {
   return bzg_icsrmv(context(), p, rp, ci, v, x, y);
 }
+// the plan's sliced-ELL copy (all rows <= 32 nonzeros): element type code and product
+template <typename T> constexpr int sell_dtype();
+template <> constexpr int sell_dtype<double>() { return 0; }
+template <> constexpr int sell_dtype<float>() { return 1; }
+template <> constexpr int sell_dtype<int>() { return 2; }
+inline bzg_status sellmv(const bzg_csr_plan* p, const double* x, double* y) { return bzg_dsellmv(context(), p, x, y); }
+inline bzg_status sellmv(const bzg_csr_plan* p, const float* x, float* y) { return bzg_ssellmv(context(), p, x, y); }
+inline bzg_status sellmv(const bzg_csr_plan* p, const int* x, int* y) { return bzg_isellmv(context(), p, x, y); }
 template <typename T>
 void h2d(T* d, const T* h, size_t n) {
   check(bzg_memcpy(context(), d, h, n * sizeof(T), 0), "bzg_memcpy H2D");
@@ -665,6 +673,11 @@ class CompressedMatrix {  // row-major (the column-major form is the specialisat
     upload();
     return plan_;
   }
+  // true when the plan holds the sliced-ELL copy (every row <= 32 nonzeros)
+  bool sliced() const {
+    upload();
+    return sliced_;
+  }
   const int32_t* d_rowptr() const { upload(); return d_rowptr_.ptr; }
   const int32_t* d_colidx() const { upload(); return d_col_.ptr; }
   const T* d_values() const { upload(); return d_val_.ptr; }
@@ -716,6 +729,10 @@ class CompressedMatrix {  // row-major (the column-major form is the specialisat
     bzg_csr_plan* p = nullptr;
     detail::check(bzg_csr_plan_create(context(), m_, n_, rowptr_.data(), &p), "CSR plan");
     self->plan_ = p;
+    // matrices whose rows all hold <= 32 nonzeros (stencils) also get the sliced-ELL copy: the
+    // same sums with coalesced loads and no row pointers
+    self->sliced_ = bzg_csr_plan_pack_sell(context(), p, d_rowptr_.ptr, d_col_.ptr, d_val_.ptr,
+                                           detail::sell_dtype<T>()) == BZG_OK;
   }
   void upload_trans() const {
     if (t_plan_) return;
@@ -743,6 +760,7 @@ class CompressedMatrix {  // row-major (the column-major form is the specialisat
   void drop_device() {
     if (plan_) bzg_csr_plan_destroy(plan_);
     plan_ = nullptr;
+    sliced_ = false;
     d_rowptr_.release();
     d_col_.release();
     d_val_.release();
@@ -757,6 +775,7 @@ class CompressedMatrix {  // row-major (the column-major form is the specialisat
   std::vector<int32_t> rowptr_{0}, col_;
   std::vector<T> val_;
   bzg_csr_plan* plan_ = nullptr;
+  bool sliced_ = false;
   detail::DeviceBuffer<int32_t> d_rowptr_, d_col_;
   detail::DeviceBuffer<T> d_val_;
   bzg_csr_plan* t_plan_ = nullptr;
@@ -793,7 +812,9 @@ template <typename T>
 void DynamicVector<T>::assign(const SMatDVecMultExpr<T>& e) {
   const size_t m = e.mat.rows();
   resize(m, false);
-  if (m)
+  if (m && e.mat.sliced())
+    detail::check(detail::sellmv(e.mat.plan(), e.vec.data(), buf_.ptr), "y = S * x");
+  else if (m)
     detail::check(detail::csrmv(e.mat.plan(), e.mat.d_rowptr(), e.mat.d_colidx(), e.mat.d_values(),
                                 e.vec.data(), buf_.ptr),
                   "y = S * x");

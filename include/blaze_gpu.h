@@ -262,6 +262,19 @@ bzg_status bzg_mm_read_csr(const char* path, bzg_csr_host* out);
 bzg_status bzg_csr_host_free(bzg_csr_host* matrix);
 
 bzg_status bzg_csr_plan_destroy(bzg_csr_plan* plan);
+/* Sliced-ELL copy of an all-short-row matrix (every row <= 32 nonzeros; BZG_ERR_UNSUPPORTED
+ * otherwise) held by the plan: slices of 64 consecutive rows stored slot by slot, padding slots
+ * with column -1.  dtype: 0 double, 1 float, 2 int32; rowptr / colidx / val are the device CSR
+ * arrays of the matrix the plan was made for (copied, not kept).  Then y = S * x with
+ * bzg_dsellmv / bzg_ssellmv / bzg_isellmv reads only the copy and x: bit-identical to
+ * bzg_*csrmv (every row is the sequential sum in storage order, SVecDVecInnerExpr.h:127-135),
+ * with coalesced value and index loads and no row pointers.  Pack again after the values
+ * change.  Memory: (4 + sizeof element) bytes per slot. */
+bzg_status bzg_csr_plan_pack_sell(bzg_context* ctx, bzg_csr_plan* plan, const int32_t* rowptr, const int32_t* colidx,
+                                  const void* val, int32_t dtype);
+bzg_status bzg_dsellmv(bzg_context* ctx, const bzg_csr_plan* plan, const double* x, double* y);
+bzg_status bzg_ssellmv(bzg_context* ctx, const bzg_csr_plan* plan, const float* x, float* y);
+bzg_status bzg_isellmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* x, int32_t* y);
 /* Overrides the vector-CSR shape chosen from the histogram: lanes per row in {1,2,4,8,16,32},
  * rows per lane group in {1,2}, nonzeros loaded per lane per batch in {1,2,4,8}; used by the
  * autotuner.  Results stay bit-identical (every row is still summed in storage order). */
