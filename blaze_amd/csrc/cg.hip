@@ -3,8 +3,9 @@
 //   r = A x - b;  delta = r.r;  d = -r;
 //   repeat: h = A d;  alpha = delta / (d.h);  x += alpha d;  r += alpha h;
 //           beta = r.r;  d = (beta / delta) d - r;  delta = beta.
-// Per iteration: the SpMV (bzg_dcsrmv, same plan and kernels as y = S * x), then three
-// streaming kernels.  Dot products are two-stage and deterministic: each workgroup writes the
+// Per iteration: the SpMV (bzg_dcsrmv, same plan and kernels as y = S * x; the plan's
+// sliced-ELL copy through bzg_dsellmv when it holds one, same sums), then three streaming
+// kernels.  Dot products are two-stage and deterministic: each workgroup writes the
 // partial sum of its fixed slice (fixed butterfly inside), and every consumer adds the
 // partials in index order itself, so alpha and beta are computed redundantly but identically
 // by every workgroup that needs them and no scalar ever leaves the device.
@@ -139,13 +140,17 @@ bzg_status bzg_dcg(bzg_context* ctx, const bzg_csr_plan* plan, int64_t n, const 
   double* pc = pb + bzg::kCgMaxParts;    // r.r of the next delta
   bzg::ProfScope prof(ctx, "cg");
   const dim3 grid((unsigned)parts), block(bzg::kCgThreads);
+  const bool sliced = bzg::plan_has_sell(plan, 0);
+  auto spmv = [&](const double* in, double* out) {
+    return sliced ? bzg_dsellmv(ctx, plan, in, out) : bzg_dcsrmv(ctx, plan, rowptr, colidx, val, in, out);
+  };
   // r = A x - b, delta = r.r, d = -r
-  st = bzg_dcsrmv(ctx, plan, rowptr, colidx, val, x, h);
+  st = spmv(x, h);
   if (st) return st;
   hipLaunchKernelGGL(bzg::cg_init_kernel, grid, block, 0, ctx->stream, n, h, b, r, d, pa);
   BZG_LAUNCH_CHECK();
   for (int it = 0; it < iterations; ++it) {
-    st = bzg_dcsrmv(ctx, plan, rowptr, colidx, val, d, h);  // h = A d
+    st = spmv(d, h);  // h = A d
     if (st) return st;
     hipLaunchKernelGGL(bzg::cg_dot_kernel, grid, block, 0, ctx->stream, n, d, h, pb);
     hipLaunchKernelGGL(bzg::cg_update_kernel, grid, block, 0, ctx->stream, n, parts, pa, pb, x, r, d, h, pc);

@@ -98,6 +98,10 @@ bzg_status launch_dsgesv_onchip(bzg_context* ctx, const int32_t* lists, int cap,
                                 int* counters, int32_t* fb_list, int* fb_count, double tol, int max_iter,
                                 const int* wg_per_cu, int64_t count);
 
+// True when the CSR plan holds a sliced-ELL copy of element type dtype (0 double, 1 float,
+// 2 int32; spmv.hip).
+bool plan_has_sell(const bzg_csr_plan* plan, int dtype);
+
 // Returns a cached device buffer of at least `bytes` (slot-indexed); grows on demand.
 bzg_status workspace(bzg_context* ctx, int slot, size_t bytes, void** out);
 

@@ -456,6 +456,8 @@ bzg_status csrmv(bzg_context* ctx, const bzg_csr_plan* plan, const int32_t* rowp
 }  // namespace bzg
 
 namespace bzg {
+bool plan_has_sell(const bzg_csr_plan* plan, int dtype) { return plan && plan->sell_dtype == dtype; }
+
 // Groups consecutive short rows into LDS row blocks of at most `bnnz` nonzeros and `brows`
 // rows (a long row ends the current block) and uploads the descriptors.
 static bzg_status build_row_blocks(bzg_context* ctx, bzg_csr_plan* p, const int32_t* rowptr_host,

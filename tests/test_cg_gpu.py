@@ -108,3 +108,14 @@ def test_cg_general_plan_path_matches_oracle():
     xo, rro = oracle.cg(pb.rp, pb.ci, pb.va, pb.b, pb.x0, 40)
     assert np.abs(x.cpu().numpy() - xo).max() <= 1e-9 * max(np.abs(xo).max(), 1e-300)
     assert abs(rr - rro) <= 1e-6 * rro
+
+
+def test_cg_sliced_plan_is_bit_identical():
+    # the plan's sliced-ELL copy serves the loop's SpMV once packed: same row sums, same x
+    import torch
+    import blaze_amd as bz
+    pb = Problem(torch, 64, 7, False)
+    x1, rr1 = pb.run(torch, 30)
+    bz.check("pack", bz.lib().bzg_csr_plan_pack_sell(ctx().handle, pb.plan.h, _p(pb.d[0]), _p(pb.d[1]), _p(pb.d[2]), 0))
+    x2, rr2 = pb.run(torch, 30)
+    assert torch.equal(x1, x2) and rr1 == rr2
