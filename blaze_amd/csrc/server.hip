@@ -111,6 +111,7 @@ struct Csr {
   int32_t* colidx = nullptr;
   double* val = nullptr;
   bzg_csr_plan* plan = nullptr;
+  bool sliced = false;        // the plan holds the matrix as sliced ELL; the CSR arrays are freed
   size_t bytes = 0;
 };
 
@@ -250,7 +251,7 @@ bzg_status launch_single(bzg_server* s, Request* r, Group* g) {
     st = bzg_dgemm(c, r->m, r->n, r->k, 1.0, r->A, r->lda, r->B, r->ldb, 0.0, r->out, r->n);
   } else {
     const Csr& M = s->csrs[r->csr];
-    st = bzg_dcsrmv(c, M.plan, M.rowptr, M.colidx, M.val, r->A, r->out);
+    st = M.sliced ? bzg_dsellmv(c, M.plan, r->A, r->out) : bzg_dcsrmv(c, M.plan, M.rowptr, M.colidx, M.val, r->A, r->out);
   }
   return st;
 }
@@ -631,6 +632,22 @@ bzg_status bzg_server_add_csr(bzg_server* s, int64_t n_rows, int64_t n_cols, con
   M.n_cols = n_cols;
   M.nnz = nnz;
   M.bytes = (size_t)(n_rows + 1) * 4 + (size_t)nnz * 12;
+  // rows of <= 32 nonzeros throughout: the matrix is kept as the plan's sliced-ELL copy (64-row
+  // slices, padded to each slice's longest row), which the SpMV reads with coalesced loads;
+  // both forms are reserved until the copy exists, then the CSR arrays and their bytes go
+  size_t sell_bytes = 0;
+  {
+    int64_t slots = 0;
+    bool short_rows = true;
+    for (int64_t s0 = 0; s0 < n_rows && short_rows; s0 += 64) {
+      int32_t w = 0;
+      for (int64_t r = s0; r < std::min<int64_t>(n_rows, s0 + 64); ++r) w = std::max(w, rowptr[r + 1] - rowptr[r]);
+      short_rows = w <= 32;
+      slots += 64 * (int64_t)w;
+    }
+    if (short_rows && n_rows > 0) sell_bytes = (size_t)slots * 12 + (size_t)((n_rows + 63) / 64 + 1) * 8;
+  }
+  M.bytes += sell_bytes;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     BZG_REQUIRE(s->used + M.bytes <= s->cfg.hbm_budget_bytes, BZG_ERR_OUT_OF_MEMORY,
@@ -668,6 +685,21 @@ bzg_status bzg_server_add_csr(bzg_server* s, int64_t n_rows, int64_t n_cols, con
   }
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
     return fail(BZG_ERR_HIP, std::string("plan: ") + hipGetErrorString(e));
+  if (sell_bytes) {
+    st = bzg_csr_plan_pack_sell(c, M.plan, M.rowptr, M.colidx, M.val, 0);
+    if (st) return fail(st, "");
+    hipFree(M.rowptr);
+    hipFree(M.colidx);
+    hipFree(M.val);
+    M.rowptr = nullptr;
+    M.colidx = nullptr;
+    M.val = nullptr;
+    M.sliced = true;
+    const size_t csr_bytes = M.bytes - sell_bytes;
+    M.bytes = sell_bytes;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->used -= csr_bytes;
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   s->csrs.push_back(M);
   *handle = (int32_t)s->csrs.size() - 1;

@@ -552,7 +552,9 @@ bzg_status bzg_server_create(const bzg_server_config* cfg, bzg_server** server);
 /* Waits for every admitted request, then frees all server memory (results included). */
 bzg_status bzg_server_destroy(bzg_server* server);
 /* Uploads a double CSR matrix (host arrays, same format as bzg_dcsrmv) and builds its plan;
- * it counts against the budget for the server's lifetime. */
+ * it counts against the budget for the server's lifetime.  A matrix whose rows all hold <= 32
+ * nonzeros is kept as the plan's sliced-ELL copy instead (bzg_csr_plan_pack_sell; same results):
+ * both forms are reserved while the copy is made, then only the copy's bytes. */
 bzg_status bzg_server_add_csr(bzg_server* server, int64_t n_rows, int64_t n_cols,
                               const int32_t* rowptr, const int32_t* colidx, const double* val,
                               int32_t* handle);

@@ -185,7 +185,11 @@ def test_add_csr_failure_returns_budget():
     rp = np.arange(n + 1, dtype=np.int32)                 # one entry per row: 4 (n+1) + 12 n bytes
     ci = np.arange(n, dtype=np.int32); va = np.ones(n)
     need = 4 * (n + 1) + 12 * n
-    srv = Server(budget_bytes=need + 1024, max_inflight=4, streams=1)
+    # all rows short: the server keeps the sliced-ELL copy (64-row slices of width 1 and the
+    # slice offsets); both forms are reserved while the copy is made, then only the copy
+    slices = (n + 63) // 64
+    sell = slices * 64 * 12 + (slices + 1) * 8
+    srv = Server(budget_bytes=need + sell + 1024, max_inflight=4, streams=1)
     try:
         bad = rp.copy(); bad[0] = 1
         for _ in range(3):
@@ -194,6 +198,6 @@ def test_add_csr_failure_returns_budget():
             assert e.value.status == -1
             assert srv.stats()["bytes_in_use"] == 0
         h = srv.add_csr(rp, ci, va, n)
-        assert h == 0 and srv.stats()["bytes_in_use"] == need
+        assert h == 0 and srv.stats()["bytes_in_use"] == sell
     finally:
         srv.close()
