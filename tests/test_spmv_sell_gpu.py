@@ -155,3 +155,15 @@ def test_sell_repack_and_errors():
     plan2 = Plan(rp2, 64)
     d2 = [_dev(a) for a in (rp2, ci2, np.ones(int(rp2[-1])))]
     assert L.bzg_csr_plan_pack_sell(ctx().handle, plan2.h, p(d2[0]), p(d2[1]), p(d2[2]), 0) != 0
+
+
+def test_sell_laplacian_128_full_size_integer_exact():
+    # the benchmark's 128^3 matrix: integer x makes every row sum exact (6 x_i minus neighbours),
+    # and x = 1 gives each row's number of missing neighbours (0 inside, up to 3 at corners)
+    rp, ci, va = laplace3d(128)
+    n = len(rp) - 1
+    x = np.random.default_rng(33).integers(-1000, 1000, n).astype(np.float64)
+    y, yc = sell_and_csr(rp, ci, va, x, n)
+    assert same_bits(y, oracle.csrmv(rp, ci, va, x)) and same_bits(y, yc)
+    ones, _ = sell_and_csr(rp, ci, va, np.ones(n), n)
+    assert ones.min() == 0 and ones.max() == 3
