@@ -48,7 +48,7 @@ struct bzg_context {
   hipStream_t stream = nullptr;
   int num_cus = 256;
   // Cached scratch for kernels that need it (split GEMM slices, solver staging).
-  bzg::Workspace ws[16];
+  bzg::Workspace ws[20];
   // Optional HIP-event timing of named kernels (bzg_profile_enable / bzg_profile_read).
   // plan of the last FP64-accurate GEMM on this context (bzg_fp64acc_last_plan)
   int fp64acc_moduli = 0, fp64acc_bits = 0;

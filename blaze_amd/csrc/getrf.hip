@@ -20,9 +20,9 @@
 // column blocks of width <= 8 on the rows from the diagonal down, factored by one 512-thread
 // workgroup that holds the block in registers (row t + 512 i in thread t): per column a
 // wavefront max-reduction of (|a|, -row), one workgroup barrier, the interchange and the
-// rank-1 update in registers.  A leaf applies its interchanges to the whole rows (all other
-// columns, by a gather of the at most 16 affected rows), which is the net effect of dgetrf2's
-// row swaps on both sides, so no separate swap pass runs.  The triangular solves (dtrsm
+// rank-1 update in registers.  A leaf's interchanges then move the whole rows (all other
+// columns: the at most 16 affected rows, one column per thread over the device), which is the
+// net effect of dgetrf2's row swaps on both sides, so no per-level swap pass runs.  The triangular solves (dtrsm
 // semantics, Left/NoTrans) split recursively down to 32 x 32 diagonal blocks solved by one
 // thread per right-hand-side column; the off-diagonal blocks go through bzg_dgemm.
 #include "common.hpp"
@@ -43,7 +43,8 @@ constexpr int64_t kMaxOrder = 32768;
 template <int R, int WL>
 __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(double* __restrict__ A, long lda, int n, int d0,
                                                                    int w, int32_t* __restrict__ ipiv,
-                                                                   int32_t* __restrict__ info) {
+                                                                   int32_t* __restrict__ info,
+                                                                   int32_t* __restrict__ swp) {
   constexpr int NW = kLeafThreads / 64;
   __shared__ double s_v[2][NW];
   __shared__ int s_r[2][NW];
@@ -177,19 +178,12 @@ __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(double* __rest
     if (first_zero && *info == 0) *info = first_zero;
   }
   __syncthreads();
-  // the other columns of the affected rows: row s_pos[k] takes the old row s_src[k]
-  const int cnt = s_cnt;
-  if (cnt > 0) {
-    for (long col = t; col < n - w; col += kLeafThreads) {
-      const long c = col < d0 ? col : col + w;
-      double v[2 * WL];
-#pragma unroll
-      for (int k = 0; k < 2 * WL; ++k)
-        if (k < cnt) v[k] = A[s_src[k] * lda + c];
-#pragma unroll
-      for (int k = 0; k < 2 * WL; ++k)
-        if (k < cnt) A[s_pos[k] * lda + c] = v[k];
-    }
+  // the other columns of the affected rows (row s_pos[k] takes the old row s_src[k]) are moved
+  // by row_swap_kernel, spread over the device: the list goes out here
+  if (t == 0) swp[0] = s_cnt;
+  if (t < s_cnt) {
+    swp[1 + 2 * t] = (int32_t)s_pos[t];
+    swp[2 + 2 * t] = (int32_t)s_src[t];
   }
   // the block's rows, at their final positions
 #pragma unroll
@@ -201,6 +195,25 @@ __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(double* __rest
         if (c < w) P[(long)row * lda + c] = a[i][c];
     }
   }
+}
+
+// The leaf's interchanges on every column outside its block: row pos[k] takes the old row
+// src[k] (k < cnt <= 2 WL), one column per thread (the rows' loads are contiguous across a
+// wavefront).  This is the net effect of dgetrf2's row swaps on both sides of the block.
+template <int WL>
+__global__ __launch_bounds__(256) void row_swap_kernel(double* __restrict__ A, long lda, int n, int d0, int w,
+                                                       const int32_t* __restrict__ swp) {
+  const int cnt = swp[0];
+  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cnt == 0 || col >= n - w) return;
+  const long c = col < d0 ? col : col + w;
+  double v[2 * WL];
+#pragma unroll
+  for (int k = 0; k < 2 * WL; ++k)
+    if (k < cnt) v[k] = A[(long)swp[2 + 2 * k] * lda + c];
+#pragma unroll
+  for (int k = 0; k < 2 * WL; ++k)
+    if (k < cnt) A[(long)swp[1 + 2 * k] * lda + c] = v[k];
 }
 
 // ---- triangular solves on 32 x 32 diagonal blocks ---------------------------------------------
@@ -315,11 +328,21 @@ static bzg_status launch_leaf(bzg_context* ctx, double* A, long lda, int n, int 
                               int32_t* info) {
   int R, WL;
   leaf_shape(n - d0, &R, &WL);
+  // the leaf's interchange list (count, then (row, source row) pairs) in workspace slot 16
+  void* sw = nullptr;
+  bzg_status st = workspace(ctx, 16, 256, &sw);
+  if (st) return st;
+  int32_t* swp = static_cast<int32_t*>(sw);
+  const unsigned sgrid = (unsigned)std::max<long>(1, ((long)n - w + 255) / 256);
 #define BZG_LEAF(RR, WW)                                                                                   \
   if (R == RR) {                                                                                           \
     hipLaunchKernelGGL((getf2_leaf_kernel<RR, WW>), dim3(1), dim3(kLeafThreads), 0, ctx->stream, A, lda, n, d0, \
-                       w, ipiv, info);                                                                     \
+                       w, ipiv, info, swp);                                                                \
     BZG_LAUNCH_CHECK();                                                                                    \
+    if (n > w) {                                                                                           \
+      hipLaunchKernelGGL((row_swap_kernel<WW>), dim3(sgrid), dim3(256), 0, ctx->stream, A, lda, n, d0, w, swp); \
+      BZG_LAUNCH_CHECK();                                                                                  \
+    }                                                                                                      \
     return BZG_OK;                                                                                         \
   }
   BZG_LEAF(1, 8)
