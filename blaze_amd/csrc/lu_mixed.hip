@@ -13,12 +13,16 @@
 //   factorisation: wavefront w owns tile columns J = w and J = w + W (column-panel ownership),
 //     each tile held as the 16 x 16 accumulator of v_mfma_f32_16x16x4f32 (lane l, register r =
 //     element (4(l/16) + r, l % 16)).  Right-looking blocked LU, panel = one tile column:
-//       - the owner of panel p factors it in registers (row-per-lane layout, the pivot search is
-//         a per-lane max and a butterfly; rows keep logical positions and are written in order
-//         at the end), writes L21 and L11^-1 to LDS, and builds the table of the panel's row
-//         interchanges;
+//       - the owner of panel p factors it in registers (row-per-lane layout).  It first tries
+//         every pivot on the diagonal, each lane checking that no candidate row beats the
+//         pivot; when that holds (always for diagonally dominant matrices) the result is the
+//         partial-pivoting one without a search, and the panel has no interchanges.  Otherwise
+//         the panel is factored again with the search (a per-lane max and a butterfly; rows
+//         keep logical positions and are written in order at the end), and the system's later
+//         panels search directly.  The owner writes L21 and L11^-1 to LDS and, after a search,
+//         the table of the panel's row interchanges;
 //       - one workgroup barrier;
-//       - every wavefront applies the interchanges to its own tile columns (the affected rows
+//       - every wavefront applies the interchanges (if any) to its own tile columns (the affected rows
 //         go through a per-wavefront LDS staging area), forms U12 = L11^-1 A12 with one MFMA
 //         group per tile and updates its trailing tiles A22 -= L21 U12 with MFMAs whose B
 //         operand is U12 itself (the accumulator layout of a tile is the B-operand layout of the
@@ -104,6 +108,8 @@ struct Shm {
   unsigned long long cmax[2 * RMAX + 2];  // |r|, |x| column maxima, ||A||_inf (bit patterns)
   int sys;
   int info;
+  int noswap[2];  // panel p kept every pivot on the diagonal: no interchanges to apply, by parity
+  int spec;       // 1 while the diagonal-pivot attempt has not failed on this system
 };
 
 extern "C" __device__ __attribute__((const)) unsigned __ockl_wfred_max_u32(unsigned);
@@ -188,7 +194,59 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
       P[q][4 * c4] = v[0]; P[q][4 * c4 + 1] = v[1]; P[q][4 * c4 + 2] = v[2]; P[q][4 * c4 + 3] = v[3];
     }
   }
+  // First attempt (while no panel of this system has failed it): every pivot on the diagonal.
+  // Row jj of the panel sits in lane jj, so a column step needs no search; each lane checks
+  // that its candidate rows' |a| do not exceed |pivot| (a tie keeps the first row, the
+  // diagonal), and a zero, infinite or NaN pivot fails the attempt.  If no lane saw a larger
+  // entry, the diagonal was the partial-pivoting choice at every step and the arithmetic below
+  // is that of the searching loop with those pivots, bit for bit; otherwise the panel is
+  // reloaded and factored by the searching loop, and later panels of the system skip the attempt.
+  bool fast = false;
+  if (sh.spec) {
+    unsigned bad = 0u;
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+      float prow[16];
+#pragma unroll
+      for (int c = jj; c < 16; ++c)
+        prow[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(P[0][c]), jj));
+      const unsigned pbits = __float_as_uint(prow[jj]) & 0x7FFFFFFFu;
+      bad |= (pbits == 0u || pbits >= 0x7F800000u) ? 1u : 0u;
+      const float rinv = __builtin_amdgcn_rcpf(prow[jj]);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        if (q >= qm) break;
+        const int rr = lane + 64 * q;
+        if (rr > jj && rr < m) {
+          const float a = P[q][jj];
+          bad |= ((__float_as_uint(a) & 0x7FFFFFFFu) > pbits) ? 1u : 0u;
+          const float l = a * rinv;
+          P[q][jj] = l;
+#pragma unroll
+          for (int c = 0; c < 16; ++c)
+            if (c > jj) P[q][c] -= l * prow[c];
+        }
+      }
+    }
+    fast = __ballot(bad != 0u) == 0ull;
+    if (!fast) {
+      // reload the panel for the searching loop
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int rr = lane + 64 * q;
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4) {
+          float4v v = {0.f, 0.f, 0.f, 0.f};
+          if (rr < m) v = *reinterpret_cast<const float4v*>(&pb[j0 + rr][4 * c4]);
+          P[q][4 * c4] = v[0]; P[q][4 * c4 + 1] = v[1]; P[q][4 * c4 + 2] = v[2]; P[q][4 * c4 + 3] = v[3];
+        }
+      }
+      if (lane == 0) sh.spec = 0;
+    }
+  }
+  if (lane == 0) sh.noswap[par] = fast ? 1 : 0;
   int info = 0;
+  if (!fast) {
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) {
     // candidates: logical rows jj .. m-1, ranked by v = |a| bits + 2 (1 for NaN, 0 for none):
@@ -259,6 +317,7 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
       if (info == 0) info = j0 + jj + 1;
     }
   }
+  }
   // rows to their final positions (pbuf), then back into the tiles
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -283,6 +342,9 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
 #pragma unroll
     for (int i = 0; i < 16; ++i) sh.u.f.linv[par][i][lane] = x[i];
   }
+  // no interchanges: the tables keep the last searched panel of this parity (cleared when the
+  // next one is built) and apply_swaps skips the panel
+  if (fast) return 0;
   // interchange tables: slots 0..15 = rows j0..j0+15, then each distinct pivot row below
   const int pv = lane < 16 ? sh.piv[par][lane] : -1;
   bool first = lane < 16 && pv >= j0 + 16;
@@ -330,6 +392,7 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
 template <int NM>
 __device__ __forceinline__ void apply_swaps(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], int p, int NTn, Shm<NM>& sh, int par, int skip_jc, int tid) {
   constexpr int NT = Cfg<NM>::NT, W = Cfg<NM>::W, NTC = Cfg<NM>::NTC;
+  if (sh.noswap[par]) return;
   const int lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, c16 = lane & 15;
   float(*stg)[16 * NTC + 1] = sh.u.f.stg[wave];
@@ -610,7 +673,10 @@ __global__ __launch_bounds__(Cfg<NM>::THREADS, Cfg<NM>::WAVES_PER_EU) void dsges
     const double* a = A + a_off[s];
     const double* b = B + b_off[s];
     for (int i = tid; i < NM; i += Cfg<NM>::THREADS) sh.perm[i] = i;
-    if (tid == 0) sh.info = 0x7fffffff;  // smallest zero-pivot column over the panels (none: max)
+    if (tid == 0) {
+      sh.info = 0x7fffffff;  // smallest zero-pivot column over the panels (none: max)
+      sh.spec = 1;
+    }
     // ---- A (FP64, HBM) -> FP32 tiles; identity beyond n ----
     f4 T[NT][NTC];
 #pragma unroll
