@@ -29,7 +29,9 @@
 //         next MFMA when both operands index K as 4(l/16) + kk);
 //       - the owner of panel p+1 updates that panel first, factors it, and only then updates its
 //         other tile column (look-ahead: the next panel is factored while the other wavefronts
-//         still update).
+//         still update); in the classes of 128 and 256 that other update waits until the start
+//         of step p+1, where the owner of panel p+2 is on the critical path instead, so the
+//         factors of a panel are kept for two steps (three panel buffers).
 //   solves: the factors are redistributed through LDS (two rounds) to row-block ownership
 //     (wavefront w owns tile rows I = w and w + W), held as MFMA A operands.  Forward and
 //     backward substitution run block by block: the owner of block s forms y_s = L_ss^-1 (c_s -
@@ -58,6 +60,11 @@ struct Cfg {
   static constexpr int THREADS = 64 * W;
   // resident wavefronts per SIMD the register budget is sized for (256, 256, 168, 128 registers)
   static constexpr int WAVES_PER_EU = NM >= 128 ? 2 : (NM == 64 ? 3 : 4);
+  // The owner of panel p + 1 leaves its other tile column's update with panel p to the next
+  // step (DEFER), so panel p's factors must outlive step p: NBUF panel buffers, by p % NBUF.
+  // Only for the multi-wavefront classes whose LDS budget allows the third buffer.
+  static constexpr bool DEFER = NM >= 128;
+  static constexpr int NBUF = DEFER ? 3 : 2;
 };
 
 typedef float float4v __attribute__((ext_vector_type(4)));
@@ -86,8 +93,8 @@ struct Shm {
   union {
     struct {
       float prw[16];           // pivot row being broadcast (panel factorisation)
-      float pbuf[2][NM][20];   // factored panel p (rows at final positions), by panel parity
-      float linv[2][16][20];   // L11^-1 of panel p, by parity
+      float pbuf[Cfg<NM>::NBUF][NM][20];  // factored panel p (rows at final positions), by p % NBUF
+      float linv[Cfg<NM>::NBUF][16][20];  // L11^-1 of panel p, by p % NBUF
       float stg[W][32][16 * Cfg<NM>::NTC + 1];  // per-wavefront interchange staging: 32 row slots
     } f;
     float rbuf[2][W][W][256];  // redistribution rounds: [I - h W][J - jc W] 16 x 16 tiles, swizzled
@@ -98,17 +105,18 @@ struct Shm {
       float dinv[W][16][33];   // its L^-1 (columns 0..15) and U^-1 (columns 16..31)
     } s;
   } u;
-  signed char slot[2][NM];     // row -> interchange slot of panel p (-1: not affected), by parity
-  signed char srcslot[2][32];  // slot of the row whose values move into slot s
-  int dlist[2][32];            // rows of the slots
-  int dcount[2];
-  int piv[2][16];              // absolute pivot rows of panel p
+  // interchange tables of panel p, by p % NBUF
+  signed char slot[Cfg<NM>::NBUF][NM];     // row -> interchange slot (-1: not affected)
+  signed char srcslot[Cfg<NM>::NBUF][32];  // slot of the row whose values move into slot s
+  int dlist[Cfg<NM>::NBUF][32];            // rows of the slots
+  int dcount[Cfg<NM>::NBUF];
+  int piv[Cfg<NM>::NBUF][16];              // absolute pivot rows
   int perm[NM];                // row i of the factored system = original row perm[i]
   int pinv[NM];
   unsigned long long cmax[2 * RMAX + 2];  // |r|, |x| column maxima, ||A||_inf (bit patterns)
   int sys;
   int info;
-  int noswap[2];  // panel p kept every pivot on the diagonal: no interchanges to apply, by parity
+  int noswap[Cfg<NM>::NBUF];  // panel p kept every pivot on the diagonal: no interchanges to apply
   int spec;       // 1 while the diagonal-pivot attempt has not failed on this system
 };
 
@@ -342,7 +350,7 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
 #pragma unroll
     for (int i = 0; i < 16; ++i) sh.u.f.linv[par][i][lane] = x[i];
   }
-  // no interchanges: the tables keep the last searched panel of this parity (cleared when the
+  // no interchanges: the tables keep the last searched panel with this index (cleared when the
   // next one is built) and apply_swaps skips the panel
   if (fast) return 0;
   // interchange tables: slots 0..15 = rows j0..j0+15, then each distinct pivot row below
@@ -355,7 +363,7 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
   }
   const unsigned long long fm = __ballot(first);
   const int prev = sh.dcount[par];
-  if (lane < prev) sh.slot[par][sh.dlist[par][lane]] = -1;  // entries of panel p - 2
+  if (lane < prev) sh.slot[par][sh.dlist[par][lane]] = -1;  // entries of the last panel built here
   wave_sync();
   if (lane < 16) {
     sh.slot[par][j0 + lane] = (signed char)lane;
@@ -488,8 +496,8 @@ __device__ __forceinline__ int lookahead(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], int
   if constexpr (JC < Cfg<NM>::NTC) {
     if (JC == jn) {
       update_column<NM, JC>(T, p, NTn, sh, par, tid);
-      const int inf = factor_panel<NM, JC>(T, p + 1, NTn, sh, par ^ 1, opaque_tid(tid));
-      update_rest<NM>(T, p, NTn, sh, par, JC, opaque_tid(tid));
+      const int inf = factor_panel<NM, JC>(T, p + 1, NTn, sh, (p + 1) % Cfg<NM>::NBUF, opaque_tid(tid));
+      if (!Cfg<NM>::DEFER) update_rest<NM>(T, p, NTn, sh, par, JC, opaque_tid(tid));
       return inf;
     }
     return lookahead<NM, JC + 1>(T, p, NTn, sh, par, jn, tid);
@@ -657,8 +665,8 @@ __global__ __launch_bounds__(Cfg<NM>::THREADS, Cfg<NM>::WAVES_PER_EU) void dsges
   __shared__ Shm<NM> sh;
   const int tid0 = threadIdx.x;
   const int count = *list_count;
-  if (tid0 < 2) sh.dcount[tid0] = 0;
-  for (int i = tid0; i < 2 * NM; i += Cfg<NM>::THREADS) (&sh.slot[0][0])[i] = -1;
+  if (tid0 < Cfg<NM>::NBUF) sh.dcount[tid0] = 0;
+  for (int i = tid0; i < Cfg<NM>::NBUF * NM; i += Cfg<NM>::THREADS) (&sh.slot[0][0])[i] = -1;
   for (;;) {
     const int tid = opaque_tid(tid0), lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, c16 = lane & 15;
@@ -703,9 +711,13 @@ __global__ __launch_bounds__(Cfg<NM>::THREADS, Cfg<NM>::WAVES_PER_EU) void dsges
     int my_info = 0;
     if (wave == 0) my_info = factor_panel<NM, 0>(T, 0, NTn, sh, 0, tid);
     for (int p = 0; p < NTn; ++p) {
-      const int par = p & 1;
+      const int par = p % Cfg<NM>::NBUF;
       const int tp = opaque_tid(tid), wp = tp >> 6;
       __syncthreads();
+      // the owner of panel p factored it during step p - 1 and left its other tile columns'
+      // update with panel p - 1 for now (before panel p's interchanges, as in the other columns)
+      if (Cfg<NM>::DEFER && p >= 1 && wp == p % W)
+        update_rest<NM>(T, p - 1, NTn, sh, (p - 1) % Cfg<NM>::NBUF, p / W, opaque_tid(tp));
       apply_swaps<NM>(T, p, NTn, sh, par, (wp == p % W) ? p / W : -1, tp);
       if (p + 1 < NTn && wp == (p + 1) % W) {
         const int inf = lookahead<NM>(T, p, NTn, sh, par, (p + 1) / W, opaque_tid(tp));
