@@ -68,6 +68,7 @@ struct Cfg {
 };
 
 typedef float float4v __attribute__((ext_vector_type(4)));
+typedef double dvec4u __attribute__((ext_vector_type(4), aligned(8)));
 
 __device__ __forceinline__ f4 mma32(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -98,6 +99,7 @@ struct Shm {
       float stg[W][32][16 * Cfg<NM>::NTC + 1];  // per-wavefront interchange staging: 32 row slots
     } f;
     float rbuf[2][W][W][256];  // redistribution rounds: [I - h W][J - jc W] 16 x 16 tiles, swizzled
+    float ld[W][4][16][20];    // loading A: four row-major FP32 tiles per wavefront
     struct {
       double x64[NM][RMAX];    // FP64 solution
       float cbuf[NM][16];      // right-hand side / solution blocks (columns >= nrhs are zero)
@@ -610,8 +612,14 @@ __device__ __forceinline__ void residual(const double* __restrict__ a, const dou
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int col0 = 16 * (c0 + u) + 4 * g;
+        if (row < n && col0 + 3 < n) {
+          // four consecutive doubles of the row in one 32-byte load (8-byte aligned)
+          const dvec4u v = *reinterpret_cast<const dvec4u*>(arow + col0);
+          av[u][0] = v[0]; av[u][1] = v[1]; av[u][2] = v[2]; av[u][3] = v[3];
+        } else {
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) av[u][kk] = (row < n && col0 + kk < n) ? arow[col0 + kk] : 0.0;
+          for (int kk = 0; kk < 4; ++kk) av[u][kk] = (row < n && col0 + kk < n) ? arow[col0 + kk] : 0.0;
+        }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -686,24 +694,49 @@ __global__ __launch_bounds__(Cfg<NM>::THREADS, Cfg<NM>::WAVES_PER_EU) void dsges
       sh.spec = 1;
     }
     // ---- A (FP64, HBM) -> FP32 tiles; identity beyond n ----
+    // Four tiles at a time: lane l reads row l / 4 of each, columns 4 (l % 4) .. + 3, in one
+    // 32-byte load (the scalar form at the matrix's right edge), converts to FP32 and writes the
+    // row-major tile to its wavefront's LDS area, from which the accumulator layout is read.
     f4 T[NT][NTC];
+    {
+      float(*ldt)[16][20] = sh.u.ld[wave];
+      const int lr = lane >> 2, lc = 4 * (lane & 3);
 #pragma unroll
-    for (int jc = 0; jc < NTC; ++jc) {
-      const int J = jc * W + wave;
+      for (int jc = 0; jc < NTC; ++jc) {
+        const int J = jc * W + wave;
 #pragma unroll
-      for (int I = 0; I < NT; ++I) {
-        if (I < NTn && J < NTn) {
+        for (int I0 = 0; I0 < NT; I0 += 4) {
+          double v[4][4];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = 16 * I + 4 * g + r, col = 16 * J + c16;
-            const double v = (row < n && col < n) ? a[(long)row * n + col] : (row == col ? 1.0 : 0.0);
-            T[I][jc][r] = (float)v;
+          for (int t = 0; t < 4; ++t) {
+            const int row = 16 * (I0 + t) + lr, col = 16 * J + lc;
+            if (row < n && col + 3 < n) {
+              const dvec4u x = *reinterpret_cast<const dvec4u*>(a + (long)row * n + col);
+              v[t][0] = x[0]; v[t][1] = x[1]; v[t][2] = x[2]; v[t][3] = x[3];
+            } else {
+#pragma unroll
+              for (int kk = 0; kk < 4; ++kk)
+                v[t][kk] = (row < n && col + kk < n) ? a[(long)row * n + col + kk] : (row == col + kk ? 1.0 : 0.0);
+            }
           }
-        } else {
-          T[I][jc] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            *reinterpret_cast<float4v*>(&ldt[t][lr][lc]) =
+                float4v{(float)v[t][0], (float)v[t][1], (float)v[t][2], (float)v[t][3]};
+          wave_sync();
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int I = I0 + t;
+            if (I < NTn && J < NTn) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) T[I][jc][r] = ldt[t][4 * g + r][c16];
+            } else {
+              T[I][jc] = f4{0.f, 0.f, 0.f, 0.f};
+            }
+          }
+          wave_sync();
+          __builtin_amdgcn_sched_barrier(0);
         }
-        // keep 4 tile rows (16 FP64 loads per lane) in flight at a time
-        if ((I & 3) == 3) __builtin_amdgcn_sched_barrier(0);
       }
     }
     __syncthreads();
