@@ -115,6 +115,8 @@ bzg_status igemm_digits_impl(bzg_context* ctx, bool ta, bool tb, int64_t m, int6
                              int32_t* C, int64_t ldc);
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+// four doubles that need only 8-byte alignment (one 32-byte global access, split as needed)
+typedef double dvec4u __attribute__((ext_vector_type(4), aligned(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));

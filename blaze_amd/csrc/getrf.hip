@@ -56,12 +56,22 @@ __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(double* __rest
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int m = n - d0;
   double* P = A + (long)d0 * lda + d0;
+  // a full-width row of 4 or 8 columns moves as 32-byte vectors (fewer load instructions: what
+  // limits one workgroup's share of the memory bandwidth), others element by element
   double a[R][WL];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int row = t + kLeafThreads * i;
+    if (WL % 4 == 0 && row < m && w == WL) {
 #pragma unroll
-    for (int c = 0; c < WL; ++c) a[i][c] = (row < m && c < w) ? P[(long)row * lda + c] : 0.0;
+      for (int c4 = 0; c4 < WL / 4; ++c4) {
+        const dvec4u v = *reinterpret_cast<const dvec4u*>(P + (long)row * lda + 4 * c4);
+        a[i][4 * c4] = v[0]; a[i][4 * c4 + 1] = v[1]; a[i][4 * c4 + 2] = v[2]; a[i][4 * c4 + 3] = v[3];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < WL; ++c) a[i][c] = (row < m && c < w) ? P[(long)row * lda + c] : 0.0;
+    }
   }
   int first_zero = 0;
 #pragma unroll
@@ -189,7 +199,12 @@ __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(double* __rest
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int row = t + kLeafThreads * i;
-    if (row < m) {
+    if (WL % 4 == 0 && row < m && w == WL) {
+#pragma unroll
+      for (int c4 = 0; c4 < WL / 4; ++c4)
+        *reinterpret_cast<dvec4u*>(P + (long)row * lda + 4 * c4) =
+            dvec4u{a[i][4 * c4], a[i][4 * c4 + 1], a[i][4 * c4 + 2], a[i][4 * c4 + 3]};
+    } else if (row < m) {
 #pragma unroll
       for (int c = 0; c < WL; ++c)
         if (c < w) P[(long)row * lda + c] = a[i][c];

@@ -68,7 +68,6 @@ struct Cfg {
 };
 
 typedef float float4v __attribute__((ext_vector_type(4)));
-typedef double dvec4u __attribute__((ext_vector_type(4), aligned(8)));
 
 __device__ __forceinline__ f4 mma32(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
