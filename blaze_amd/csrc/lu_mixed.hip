@@ -102,8 +102,8 @@ struct Shm {
     struct {
       double x64[NM][RMAX];    // FP64 solution
       float cbuf[NM][16];      // right-hand side / solution blocks (columns >= nrhs are zero)
-      float dscr[W][16][20];   // diagonal block being inverted (16-byte aligned rows)
-      float dinv[W][16][33];   // its L^-1 (columns 0..15) and U^-1 (columns 16..31)
+      float dscr[W][Cfg<NM>::NTC][16][20];  // diagonal blocks being inverted (16-byte aligned rows)
+      float dinv[W][Cfg<NM>::NTC][16][33];  // their L^-1 (columns 0..15) and U^-1 (columns 16..31)
     } s;
   } u;
   // interchange tables of panel p, by p % NBUF
@@ -816,42 +816,50 @@ __global__ __launch_bounds__(Cfg<NM>::THREADS, Cfg<NM>::WAVES_PER_EU) void dsges
       receive(1, 1, 0);
       __syncthreads();
       // ---- inverses of the diagonal blocks (L unit lower, U upper) of the own row blocks ----
+      // Both row blocks at once: lanes 0..15 / 16..31 form column c of L^-1 / U^-1 of the first
+      // block, lanes 32..47 / 48..63 those of the second; then back in the A-operand layout.
       f4 Dinv[NTC][2];
+      {
+        float(*ds)[16][20] = sh.u.s.dscr[wave];
+        float(*di)[16][33] = sh.u.s.dinv[wave];
 #pragma unroll
-      for (int ir = 0; ir < NTC; ++ir) {
-        const int I = ir * W + wave;
-        f4 d = T[0][ir];
+        for (int ir = 0; ir < NTC; ++ir) {
+          const int I = ir * W + wave;
+          f4 d = T[0][ir];
 #pragma unroll
-        for (int J = 1; J < NT; ++J)
-          if (J == I) d = T[J][ir];
-        float(*ds)[20] = sh.u.s.dscr[wave];
-        float(*di)[33] = sh.u.s.dinv[wave];
-        if (I < NTn) {
+          for (int J = 1; J < NT; ++J)
+            if (J == I) d = T[J][ir];
+          if (I < NTn) {
 #pragma unroll
-          for (int kk = 0; kk < 4; ++kk) ds[c16][4 * g + kk] = d[kk];
-        }
-        wave_sync();
-        // lanes 0..15: column c of L^-1; lanes 16..31: column c of U^-1; then both to di (the
-        // U inverse in the second block of the scratch) and back in the A-operand layout
-        if (I < NTn && lane < 32) {
-          float x[16];
-          if (lane < 16) unit_lower_inverse_column(&ds[0][0], 20, lane, x);
-          else upper_inverse_column(&ds[0][0], 20, lane - 16, x);
-#pragma unroll
-          for (int i = 0; i < 16; ++i) di[i][lane] = x[i];
-        }
-        wave_sync();
-        f4 li = {0.f, 0.f, 0.f, 0.f}, ui = {0.f, 0.f, 0.f, 0.f};
-        if (I < NTn) {
-#pragma unroll
-          for (int kk = 0; kk < 4; ++kk) {
-            li[kk] = di[c16][4 * g + kk];
-            ui[kk] = di[c16][16 + 4 * g + kk];
+            for (int kk = 0; kk < 4; ++kk) ds[ir][c16][4 * g + kk] = d[kk];
           }
         }
         wave_sync();
-        Dinv[ir][0] = li;
-        Dinv[ir][1] = ui;
+        {
+          const int ir = lane >> 5, h = lane & 31;
+          if (ir * W + wave < NTn) {
+            float x[16];
+            if (h < 16) unit_lower_inverse_column(&ds[ir][0][0], 20, h, x);
+            else upper_inverse_column(&ds[ir][0][0], 20, h - 16, x);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) di[ir][i][h] = x[i];
+          }
+        }
+        wave_sync();
+#pragma unroll
+        for (int ir = 0; ir < NTC; ++ir) {
+          f4 li = {0.f, 0.f, 0.f, 0.f}, ui = {0.f, 0.f, 0.f, 0.f};
+          if (ir * W + wave < NTn) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+              li[kk] = di[ir][c16][4 * g + kk];
+              ui[kk] = di[ir][c16][16 + 4 * g + kk];
+            }
+          }
+          Dinv[ir][0] = li;
+          Dinv[ir][1] = ui;
+        }
+        wave_sync();
       }
       // ---- x = FP32 solve of P b ----
       for (int i = tid; i < NM; i += Cfg<NM>::THREADS) sh.pinv[sh.perm[i]] = i;
