@@ -127,7 +127,8 @@ extern "C" __device__ __attribute__((const)) unsigned __ockl_wfred_min_u32(unsig
 // Column c of the inverse of a 16 x 16 unit lower triangular L (row-major, leading dimension
 // ldl, a multiple of 4, 16-byte aligned rows; the diagonal and upper part are not read), by
 // forward substitution.  The rows are read four values per load, independently of the
-// recurrence, which runs in registers; returns the column in x.
+// recurrence, which runs in registers; returns the column in x.  No scheduling barriers: row
+// i + 1's sum can start while row i's finishes (it needs x[i] only at its last step).
 __device__ __forceinline__ void unit_lower_inverse_column(const float* L, int ldl, int c, float (&x)[16]) {
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
@@ -143,7 +144,6 @@ __device__ __forceinline__ void unit_lower_inverse_column(const float* L, int ld
 #pragma unroll
     for (int k = 0; k < i; ++k) v -= row[k] * x[k];
     x[i] = v;
-    if ((i & 3) == 3) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -164,7 +164,6 @@ __device__ __forceinline__ void upper_inverse_column(const float* U, int ldu, in
 #pragma unroll
     for (int k = i + 1; k < 16; ++k) v -= row[k] * x[k];
     x[i] = v / row[i];
-    if ((i & 3) == 0) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -339,17 +338,18 @@ __device__ __forceinline__ int factor_panel(f4 (&T)[Cfg<NM>::NT][Cfg<NM>::NTC], 
     }
   }
   wave_sync();
-#pragma unroll
-  for (int I = 0; I < NT; ++I) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) T[I][JC][r] = pb[16 * I + 4 * g + r][c16];
-  }
-  // L11^-1 (unit lower): lane c < 16 solves for column c, in place in linv
+  // L11^-1 (unit lower): lane c < 16 solves for column c, in place in linv (before the tiles
+  // are reloaded, while the panel's registers are free)
   if (lane < 16) {
     float x[16];
     unit_lower_inverse_column(&pb[j0][0], 20, lane, x);
 #pragma unroll
     for (int i = 0; i < 16; ++i) sh.u.f.linv[par][i][lane] = x[i];
+  }
+#pragma unroll
+  for (int I = 0; I < NT; ++I) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) T[I][JC][r] = pb[16 * I + 4 * g + r][c16];
   }
   // no interchanges: the tables keep the last searched panel with this index (cleared when the
   // next one is built) and apply_swaps skips the panel
